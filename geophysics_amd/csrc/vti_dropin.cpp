@@ -1,0 +1,199 @@
+// vti_dropin.cpp — the reference's three extern "C" entry points with
+// their exact signatures and file semantics, computed on the GPU through
+// the vti_* context API.
+//
+//   GPU_vti2dfd  GPU_vti2dfd_kernels.cu:301-484
+//   GPU_vti3dfd  GPU_vti3dfd_kernels.cu:371-573
+//   CPU_vti2dfd  CPU_vti2dfd_kernels.c:366-536 (the CPU driver's semantics)
+//
+// Differences from the reference, all on error paths or defects: errors
+// set vti_dropin_status() instead of exit(0) / continuing with NULL files;
+// ISO models are padded completely (CPU_vti2dfd_kernels.c:244 stops
+// padding at the first zero); the 3D y tables are sized nny, not nnx
+// (GPU_vti3dfd_kernels.cu:471,474).
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "vti_internal.h"
+
+namespace {
+
+std::atomic<int> g_status{VTI_OK};
+
+int read_model(const char *fn, std::vector<float> &dst, size_t n)
+{
+    FILE *fp = std::fopen(fn, "rb");
+    if (!fp) {
+        std::printf("error open <%s>!\n", fn);
+        vti::set_error(std::string("cannot open ") + fn);
+        return VTI_ERR_IO;
+    }
+    dst.resize(n);
+    const size_t got = std::fread(dst.data(), sizeof(float), n, fp);
+    std::fclose(fp);
+    if (got != n) {
+        vti::set_error(std::string("short read from ") + fn);
+        return VTI_ERR_IO;
+    }
+    return VTI_OK;
+}
+
+struct Ctx {
+    vti_ctx *c = nullptr;
+    ~Ctx() { vti_destroy(c); }
+};
+
+// 2D driver shared by GPU_vti2dfd (mode GPU) and CPU_vti2dfd (mode CPU).
+int run2d(int mode, int nx, int nz, int dx, int dz, int npd, int SV, const char *FNv,
+          const char *FNe, const char *FNd, int favg, int ns, int fs, int ds, int zs,
+          const char *FNshot, const char *FNsnap, int nt, int dt)
+{
+    vti_cfg cfg;
+    int st = vti_cfg_ref2d(&cfg, mode, nx, nz, dx, dz, npd, SV, favg, nt, dt);
+    if (st) return st;
+    const size_t n = (size_t)nx * nz;
+    std::vector<float> v, e, d;
+    if ((st = read_model(FNv, v, n)) || (st = read_model(FNe, e, n)) || (st = read_model(FNd, d, n)))
+        return st;
+    Ctx x;
+    if ((st = vti_create(&cfg, &x.c))) return st;
+    if ((st = vti_set_model(x.c, v.data(), e.data(), d.data(), 0))) return st;
+    FILE *fshot = std::fopen(FNshot, "wb");
+    FILE *fsnap = std::fopen(FNsnap, "wb");
+    if (!fshot || !fsnap) {
+        if (fshot) std::fclose(fshot);
+        if (fsnap) std::fclose(fsnap);
+        vti::set_error("cannot open output files");
+        return VTI_ERR_IO;
+    }
+    int nnx, nny, nnz;
+    vti_padded_dims(x.c, &nnx, &nny, &nnz);
+    std::vector<float> frame((size_t)nnx * nnz), shot((size_t)nx * nt);
+    for (int is = 1; is <= ns && st == VTI_OK; ++is) {
+        vti_shot sh;
+        if ((st = vti_shot_ref2d(&sh, &cfg, is, fs, ds, zs, 1))) break;
+        if ((st = vti_set_shot(x.c, &sh)) || (st = vti_reset(x.c))) break;
+        // snapshots of P every 50 steps for shot 1 (GPU_vti2dfd_kernels.cu:432-437,
+        // CPU_vti2dfd_kernels.c:502-506)
+        int it = 0;
+        while (it < nt && st == VTI_OK) {
+            int last = is == 1 ? (it + 49) / 50 * 50 : nt - 1;   // next snapshot step >= it
+            if (last > nt - 1) last = nt - 1;
+            if ((st = vti_step(x.c, it, last - it + 1))) break;
+            it = last + 1;
+            if (is == 1 && last % 50 == 0) {
+                if ((st = vti_get_field(x.c, VTI_F_P, frame.data()))) break;
+                std::fseek(fsnap, (long)(last / 50) * nnx * nnz * 4L, SEEK_SET);
+                std::fwrite(frame.data(), 4, frame.size(), fsnap);
+            }
+        }
+        if (st) break;
+        if (mode == VTI_MODE_GPU_REF && (st = vti_mute(x.c, fs, ds, zs, is, 100))) break;   // :439
+        if ((st = vti_get_record(x.c, shot.data()))) break;
+        std::fseek(fshot, (long)(is - 1) * nt * nx * (long)sizeof(float), SEEK_SET);
+        std::fwrite(shot.data(), sizeof(float), shot.size(), fshot);
+    }
+    std::fclose(fshot);
+    std::fclose(fsnap);
+    return st;
+}
+
+int run3d(int nx, int ny, int nz, int dx, int dy, int dz, int npml, int SV, const char *FNv,
+          const char *FNe, const char *FNd, int favg, int ns, int fsx, int dsx, int fsy, int dsy,
+          int fsz, int dsz, const char *FNshot, const char *FNsnap, int nt, int dt)
+{
+    vti_cfg cfg;
+    int st = vti_cfg_ref3d(&cfg, nx, ny, nz, dx, dy, dz, npml, SV, favg, nt, dt);
+    if (st) return st;
+    const size_t n = (size_t)nx * ny * nz;
+    std::vector<float> v, e, d;
+    if ((st = read_model(FNv, v, n)) || (st = read_model(FNe, e, n)) || (st = read_model(FNd, d, n)))
+        return st;
+    Ctx x;
+    if ((st = vti_create(&cfg, &x.c))) return st;
+    if ((st = vti_set_model(x.c, v.data(), e.data(), d.data(), 0))) return st;
+    FILE *fshot = std::fopen(FNshot, "wb");
+    FILE *fsnap = std::fopen(FNsnap, "wb");
+    if (!fshot || !fsnap) {
+        if (fshot) std::fclose(fshot);
+        if (fsnap) std::fclose(fsnap);
+        vti::set_error("cannot open output files");
+        return VTI_ERR_IO;
+    }
+    int nnx, nny, nnz;
+    vti_padded_dims(x.c, &nnx, &nny, &nnz);
+    std::vector<float> shot((size_t)nx * ny * nt), full, win;
+    for (int is = 0; is < ns && st == VTI_OK; ++is) {
+        vti_shot sh;
+        if ((st = vti_shot_ref3d(&sh, &cfg, is, fsx, dsx, fsy, dsy, fsz, dsz, 1, 1))) break;
+        if ((st = vti_set_shot(x.c, &sh)) || (st = vti_reset(x.c))) break;
+        // windowed snapshots every 300 steps of shot 0 (GPU_vti3dfd_kernels.cu:526-532)
+        int it = 0;
+        while (it < nt && st == VTI_OK) {
+            int last = nt - 1;
+            if (is == 0) last = std::max(300, (it + 299) / 300 * 300);   // next snapshot step
+            if (last > nt - 1) last = nt - 1;
+            if ((st = vti_step(x.c, it, last - it + 1))) break;
+            it = last + 1;
+            if (is == 0 && last != 0 && last % 300 == 0) {
+                full.resize((size_t)nnx * nny * nnz);
+                win.resize(n);
+                if ((st = vti_get_field(x.c, VTI_F_P, full.data()))) break;
+                for (int iy = 0; iy < ny; ++iy)          // window3d :329-340
+                    for (int ix = 0; ix < nx; ++ix)
+                        for (int iz = 0; iz < nz; ++iz)
+                            win[iz + (size_t)nz * ix + (size_t)nz * nx * iy] =
+                                full[(iz + npml) + (size_t)nnz * (ix + npml) + (size_t)nnz * nnx * (iy + npml)];
+                std::fseek(fsnap, (long)(last / 300 - 1) * (long)n * 4L, SEEK_SET);
+                std::fwrite(win.data(), 4, n, fsnap);
+            }
+        }
+        if (st) break;
+        if ((st = vti_mute3d(x.c, fsx, dsx, fsy, dsy, fsz, is, 100))) break;   // :534
+        if ((st = vti_get_record(x.c, shot.data()))) break;
+        std::fseek(fshot, (long)is * nt * nx * ny * (long)sizeof(float), SEEK_SET);
+        std::fwrite(shot.data(), sizeof(float), shot.size(), fshot);
+    }
+    std::fclose(fshot);
+    std::fclose(fsnap);
+    return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+void GPU_vti2dfd(int nx, int nz, int dx, int dz, int npd, int SV, const char FNv[],
+                 const char FNe[], const char FNd[], int favg, int ns, int fs, int ds, int zs,
+                 const char FNshot[], const char FNsnap[], int nt, int dt, int run_count)
+{
+    (void)run_count;
+    g_status = run2d(VTI_MODE_GPU_REF, nx, nz, dx, dz, npd, SV, FNv, FNe, FNd, favg, ns, fs, ds, zs,
+                     FNshot, FNsnap, nt, dt);
+}
+
+void CPU_vti2dfd(int nx, int nz, int dx, int dz, int npd, int SV, const char FNv[],
+                 const char FNe[], const char FNd[], int favg, int ns, int fs, int ds, int zs,
+                 const char FNshot[], const char FNsnap[], int nt, int dt, int run_count)
+{
+    (void)run_count;
+    g_status = run2d(VTI_MODE_CPU_REF, nx, nz, dx, dz, npd, SV, FNv, FNe, FNd, favg, ns, fs, ds, zs,
+                     FNshot, FNsnap, nt, dt);
+}
+
+void GPU_vti3dfd(int nx, int ny, int nz, int dx, int dy, int dz, int npml, int SV,
+                 const char FNv[], const char FNe[], const char FNd[], int favg, int ns, int fsx,
+                 int dsx, int fsy, int dsy, int fsz, int dsz, const char FNshot[],
+                 const char FNsnap[], int nt, int dt, int run_count)
+{
+    (void)run_count;
+    g_status = run3d(nx, ny, nz, dx, dy, dz, npml, SV, FNv, FNe, FNd, favg, ns, fsx, dsx, fsy, dsy,
+                     fsz, dsz, FNshot, FNsnap, nt, dt);
+}
+
+int vti_dropin_status(void) { return g_status.load(); }
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+// vti_internal.h — structures shared by the host side (vti_api.hip,
+// vti_tables.cpp, vti_dropin.cpp) and the gfx950 kernels (vti_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/vti.h"
+
+namespace vti {
+
+// #define pi 3.141592653 (GPU_vti2dfd_kernels.cu:31)
+constexpr double kPi = 3.141592653;
+
+// Padded grid as laid out in HBM: z fastest with a pitch ldz >= nnz that
+// keeps every (x,y) row 128-byte aligned, then x, then y.
+struct Grid {
+    int nnx, nny, nnz;
+    int ldz;
+    long long ldxy;   // ldz * nnx
+};
+
+// Velocity update: update_vel2d GPU_vti2dfd_kernels.cu:209-238,
+// update_vel3d GPU_vti3dfd_kernels.cu:90-129.
+struct VelArgs {
+    Grid g;
+    float *u, *v, *w;
+    const float *P, *Q;
+    const float *cx1, *cx2, *cy1, *cy2, *cz1, *cz2;
+    float dtx, dty, dtz;
+    long long src;   // cell that receives the point source before the update (-1: none)
+    float amp;       // pfac * wavelet(t) of this step
+};
+
+// Stress update: update_stress2d GPU_vti2dfd_kernels.cu:240-299,
+// update_stress3d GPU_vti3dfd_kernels.cu:131-201, with the receiver gather
+// (shot_record2d :141-149) and the CPU driver's in-stress source
+// (CPU_vti2dfd_kernels.c:173-174) fused in.
+struct StressArgs {
+    Grid g;
+    const float *u, *v, *w;
+    float *px, *py, *pz, *qx, *qy, *qz, *P, *Q;
+    const float *vp, *ep, *de;
+    const float *ax1, *ax2, *ay1, *ay2, *az1, *az2;
+    float dtx, dty, dtz;
+    // SV taper: factor tap[r2] for integer r2 <= svR*svR around (cx,cy,cz)
+    int sv, svR, cx, cy, cz;
+    const double *tap;
+    // in-stress 7x7 Gaussian footprint centred on (gx, gy, gz); gw[(dx+3)*7+(dz+3)]
+    int gsrc, gx, gy, gz;
+    float gamp;
+    const double *gw;
+    // receivers: row rz, x = rx0 + k*rdx (k < rnx), y = ry0 + j*rdy (j < rny)
+    float *rec;      // row of this step, nrec floats (nullptr: no record)
+    int rec_kind, rz, rx0, rdx, rnx, ry0, rdy, rny;
+};
+
+// ---- kernel launchers (vti_kernels.hip) ---------------------------------
+hipError_t launch_vel(int ndim, const VelArgs &a, hipStream_t s);
+hipError_t launch_stress(int ndim, const StressArgs &a, hipStream_t s);
+hipError_t launch_pad_model(const Grid &g, int ndim, int nx, int ny, int nz,
+                            int npml, const float *src, float *dst, hipStream_t s);
+hipError_t launch_zero(float *p, size_t n, hipStream_t s);
+hipError_t launch_transpose(const float *in, float *out, int rows, int cols,
+                            hipStream_t s);
+hipError_t launch_mute2d(float *rec_t, int nrec, int nt, float dt, float favg,
+                         float dx, float dz, int fs, int ds, int zs, int is,
+                         float vp1, float eps1, int tt, hipStream_t s);
+hipError_t launch_mute3d(float *rec_t, int nx, int ny, int nt, float dt,
+                         float favg, float dx, float dy, float dz, int fsx,
+                         int dsx, int fsy, int dsy, int fsz, int is,
+                         float vp1, float eps1, int tt, hipStream_t s);
+hipError_t launch_copy(void *dst, const void *src, size_t nbytes, hipStream_t s);
+
+// ---- host table construction (vti_tables.cpp) ---------------------------
+void axis_tables(int nn_int, int npml, float dtd0, int cpu_form,
+                 float *c1, float *c2, float *a1, float *a2);
+float wavelet(float ts, float favg, int wtype);
+void source_gains(const vti_cfg &c, std::vector<float> &amp);
+void taper_table(int cpu_form, int R, int r, std::vector<double> &tap);
+void gaussian_weights(double w[49]);
+
+void set_error(const std::string &msg);
+
+}  // namespace vti

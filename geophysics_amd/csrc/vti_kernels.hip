@@ -1,0 +1,428 @@
+// vti_kernels.hip — gfx950 kernels of the staggered-grid VTI FD path.
+//
+// Arithmetic follows the reference expressions term by term and in the
+// same left-to-right order (this file is compiled with -ffp-contract=off),
+// so a step is bit-for-bit the reference's float arithmetic:
+//   u = coffx2*u - coffx1*dtx*sum_m c_m (P[x+m+1]-P[x-m])
+//   px = acoffx2*px - acoffx1*vp*vp*(1+2e)*dtx*sum_m c_m (u[x+m]-u[x-m-1])
+// (GPU_vti2dfd_kernels.cu:229-235, :285-296; GPU_vti3dfd_kernels.cu:114-122,
+//  :182-197).
+//
+// Work layout.  Every volume is z-fastest with a 128-byte-aligned row pitch
+// (Grid::ldz).  A workgroup is 64 (z) x 4 (x) threads: one wavefront covers
+// 64 consecutive z cells of one x row, so every load and store of a row is a
+// single 256-byte coalesced access.  In 3D each thread owns one (z,x)
+// column and marches the slowest axis y, keeping the 8-deep y-stencil of P
+// (velocity kernel) or v (stress kernel) in a register queue, so each value
+// of those fields is fetched once per kernel; x- and z-neighbours are a few
+// KB away and are served by L1/L2.  Workgroup ids are remapped so that each
+// XCD (workgroups are dealt round-robin over the 8 XCDs) receives a
+// contiguous slab of x-rows and halo rows stay inside one L2.
+#include <hip/hip_runtime.h>
+
+#include "vti_internal.h"
+
+namespace vti {
+namespace {
+
+// c[4] = {1.196289,-0.0797526,0.009570313,-0.0006975447}
+// (GPU_vti2dfd_kernels.cu:37) — the float values of those decimals.
+__device__ __forceinline__ float cst(int m)
+{
+    return m == 0 ? 1.196289f : (m == 1 ? -0.0797526f : (m == 2 ? 0.009570313f : -0.0006975447f));
+}
+
+constexpr int TZ = 64;    // z cells per workgroup (one wavefront)
+constexpr int TX = 4;     // x rows per workgroup
+constexpr int NXCD = 8;   // MI355X: 8 XCDs, workgroups dealt round-robin
+
+// Map the hardware workgroup id to a logical tile so that XCD k owns the
+// contiguous logical range [k*per, (k+1)*per).
+__device__ __forceinline__ int xcd_tile(int b, int ntile_padded)
+{
+    const int per = ntile_padded / NXCD;
+    return (b % NXCD) * per + (b / NXCD);
+}
+
+// ---------------------------------------------------------------- 2D ----
+template <bool INJ>
+__global__ __launch_bounds__(256) void vel2d_kernel(VelArgs a)
+{
+    const Grid g = a.g;
+    const int iz = blockIdx.x * TZ + threadIdx.x;
+    const int ix = blockIdx.y * TX + threadIdx.y;
+    if (iz < 4 || iz >= g.nnz - 4 || ix < 4 || ix >= g.nnx - 4) return;
+    const long long id = iz + (long long)ix * g.ldz;
+    float xx = 0.0f, zz = 0.0f;
+#pragma unroll
+    for (int im = 0; im < 4; ++im) {
+        const long long i1 = id + (long long)(im + 1) * g.ldz, i0 = id - (long long)im * g.ldz;
+        const long long j1 = id + im + 1, j0 = id - im;
+        float p1 = a.P[i1], p0 = a.P[i0], q1 = a.Q[j1], q0 = a.Q[j0];
+        if (INJ) {   // P[src] += amp, Q[src] += amp before the update (add_source2d :204-205)
+            if (i1 == a.src) p1 = p1 + a.amp;
+            if (i0 == a.src) p0 = p0 + a.amp;
+            if (j1 == a.src) q1 = q1 + a.amp;
+            if (j0 == a.src) q0 = q0 + a.amp;
+        }
+        xx += cst(im) * (p1 - p0);
+        zz += cst(im) * (q1 - q0);
+    }
+    a.u[id] = a.cx2[ix] * a.u[id] - a.cx1[ix] * a.dtx * xx;
+    a.w[id] = a.cz2[iz] * a.w[id] - a.cz1[iz] * a.dtz * zz;
+}
+
+// Effective eps / delta after the SV taper (GPU_vti2dfd_kernels.cu:260-278):
+// the factor depends only on the integer r^2, looked up in a host table.
+__device__ __forceinline__ void taper(const StressArgs &a, int r2, float &ee, float &dd)
+{
+    if (r2 <= a.svR * a.svR) {
+        const double f = a.tap[r2];
+        ee = (float)(f * (double)ee);
+        dd = (float)(f * (double)dd);
+    }
+}
+
+__global__ __launch_bounds__(256) void stress2d_kernel(StressArgs a)
+{
+    const Grid g = a.g;
+    const int iz = blockIdx.x * TZ + threadIdx.x;
+    const int ix = blockIdx.y * TX + threadIdx.y;
+    if (iz < 4 || iz >= g.nnz - 4 || ix < 4 || ix >= g.nnx - 4) return;
+    const long long id = iz + (long long)ix * g.ldz;
+    float xx = 0.0f, zz = 0.0f;
+#pragma unroll
+    for (int im = 0; im < 4; ++im) {
+        xx += cst(im) * (a.u[id + (long long)im * g.ldz] - a.u[id - (long long)(im + 1) * g.ldz]);
+        zz += cst(im) * (a.w[id + im] - a.w[id - im - 1]);
+    }
+    const float vv = a.vp[id];
+    float ee = a.ep[id], dd = a.de[id];
+    if (a.sv) {
+        const int rx = ix - a.cx, rz = iz - a.cz;
+        taper(a, rx * rx + rz * rz, ee, dd);
+    }
+    const float e2 = 1.0f + 2.0f * ee;
+    const float sd = sqrtf(1.0f + 2.0f * dd);
+    const float px = a.ax2[ix] * a.px[id] - a.ax1[ix] * vv * vv * e2 * a.dtx * xx;
+    const float pz = a.az2[iz] * a.pz[id] - a.az1[iz] * vv * vv * sd * a.dtz * zz;
+    const float qx = a.ax2[ix] * a.qx[id] - a.ax1[ix] * vv * vv * sd * a.dtx * xx;
+    const float qz = a.az2[iz] * a.qz[id] - a.az1[iz] * vv * vv * a.dtz * zz;
+    a.px[id] = px; a.pz[id] = pz; a.qx[id] = qx; a.qz[id] = qz;
+    float Pn = px + pz, Qn = qx + qz;
+    if (a.gsrc) {   // P = px+pz+s (CPU_vti2dfd_kernels.c:173-174), 7x7 footprint
+        const int dx = ix - a.gx, dz = iz - a.gz;
+        if (dx >= -3 && dx <= 3 && dz >= -3 && dz <= 3) {
+            const float s = (float)((double)a.gamp * a.gw[(dx + 3) * 7 + (dz + 3)]);
+            Pn = px + pz + s;
+            Qn = qx + qz + s;
+        }
+    }
+    a.P[id] = Pn;
+    a.Q[id] = Qn;
+    if (a.rec && iz == a.rz) {
+        const int k = ix - a.rx0;
+        if (k >= 0 && k % a.rdx == 0 && k / a.rdx < a.rnx) {
+            const float r = a.rec_kind == 0 ? Pn : (a.rec_kind == 1 ? Pn + Pn : Pn + Qn);
+            a.rec[k / a.rdx] = r;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- 3D ----
+template <bool INJ>
+__global__ __launch_bounds__(256) void vel3d_kernel(VelArgs a, int nzt, int ntile, int ntile_p)
+{
+    const Grid g = a.g;
+    const int t = xcd_tile(blockIdx.x, ntile_p);
+    if (t >= ntile) return;
+    const int iz = (t % nzt) * TZ + threadIdx.x;
+    const int ix = (t / nzt) * TX + threadIdx.y;
+    if (iz < 4 || iz >= g.nnz - 4 || ix < 4 || ix >= g.nnx - 4) return;
+    const long long col = iz + (long long)ix * g.ldz;
+    const long long L = g.ldxy;
+    auto ld = [&](const float *f, long long i) -> float {
+        float v = f[i];
+        if (INJ && i == a.src) v = v + a.amp;   // add_source3d GPU_vti3dfd_kernels.cu:85-86
+        return v;
+    };
+    // q[k] = P at y-3+k for the current y
+    float q[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) q[k] = ld(a.P, col + (long long)(1 + k) * L);
+    const int yend = g.nny - 4;
+    for (int iy = 4; iy < yend; ++iy) {
+        const long long id = col + (long long)iy * L;
+        float xx = 0.0f, yy = 0.0f, zz = 0.0f;
+#pragma unroll
+        for (int im = 0; im < 4; ++im) {
+            yy += cst(im) * (q[3 + im + 1] - q[3 - im]);
+            const float p1 = ld(a.P, id + (long long)(im + 1) * g.ldz);
+            const float p0 = im == 0 ? q[3] : ld(a.P, id - (long long)im * g.ldz);
+            xx += cst(im) * (p1 - p0);
+            zz += cst(im) * (ld(a.Q, id + im + 1) - ld(a.Q, id - im));
+        }
+        a.u[id] = a.cx2[ix] * a.u[id] - a.cx1[ix] * a.dtx * xx;
+        a.v[id] = a.cy2[iy] * a.v[id] - a.cy1[iy] * a.dty * yy;
+        a.w[id] = a.cz2[iz] * a.w[id] - a.cz1[iz] * a.dtz * zz;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) q[k] = q[k + 1];
+        if (iy + 1 < yend) q[7] = ld(a.P, id + 5 * L);
+    }
+}
+
+__global__ __launch_bounds__(256) void stress3d_kernel(StressArgs a, int nzt, int ntile, int ntile_p)
+{
+    const Grid g = a.g;
+    const int t = xcd_tile(blockIdx.x, ntile_p);
+    if (t >= ntile) return;
+    const int iz = (t % nzt) * TZ + threadIdx.x;
+    const int ix = (t / nzt) * TX + threadIdx.y;
+    if (iz < 4 || iz >= g.nnz - 4 || ix < 4 || ix >= g.nnx - 4) return;
+    const long long col = iz + (long long)ix * g.ldz;
+    const long long L = g.ldxy;
+    // per-column flags for the rare fused work
+    const bool sv_col = a.sv && abs(ix - a.cx) <= a.svR && abs(iz - a.cz) <= a.svR;
+    const bool src_col = a.gsrc && abs(ix - a.gx) <= 3 && abs(iz - a.gz) <= 3;
+    int rk = -1;
+    if (a.rec && iz == a.rz) {
+        const int k = ix - a.rx0;
+        if (k >= 0 && k % a.rdx == 0 && k / a.rdx < a.rnx) rk = k / a.rdx;
+    }
+    const float ax1 = a.ax1[ix], ax2 = a.ax2[ix], az1 = a.az1[iz], az2 = a.az2[iz];
+    // vq[k] = v at y-4+k for the current y
+    float vq[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) vq[k] = a.v[col + (long long)k * L];
+    const int yend = g.nny - 4;
+    for (int iy = 4; iy < yend; ++iy) {
+        const long long id = col + (long long)iy * L;
+        float xx = 0.0f, yy = 0.0f, zz = 0.0f;
+#pragma unroll
+        for (int im = 0; im < 4; ++im) {
+            yy += cst(im) * (vq[4 + im] - vq[3 - im]);
+            xx += cst(im) * (a.u[id + (long long)im * g.ldz] - a.u[id - (long long)(im + 1) * g.ldz]);
+            zz += cst(im) * (a.w[id + im] - a.w[id - im - 1]);
+        }
+        const float vv = a.vp[id];
+        float ee = a.ep[id], dd = a.de[id];
+        if (sv_col) {
+            const int rx = ix - a.cx, ry = iy - a.cy, rz = iz - a.cz;
+            if (ry <= a.svR && ry >= -a.svR) taper(a, rx * rx + ry * ry + rz * rz, ee, dd);
+        }
+        const float e2 = 1.0f + 2.0f * ee;
+        const float sd = sqrtf(1.0f + 2.0f * dd);
+        const float ay1 = a.ay1[iy], ay2 = a.ay2[iy];
+        const float px = ax2 * a.px[id] - ax1 * vv * vv * e2 * a.dtx * xx;
+        const float py = ay2 * a.py[id] - ay1 * vv * vv * e2 * a.dty * yy;
+        const float pz = az2 * a.pz[id] - az1 * vv * vv * sd * a.dtz * zz;
+        const float qx = ax2 * a.qx[id] - ax1 * vv * vv * sd * a.dtx * xx;
+        const float qy = ay2 * a.qy[id] - ay1 * vv * vv * sd * a.dty * yy;
+        const float qz = az2 * a.qz[id] - az1 * vv * vv * a.dtz * zz;
+        a.px[id] = px; a.py[id] = py; a.pz[id] = pz;
+        a.qx[id] = qx; a.qy[id] = qy; a.qz[id] = qz;
+        float Pn = px + py + pz, Qn = qx + qy + qz;
+        if (src_col && iy == a.gy) {
+            const float s = (float)((double)a.gamp * a.gw[(ix - a.gx + 3) * 7 + (iz - a.gz + 3)]);
+            Pn = px + py + pz + s;
+            Qn = qx + qy + qz + s;
+        }
+        a.P[id] = Pn;
+        a.Q[id] = Qn;
+        if (rk >= 0) {
+            const int j = iy - a.ry0;
+            if (j >= 0 && j % a.rdy == 0 && j / a.rdy < a.rny) {
+                const float r = a.rec_kind == 0 ? Pn : (a.rec_kind == 1 ? Pn + Pn : Pn + Qn);
+                a.rec[rk + (long long)a.rnx * (j / a.rdy)] = r;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 7; ++k) vq[k] = vq[k + 1];
+        if (iy + 1 < yend) vq[7] = a.v[id + 4 * L];
+    }
+}
+
+// ------------------------------------------------------------ helpers ----
+// Edge-replicating pad of an unpadded (ny, nx, nz) model into the padded
+// pitched volume: equal to pad_vv2d / pad_vv3d (GPU_vti2dfd_kernels.cu:49-74,
+// GPU_vti3dfd_kernels.cu:208-255), which copy x then y then z borders.
+__global__ void pad_kernel(Grid g, int ndim, int nx, int ny, int nz, int np,
+                           const float *__restrict__ src, float *__restrict__ dst)
+{
+    const int iz = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ix = blockIdx.y;
+    const int iy = blockIdx.z;
+    if (iz >= g.ldz) return;
+    const long long o = iz + (long long)ix * g.ldz + (long long)iy * g.ldxy;
+    if (iz >= g.nnz) { dst[o] = 0.0f; return; }
+    const int jx = ix < np ? 0 : (ix >= np + nx ? nx - 1 : ix - np);
+    const int jz = iz < np ? 0 : (iz >= np + nz ? nz - 1 : iz - np);
+    int jy = 0;
+    if (ndim == 3) jy = iy < np ? 0 : (iy >= np + ny ? ny - 1 : iy - np);
+    dst[o] = src[jz + (long long)jx * nz + (long long)jy * nz * nx];
+}
+
+__global__ void zero_kernel(float4 *p, size_t n4)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ void zero_tail_kernel(float *p, size_t n)
+{
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+// out[c][r] = in[r][c] through a 32x33 LDS tile.
+__global__ void transpose_kernel(const float *__restrict__ in, float *__restrict__ out, int rows, int cols)
+{
+    __shared__ float tile[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+        const int r = r0 + k, c = c0 + threadIdx.x;
+        if (r < rows && c < cols) tile[k][threadIdx.x] = in[(long long)r * cols + c];
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+        const int c = c0 + k, r = r0 + threadIdx.x;
+        if (r < rows && c < cols) out[(long long)c * rows + r] = tile[threadIdx.x][k];
+    }
+}
+
+// mute_directwave2d GPU_vti2dfd_kernels.cu:151-174 on a [irec][it] record.
+__global__ void mute2d_kernel(float *shot, int nx, int nt, float dt, float favg, float dx,
+                              float dz, int fs, int ds, int zs, int is, float vp1, float eps1, int tt)
+{
+    const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (id >= (long long)nx * nt) return;
+    const int ix = (int)(id / nt), it = (int)(id % nt);
+    const float mu_x = dx * abs(ix - fs - (is - 1) * ds);
+    const float mu_z = dz * zs;
+    const float mu_t0 = sqrtf(mu_x * mu_x + mu_z * mu_z) / (vp1 * sqrtf(1 + 2 * eps1));
+    const int mu_t = (int)(2.0 / (dt * favg));
+    const int mu_nt = (int)(mu_t0 / dt) + mu_t + tt;
+    if ((it > (int)(mu_t0 / dt) - tt) && it < mu_nt) shot[id] = 0.0f;
+}
+
+// mute_directwave3d GPU_vti3dfd_kernels.cu:342-369 on a [iy][ix][it] record.
+__global__ void mute3d_kernel(float *shot, int nx, int ny, int nt, float dt, float favg,
+                              float dx, float dy, float dz, int fsx, int dsx, int fsy,
+                              int dsy, int fsz, int is, float vp1, float eps1, int tt)
+{
+    const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (id >= (long long)nx * ny * nt) return;
+    const int it = (int)(id % nt);
+    const long long r = id / nt;
+    const int ix = (int)(r % nx), iy = (int)(r / nx);
+    const float mu_x = dx * abs(ix - fsx - is * dsx);
+    const float mu_y = dy * abs(iy - fsy - is * dsy);
+    const float mu_z = dz * fsz;
+    const float mu_t0 = sqrtf(mu_x * mu_x + mu_y * mu_y + mu_z * mu_z) / (vp1 * sqrtf(1 + 2 * eps1));
+    const int mu_t = (int)(2.0 / (dt * favg));
+    const int mu_nt = (int)(mu_t0 / dt) + mu_t + tt;
+    if (it < mu_nt) shot[id] = 0.0f;
+}
+
+__global__ void copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n4)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------- launchers ----
+hipError_t launch_vel(int ndim, const VelArgs &a, hipStream_t s)
+{
+    const bool inj = a.src >= 0;
+    if (ndim == 2) {
+        dim3 blk(TZ, TX), grd((a.g.nnz + TZ - 1) / TZ, (a.g.nnx + TX - 1) / TX);
+        if (inj) hipLaunchKernelGGL(vel2d_kernel<true>, grd, blk, 0, s, a);
+        else hipLaunchKernelGGL(vel2d_kernel<false>, grd, blk, 0, s, a);
+    } else {
+        const int nzt = (a.g.nnz + TZ - 1) / TZ, nxt = (a.g.nnx + TX - 1) / TX;
+        const int nt = nzt * nxt, ntp = (nt + NXCD - 1) / NXCD * NXCD;
+        dim3 blk(TZ, TX), grd(ntp);
+        if (inj) hipLaunchKernelGGL(vel3d_kernel<true>, grd, blk, 0, s, a, nzt, nt, ntp);
+        else hipLaunchKernelGGL(vel3d_kernel<false>, grd, blk, 0, s, a, nzt, nt, ntp);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_stress(int ndim, const StressArgs &a, hipStream_t s)
+{
+    if (ndim == 2) {
+        dim3 blk(TZ, TX), grd((a.g.nnz + TZ - 1) / TZ, (a.g.nnx + TX - 1) / TX);
+        hipLaunchKernelGGL(stress2d_kernel, grd, blk, 0, s, a);
+    } else {
+        const int nzt = (a.g.nnz + TZ - 1) / TZ, nxt = (a.g.nnx + TX - 1) / TX;
+        const int nt = nzt * nxt, ntp = (nt + NXCD - 1) / NXCD * NXCD;
+        dim3 blk(TZ, TX), grd(ntp);
+        hipLaunchKernelGGL(stress3d_kernel, grd, blk, 0, s, a, nzt, nt, ntp);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pad_model(const Grid &g, int ndim, int nx, int ny, int nz, int npml,
+                            const float *src, float *dst, hipStream_t s)
+{
+    dim3 blk(256), grd((g.ldz + 255) / 256, g.nnx, g.nny);
+    hipLaunchKernelGGL(pad_kernel, grd, blk, 0, s, g, ndim, nx, ny, nz, npml, src, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(float *p, size_t n, hipStream_t s)
+{
+    const size_t n4 = n / 4;
+    if (n4) {
+        const size_t blocks = std::min<size_t>((n4 + 255) / 256, 8192);
+        hipLaunchKernelGGL(zero_kernel, dim3(blocks), dim3(256), 0, s,
+                           reinterpret_cast<float4 *>(p), n4);
+    }
+    const size_t tail = n - n4 * 4;
+    if (tail) hipLaunchKernelGGL(zero_tail_kernel, dim3(1), dim3(4), 0, s, p + n4 * 4, tail);
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose(const float *in, float *out, int rows, int cols, hipStream_t s)
+{
+    dim3 blk(32, 8), grd((cols + 31) / 32, (rows + 31) / 32);
+    hipLaunchKernelGGL(transpose_kernel, grd, blk, 0, s, in, out, rows, cols);
+    return hipGetLastError();
+}
+
+hipError_t launch_mute2d(float *rec_t, int nrec, int nt, float dt, float favg, float dx,
+                         float dz, int fs, int ds, int zs, int is, float vp1, float eps1,
+                         int tt, hipStream_t s)
+{
+    const long long n = (long long)nrec * nt;
+    hipLaunchKernelGGL(mute2d_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       rec_t, nrec, nt, dt, favg, dx, dz, fs, ds, zs, is, vp1, eps1, tt);
+    return hipGetLastError();
+}
+
+hipError_t launch_mute3d(float *rec_t, int nx, int ny, int nt, float dt, float favg,
+                         float dx, float dy, float dz, int fsx, int dsx, int fsy, int dsy,
+                         int fsz, int is, float vp1, float eps1, int tt, hipStream_t s)
+{
+    const long long n = (long long)nx * ny * nt;
+    hipLaunchKernelGGL(mute3d_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       rec_t, nx, ny, nt, dt, favg, dx, dy, dz, fsx, dsx, fsy, dsy, fsz, is,
+                       vp1, eps1, tt);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(void *dst, const void *src, size_t nbytes, hipStream_t s)
+{
+    const size_t n4 = nbytes / 16;
+    const size_t blocks = std::min<size_t>((n4 + 255) / 256, 16384);
+    if (n4)
+        hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, s,
+                           reinterpret_cast<const float4 *>(src), reinterpret_cast<float4 *>(dst), n4);
+    return hipGetLastError();
+}
+
+}  // namespace vti

@@ -1,0 +1,328 @@
+"""Python binding of libvti.so (include/vti.h) — the MI355X VTI FD path.
+
+Mirrors the reference's entry points for this path:
+
+* ``GPU_vti2dfd``, ``GPU_vti3dfd``, ``CPU_vti2dfd`` — the file-in / file-out
+  drivers with the reference argument lists
+  (2D-3D-vti-fd-cuda-gtk-GUI/GPU_vti2dfd_kernels.cu:301-304,
+  GPU_vti3dfd_kernels.cu:371-374, CPU_vti2dfd_kernels.c:366-367).
+  The reference returns void; these raise ``VtiError`` on failure.
+* ``Propagator`` — the per-time-step structure of those drivers on arrays
+  already in memory (velocity update, stress update with absorbing boundary,
+  source injection, receiver recording).
+
+Everything computes on the GPU through the HIP library; there is no CPU
+path.  Importing works without a GPU; calls that need one raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvti.so")
+
+MODE_GPU_REF = 0
+MODE_CPU_REF = 1
+REC_DEFAULT, REC_P, REC_2P, REC_PQ = -1, 0, 1, 2
+
+FIELDS = {n: i for i, n in enumerate(
+    ["u", "v", "w", "px", "py", "pz", "qx", "qy", "qz", "P", "Q", "vp", "eps", "del"])}
+TABLES = {n: 20 + i for i, n in enumerate(
+    ["coffx1", "coffx2", "coffy1", "coffy2", "coffz1", "coffz2",
+     "acoffx1", "acoffx2", "acoffy1", "acoffy2", "acoffz1", "acoffz2"])}
+T_SRCAMP, T_D0 = 40, 41
+
+
+class VtiError(RuntimeError):
+    pass
+
+
+class Cfg(C.Structure):
+    _fields_ = [("ndim", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("npml", C.c_int), ("dx", C.c_float), ("dy", C.c_float), ("dz", C.c_float),
+                ("dt", C.c_float), ("nt", C.c_int), ("favg", C.c_float), ("pfac", C.c_float),
+                ("wtype", C.c_int), ("mode", C.c_int), ("sv", C.c_int), ("sv_R", C.c_int),
+                ("sv_r", C.c_int), ("rec_kind", C.c_int)]
+
+
+class Shot(C.Structure):
+    _fields_ = [(n, C.c_int) for n in (
+        "sx", "sy", "sz", "sv_cx", "sv_cy", "sv_cz", "rec_z",
+        "rec_x0", "rec_dx", "rec_nx", "rec_y0", "rec_dy", "rec_ny")]
+
+
+_lib = None
+FP = C.POINTER(C.c_float)
+DP = C.POINTER(C.c_double)
+VP = C.c_void_p
+
+
+def _declare(L):
+    sig = {
+        "vti_strerror": (C.c_char_p, [C.c_int]),
+        "vti_last_error": (C.c_char_p, []),
+        "vti_cfg_ref2d": (C.c_int, [C.POINTER(Cfg)] + [C.c_int] * 10),
+        "vti_cfg_ref3d": (C.c_int, [C.POINTER(Cfg)] + [C.c_int] * 11),
+        "vti_shot_ref2d": (C.c_int, [C.POINTER(Shot), C.POINTER(Cfg)] + [C.c_int] * 5),
+        "vti_shot_ref3d": (C.c_int, [C.POINTER(Shot), C.POINTER(Cfg)] + [C.c_int] * 9),
+        "vti_host_tables": (C.c_int, [C.POINTER(Cfg), FP, FP, FP, FP]),
+        "vti_host_source_gains": (C.c_int, [C.POINTER(Cfg), FP]),
+        "vti_create": (C.c_int, [C.POINTER(Cfg), C.POINTER(VP)]),
+        "vti_destroy": (C.c_int, [VP]),
+        "vti_set_stream": (C.c_int, [VP, VP]),
+        "vti_set_model": (C.c_int, [VP, VP, VP, VP, C.c_int]),
+        "vti_set_shot": (C.c_int, [VP, C.POINTER(Shot)]),
+        "vti_reset": (C.c_int, [VP]),
+        "vti_update_vel": (C.c_int, [VP, C.c_int]),
+        "vti_update_stress": (C.c_int, [VP, C.c_int]),
+        "vti_step": (C.c_int, [VP, C.c_int, C.c_int]),
+        "vti_run": (C.c_int, [VP]),
+        "vti_nrec": (C.c_int, [VP]),
+        "vti_mute": (C.c_int, [VP] + [C.c_int] * 5),
+        "vti_mute3d": (C.c_int, [VP] + [C.c_int] * 7),
+        "vti_get_record": (C.c_int, [VP, FP]),
+        "vti_record_device": (VP, [VP]),
+        "vti_get_field": (C.c_int, [VP, C.c_int, FP]),
+        "vti_set_field": (C.c_int, [VP, C.c_int, FP]),
+        "vti_get_table": (C.c_int, [VP, C.c_int, FP]),
+        "vti_padded_dims": (C.c_int, [VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "vti_prof_enable": (C.c_int, [VP, C.c_int]),
+        "vti_prof_read": (C.c_int, [VP, DP, DP, C.POINTER(C.c_int)]),
+        "vti_calib_copy": (C.c_int, [VP, VP, C.c_size_t, VP]),
+        "vti_sync": (C.c_int, [VP]),
+        "vti_dropin_status": (C.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    for name in ("GPU_vti2dfd", "CPU_vti2dfd"):
+        fn = getattr(L, name)
+        fn.restype = None
+        fn.argtypes = [C.c_int] * 6 + [C.c_char_p] * 3 + [C.c_int] * 5 + [C.c_char_p] * 2 + [C.c_int] * 3
+    L.GPU_vti3dfd.restype = None
+    L.GPU_vti3dfd.argtypes = [C.c_int] * 8 + [C.c_char_p] * 3 + [C.c_int] * 8 + [C.c_char_p] * 2 + \
+        [C.c_int] * 3
+
+
+def lib():
+    """Load libvti.so; raise if it has not been built (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise VtiError(f"{LIB_PATH} is missing: build it with `python -m geophysics_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def check(code, what=""):
+    if code < 0:
+        L = lib()
+        msg = L.vti_last_error().decode(errors="replace")
+        raise VtiError(f"{what}: {L.vti_strerror(code).decode()} ({code}): {msg}")
+    return code
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(FP)
+
+
+def cfg_ref2d(mode, nx, nz, dx, dz, npd, SV, favg, nt, dt_us):
+    c = Cfg()
+    check(lib().vti_cfg_ref2d(C.byref(c), mode, nx, nz, dx, dz, npd, SV, favg, nt, dt_us), "cfg_ref2d")
+    return c
+
+
+def cfg_ref3d(nx, ny, nz, dx, dy, dz, npml, SV, favg, nt, dt_us):
+    c = Cfg()
+    check(lib().vti_cfg_ref3d(C.byref(c), nx, ny, nz, dx, dy, dz, npml, SV, favg, nt, dt_us), "cfg_ref3d")
+    return c
+
+
+def shot_ref2d(cfg, is_, fs, ds, zs, rec_depth=1):
+    s = Shot()
+    check(lib().vti_shot_ref2d(C.byref(s), C.byref(cfg), is_, fs, ds, zs, rec_depth), "shot_ref2d")
+    return s
+
+
+def shot_ref3d(cfg, is_, fsx, dsx, fsy, dsy, fsz, dsz, rec_depth=1, rec_step=1):
+    s = Shot()
+    check(lib().vti_shot_ref3d(C.byref(s), C.byref(cfg), is_, fsx, dsx, fsy, dsy, fsz, dsz,
+                               rec_depth, rec_step), "shot_ref3d")
+    return s
+
+
+def host_tables(cfg, vp, eps):
+    """Damping tables and d0 built by the library's host code (no GPU needed).
+
+    Returns (tables dict, d0)."""
+    nn = max(cfg.nx, cfg.ny if cfg.ndim == 3 else 1, cfg.nz) + 2 * cfg.npml
+    out = np.zeros((12, nn), dtype=np.float32)
+    d0 = np.zeros(1, dtype=np.float32)
+    vp, pv = _f32(vp)
+    eps, pe = _f32(eps)
+    check(lib().vti_host_tables(C.byref(cfg), pv, pe, out.ctypes.data_as(FP), d0.ctypes.data_as(FP)),
+          "host_tables")
+    names = list(TABLES)
+    lens = {"x": cfg.nx + 2 * cfg.npml, "y": (cfg.ny + 2 * cfg.npml) if cfg.ndim == 3 else 1,
+            "z": cfg.nz + 2 * cfg.npml}
+    return {n: out[i, :lens[n[-2]]].copy() for i, n in enumerate(names)}, float(d0[0])
+
+
+def host_source_gains(cfg):
+    out = np.zeros(cfg.nt, dtype=np.float32)
+    check(lib().vti_host_source_gains(C.byref(cfg), out.ctypes.data_as(FP)), "host_source_gains")
+    return out
+
+
+class Propagator:
+    """One VTI FD context on the GPU (the per-step structure of GPU_vti2dfd /
+    GPU_vti3dfd / CPU_vti2dfd, GPU_vti2dfd_kernels.cu:404-444)."""
+
+    def __init__(self, cfg, vp=None, eps=None, dl=None):
+        self.cfg = cfg
+        h = VP()
+        check(lib().vti_create(C.byref(cfg), C.byref(h)), "vti_create")
+        self.h = h
+        nnx, nny, nnz = C.c_int(), C.c_int(), C.c_int()
+        check(lib().vti_padded_dims(h, C.byref(nnx), C.byref(nny), C.byref(nnz)), "padded_dims")
+        self.nnx, self.nny, self.nnz = nnx.value, nny.value, nnz.value
+        self.shot = None
+        if vp is not None:
+            self.set_model(vp, eps, dl)
+
+    @property
+    def shape(self):
+        return (self.nny, self.nnx, self.nnz) if self.cfg.ndim == 3 else (self.nnx, self.nnz)
+
+    def set_model(self, vp, eps, dl, on_device=False):
+        if on_device:   # integer device addresses (e.g. torch tensor.data_ptr())
+            check(lib().vti_set_model(self.h, VP(vp), VP(eps), VP(dl), 1), "set_model")
+            return
+        a, _ = _f32(vp)
+        b, _ = _f32(eps)
+        d, _ = _f32(dl)
+        n = self.cfg.nx * self.cfg.nz * (self.cfg.ny if self.cfg.ndim == 3 else 1)
+        if a.size != n or b.size != n or d.size != n:
+            raise VtiError(f"model arrays must have {n} cells")
+        check(lib().vti_set_model(self.h, a.ctypes.data, b.ctypes.data, d.ctypes.data, 0), "set_model")
+
+    def set_shot(self, shot):
+        check(lib().vti_set_shot(self.h, C.byref(shot)), "set_shot")
+        self.shot = shot
+
+    def reset(self):
+        check(lib().vti_reset(self.h), "reset")
+
+    def update_vel(self, it):
+        check(lib().vti_update_vel(self.h, it), "update_vel")
+
+    def update_stress(self, it):
+        check(lib().vti_update_stress(self.h, it), "update_stress")
+
+    def step(self, it0, n):
+        check(lib().vti_step(self.h, it0, n), "step")
+
+    def run(self):
+        check(lib().vti_run(self.h), "run")
+
+    def sync(self):
+        check(lib().vti_sync(self.h), "sync")
+
+    @property
+    def nrec(self):
+        return lib().vti_nrec(self.h)
+
+    def mute(self, fs, ds, zs, is_, tt=100):
+        check(lib().vti_mute(self.h, fs, ds, zs, is_, tt), "mute")
+
+    def mute3d(self, fsx, dsx, fsy, dsy, fsz, is_, tt=100):
+        check(lib().vti_mute3d(self.h, fsx, dsx, fsy, dsy, fsz, is_, tt), "mute3d")
+
+    def record(self):
+        out = np.zeros((self.nrec, self.cfg.nt), dtype=np.float32)
+        check(lib().vti_get_record(self.h, out.ctypes.data_as(FP)), "get_record")
+        return out
+
+    def field(self, name):
+        out = np.zeros(self.shape, dtype=np.float32)
+        check(lib().vti_get_field(self.h, FIELDS[name], out.ctypes.data_as(FP)), "get_field")
+        return out
+
+    def set_field(self, name, arr):
+        a, p = _f32(arr)
+        if a.size != int(np.prod(self.shape)):
+            raise VtiError("field size mismatch")
+        check(lib().vti_set_field(self.h, FIELDS[name], p), "set_field")
+
+    def table(self, name):
+        out = np.zeros(max(self.nnx, self.nny, self.nnz, self.cfg.nt, 1), dtype=np.float32)
+        n = check(lib().vti_get_table(self.h, TABLES[name], out.ctypes.data_as(FP)), "get_table")
+        return out[:n].copy()
+
+    def srcamp(self):
+        out = np.zeros(max(self.cfg.nt, 1), dtype=np.float32)
+        n = check(lib().vti_get_table(self.h, T_SRCAMP, out.ctypes.data_as(FP)), "get_table")
+        return out[:n].copy()
+
+    def d0(self):
+        out = np.zeros(1, dtype=np.float32)
+        check(lib().vti_get_table(self.h, T_D0, out.ctypes.data_as(FP)), "get_table")
+        return float(out[0])
+
+    def prof_enable(self, on=True):
+        check(lib().vti_prof_enable(self.h, 1 if on else 0), "prof_enable")
+
+    def prof_read(self):
+        a, b, n = C.c_double(), C.c_double(), C.c_int()
+        check(lib().vti_prof_read(self.h, C.byref(a), C.byref(b), C.byref(n)), "prof_read")
+        return a.value, b.value, n.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vti_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _b(s):
+    return os.fsencode(s)
+
+
+def GPU_vti2dfd(nx, nz, dx, dz, npd, SV, FNv, FNe, FNd, favg, ns, fs, ds, zs, FNshot, FNsnap,
+                nt, dt, run_count=0):
+    """GPU_vti2dfd_kernels.cu:301 — 2D GPU driver (files in, shot/snapshot files out)."""
+    lib().GPU_vti2dfd(nx, nz, dx, dz, npd, SV, _b(FNv), _b(FNe), _b(FNd), favg, ns, fs, ds, zs,
+                      _b(FNshot), _b(FNsnap), nt, dt, run_count)
+    check(lib().vti_dropin_status(), "GPU_vti2dfd")
+
+
+def CPU_vti2dfd(nx, nz, dx, dz, npd, SV, FNv, FNe, FNd, favg, ns, fs, ds, zs, FNshot, FNsnap,
+                nt, dt, run_count=0):
+    """CPU_vti2dfd_kernels.c:366 — the CPU driver's semantics, computed on the GPU."""
+    lib().CPU_vti2dfd(nx, nz, dx, dz, npd, SV, _b(FNv), _b(FNe), _b(FNd), favg, ns, fs, ds, zs,
+                      _b(FNshot), _b(FNsnap), nt, dt, run_count)
+    check(lib().vti_dropin_status(), "CPU_vti2dfd")
+
+
+def GPU_vti3dfd(nx, ny, nz, dx, dy, dz, npml, SV, FNv, FNe, FNd, favg, ns, fsx, dsx, fsy, dsy,
+                fsz, dsz, FNshot, FNsnap, nt, dt, run_count=0):
+    """GPU_vti3dfd_kernels.cu:371 — 3D GPU driver."""
+    lib().GPU_vti3dfd(nx, ny, nz, dx, dy, dz, npml, SV, _b(FNv), _b(FNe), _b(FNd), favg, ns,
+                      fsx, dsx, fsy, dsy, fsz, dsz, _b(FNshot), _b(FNsnap), nt, dt, run_count)
+    check(lib().vti_dropin_status(), "GPU_vti3dfd")

@@ -1,0 +1,214 @@
+/*
+ * vti.h — C ABI of the MI355X (gfx950) first-order staggered-grid
+ * pseudo-acoustic VTI finite-difference path (libvti.so).
+ *
+ * Plain C types only: ints, floats, pointers and sizes.  Streams are
+ * passed as `void*` holding a hipStream_t (NULL = the context's own
+ * stream).  Every call returns an int status (VTI_OK = 0, negative codes
+ * below); nothing calls exit().
+ *
+ * Array layout everywhere is the reference's: z fastest, then x, then y
+ * (2D: id = iz + ix*nz; 3D: id = iz + ix*nz + iy*nz*nx).  Shot records are
+ * [irec][it] (t fastest), receivers ordered x fastest then y.
+ *
+ * Citations are path:line in the reference repository
+ * (liaman/Geophysics).
+ */
+#ifndef VTI_H
+#define VTI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define VTI_OK          0
+#define VTI_ERR_ARG    -1   /* invalid argument or configuration        */
+#define VTI_ERR_ALLOC  -2   /* device or host allocation failed         */
+#define VTI_ERR_HIP    -3   /* HIP runtime error                        */
+#define VTI_ERR_IO     -4   /* file open / read / write failed          */
+#define VTI_ERR_CFL    -5   /* stability / dispersion check failed      */
+#define VTI_ERR_STATE  -6   /* call out of order (no model, no shot...) */
+#define VTI_ERR_NODEV  -7   /* no HIP device                            */
+
+/* ---- semantics ------------------------------------------------------- */
+/* The reference's GPU and CPU drivers compute different seismograms
+ * (source footprint and phase, damping strength, stress-table form,
+ * receiver scale).  The stencil kernels are the same for both; the mode
+ * selects host-built tables and the fused source/receiver work. */
+#define VTI_MODE_GPU_REF 0  /* GPU_vti2dfd_kernels.cu / GPU_vti3dfd_kernels.cu */
+#define VTI_MODE_CPU_REF 1  /* CPU_vti2dfd_kernels.c                            */
+
+#define VTI_REC_DEFAULT -1  /* mode default: P (GPU), P+P (CPU)                 */
+#define VTI_REC_P        0  /* GPU_vti2dfd_kernels.cu:147                       */
+#define VTI_REC_2P       1  /* CPU_vti2dfd_kernels.c:490                        */
+#define VTI_REC_PQ       2  /* Toa_gpu_3dvti_rtm_adcigs_1orderfunction.cu:331   */
+
+typedef struct vti_cfg {
+    int ndim;          /* 2 or 3                                            */
+    int nx, ny, nz;    /* model sizes in cells (ny ignored in 2D)           */
+    int npml;          /* absorbing-layer width in cells (npd / npml)       */
+    float dx, dy, dz;  /* grid spacing, metres                              */
+    float dt;          /* time step, seconds                                */
+    int nt;            /* number of time steps                              */
+    float favg;        /* wavelet peak frequency, Hz                        */
+    float pfac;        /* source gain (reference: 10)                       */
+    int wtype;         /* 1 Ricker, 2 / 3 Gaussian forms                    */
+    int mode;          /* VTI_MODE_*                                        */
+    int sv;            /* taper eps/delta around the source (SV flag)       */
+    int sv_R, sv_r;    /* taper radii in cells; 0 = mode default            */
+    int rec_kind;      /* VTI_REC_*                                         */
+} vti_cfg;
+
+/* One shot: source cell, taper centre and a regular receiver grid, all
+ * as padded-grid indices (0-based, npml included). */
+typedef struct vti_shot {
+    int sx, sy, sz;            /* source cell                               */
+    int sv_cx, sv_cy, sv_cz;   /* SV-taper centre                           */
+    int rec_z;                 /* receiver depth row                        */
+    int rec_x0, rec_dx, rec_nx;
+    int rec_y0, rec_dy, rec_ny;/* ignored in 2D                             */
+} vti_shot;
+
+typedef struct vti_ctx vti_ctx;
+
+/* Field selectors for vti_get_field / vti_set_field (padded volumes). */
+enum {
+    VTI_F_U = 0, VTI_F_V, VTI_F_W, VTI_F_PX, VTI_F_PY, VTI_F_PZ,
+    VTI_F_QX, VTI_F_QY, VTI_F_QZ, VTI_F_P, VTI_F_Q,
+    VTI_F_VP, VTI_F_EPS, VTI_F_DEL
+};
+/* Table selectors for vti_get_table (damping tables along one axis). */
+enum {
+    VTI_T_COFFX1 = 20, VTI_T_COFFX2, VTI_T_COFFY1, VTI_T_COFFY2,
+    VTI_T_COFFZ1, VTI_T_COFFZ2, VTI_T_ACOFFX1, VTI_T_ACOFFX2,
+    VTI_T_ACOFFY1, VTI_T_ACOFFY2, VTI_T_ACOFFZ1, VTI_T_ACOFFZ2,
+    VTI_T_SRCAMP = 40, VTI_T_D0 = 41
+};
+
+/* ---- configuration helpers: the reference drivers' index conversions -- */
+/* cfg for GPU_vti2dfd (GPU_vti2dfd_kernels.cu:301-352) or CPU_vti2dfd
+ * (CPU_vti2dfd_kernels.c:366-390) arguments; dt_us in microseconds. */
+int vti_cfg_ref2d(vti_cfg *cfg, int mode, int nx, int nz, int dx, int dz,
+                  int npd, int SV, int favg, int nt, int dt_us);
+/* cfg for GPU_vti3dfd (GPU_vti3dfd_kernels.cu:371-413). */
+int vti_cfg_ref3d(vti_cfg *cfg, int nx, int ny, int nz, int dx, int dy,
+                  int dz, int npml, int SV, int favg, int nt, int dt_us);
+/* Shot `is` (1-based, as the 2D reference loop) of a 2D run: source
+ * (GPU_vti2dfd_kernels.cu:185,202-203 / CPU_vti2dfd_kernels.c:61-66),
+ * taper centre (:260-261 / :137-138), receivers at every x at 1-based
+ * depth rec_depth (reference: 1 = row npd, :147). */
+int vti_shot_ref2d(vti_shot *shot, const vti_cfg *cfg, int is, int fs,
+                   int ds, int zs, int rec_depth);
+/* Shot `is` (0-based) of a 3D run (GPU_vti3dfd_kernels.cu:63-65,82-84,
+ * 155-157); receivers every rec_step cells in x and y at depth rec_depth
+ * (reference: every cell at row npml, :325). */
+int vti_shot_ref3d(vti_shot *shot, const vti_cfg *cfg, int is, int fsx,
+                   int dsx, int fsy, int dsy, int fsz, int dsz,
+                   int rec_depth, int rec_step);
+
+/* Host-only table construction (no device needed): d0 and the 12 damping
+ * tables (rows of max(nnx,nny,nnz) floats in the order of the VTI_T_COFF*
+ * selectors) for an unpadded vp, exactly as vti_set_model builds them.
+ * get_d02d GPU_vti2dfd_kernels.cu:102-106, initial_coffe2d :108-139,
+ * get_constant / initial_coffe CPU_vti2dfd_kernels.c:179-332. */
+int vti_host_tables(const vti_cfg *cfg, const float *vp, const float *eps,
+                    float *tables, float *d0);
+/* Host-only per-step source gains pfac*wavelet(t), nt floats
+ * (add_source2d GPU_vti2dfd_kernels.cu:183-206). */
+int vti_host_source_gains(const vti_cfg *cfg, float *amp);
+
+/* ---- context lifecycle ------------------------------------------------ */
+/* Replaces the allocation / table set-up blocks of GPU_vti2dfd
+ * (GPU_vti2dfd_kernels.cu:354-398) and GPU_vti3dfd
+ * (GPU_vti3dfd_kernels.cu:429-484) with a per-context, re-entrant form. */
+int vti_create(const vti_cfg *cfg, vti_ctx **out);
+int vti_destroy(vti_ctx *ctx);
+int vti_set_stream(vti_ctx *ctx, void *hip_stream);
+/* Model without padding (nx*nz or nx*ny*nz floats, z fastest).  on_device:
+ * 0 = host pointers, 1 = device pointers.  Pads by edge replication
+ * (pad_vv2d GPU_vti2dfd_kernels.cu:49-74, pad_vv3d GPU_vti3dfd_kernels.cu:208-255)
+ * and builds d0 and the damping tables (get_d02d :102-106,
+ * initial_coffe2d :108-139; CPU form CPU_vti2dfd_kernels.c:179-332). */
+int vti_set_model(vti_ctx *ctx, const float *vp, const float *eps,
+                  const float *del, int on_device);
+/* Select the shot: source, taper centre, receivers; builds the per-step
+ * source table (add_source2d :176-207 / ptsource CPU_vti2dfd_kernels.c:50-76). */
+int vti_set_shot(vti_ctx *ctx, const vti_shot *shot);
+/* Zero wavefields and the receiver buffer (the per-shot cudaMemset block,
+ * GPU_vti2dfd_kernels.cu:407-417). */
+int vti_reset(vti_ctx *ctx);
+
+/* ---- per-time-step calls (GPU_vti2dfd_kernels.cu:423-429) -------------- */
+/* Velocity update of step it; in GPU mode the point source of step it is
+ * injected into P and Q first (add_source2d + update_vel2d). */
+int vti_update_vel(vti_ctx *ctx, int it);
+/* Stress / pressure update of step it with damping, the in-stress source
+ * (CPU mode) and the receiver gather of row it (update_stress2d +
+ * shot_record2d, :425-429). */
+int vti_update_stress(vti_ctx *ctx, int it);
+/* nsteps full steps starting at it0. */
+int vti_step(vti_ctx *ctx, int it0, int nsteps);
+/* reset + all nt steps. */
+int vti_run(vti_ctx *ctx);
+
+/* ---- outputs ------------------------------------------------------------ */
+int vti_nrec(const vti_ctx *ctx);
+/* Direct-wave mute of the receiver buffer: mute_directwave2d
+ * (GPU_vti2dfd_kernels.cu:151-174, tt=100 at :439) / mute_directwave3d
+ * (GPU_vti3dfd_kernels.cu:342-369).  fs,ds,zs,is as the shot helpers. */
+int vti_mute(vti_ctx *ctx, int fs, int ds, int zs, int is, int tt);
+int vti_mute3d(vti_ctx *ctx, int fsx, int dsx, int fsy, int dsy, int fsz,
+               int is, int tt);
+/* Shot record [irec][it] into host memory (nrec*nt floats). */
+int vti_get_record(vti_ctx *ctx, float *dst);
+/* Device pointer of the receiver buffer, layout [it][irec]. */
+float *vti_record_device(vti_ctx *ctx);
+/* Padded field / table copies (host memory).  Fields: nnx*nnz (2D) or
+ * nnx*nny*nnz floats in the reference layout. */
+int vti_get_field(vti_ctx *ctx, int which, float *dst);
+int vti_set_field(vti_ctx *ctx, int which, const float *src);
+int vti_get_table(vti_ctx *ctx, int which, float *dst);
+/* Padded sizes (nnx, nny, nnz; nny = 1 in 2D). */
+int vti_padded_dims(const vti_ctx *ctx, int *nnx, int *nny, int *nnz);
+
+/* ---- measurement ------------------------------------------------------- */
+/* When on, vti_step brackets every velocity and stress launch with HIP
+ * events on the context stream; vti_prof_read sums them (ms). */
+int vti_prof_enable(vti_ctx *ctx, int on);
+int vti_prof_read(vti_ctx *ctx, double *vel_ms, double *stress_ms, int *nlaunch);
+/* Plain device copy of nbytes (bandwidth calibration kernel). */
+int vti_calib_copy(void *dst, const void *src, size_t nbytes, void *hip_stream);
+int vti_sync(vti_ctx *ctx);
+const char *vti_strerror(int code);
+const char *vti_last_error(void);
+
+/* ---- drop-in entry points with the reference signatures ------------------ */
+/* GPU_vti2dfd_kernels.cu:301-304 (caller GTK_gpuvti2dfd.c:88). */
+void GPU_vti2dfd(int nx, int nz, int dx, int dz, int npd, int SV,
+                 const char FNv[], const char FNe[], const char FNd[],
+                 int favg, int ns, int fs, int ds, int zs,
+                 const char FNshot[], const char FNsnap[], int nt, int dt,
+                 int run_count);
+/* GPU_vti3dfd_kernels.cu:371-374 (caller GTK_gpuvti3dfd.c:67-68). */
+void GPU_vti3dfd(int nx, int ny, int nz, int dx, int dy, int dz, int npml,
+                 int SV, const char FNv[], const char FNe[], const char FNd[],
+                 int favg, int ns, int fsx, int dsx, int fsy, int dsy,
+                 int fsz, int dsz, const char FNshot[], const char FNsnap[],
+                 int nt, int dt, int run_count);
+/* CPU_vti2dfd_kernels.c:366-367 (caller GTK_cpuvti2dfd.c:57): the CPU
+ * driver's semantics, computed on the GPU. */
+void CPU_vti2dfd(int nx, int nz, int dx, int dz, int npd, int SV,
+                 const char FNv[], const char FNe[], const char FNd[],
+                 int favg, int ns, int fs, int ds, int zs,
+                 const char FNshot[], const char FNsnap[], int nt, int dt,
+                 int run_count);
+/* Status of the last drop-in call (the reference entry points return void). */
+int vti_dropin_status(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VTI_H */

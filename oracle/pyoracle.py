@@ -1,0 +1,190 @@
+"""ctypes binding of liboracle_vti.so (the CPU restatement, vti_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: used as the checker by tests/, smoke() and the
+cpu_baseline leg of bench.py.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle_vti.so")
+
+
+class OrcParams(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("ndim", "nx", "ny", "nz", "npml")] + \
+        [(n, C.c_float) for n in ("dx", "dy", "dz", "dt")] + \
+        [("nt", C.c_int), ("favg", C.c_float), ("pfac", C.c_float)] + \
+        [(n, C.c_int) for n in (
+            "wtype", "d0_kind", "acoff_kind", "src_kind", "t0_kind",
+            "sx", "sy", "sz", "sv", "sv_kind", "sv_R", "sv_r",
+            "sv_cx", "sv_cy", "sv_cz",
+            "rec_z", "rec_x0", "rec_dx", "rec_nx", "rec_y0", "rec_dy", "rec_ny",
+            "rec_kind")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+FIELDS = {n: i for i, n in enumerate(
+    ["u", "v", "w", "px", "py", "pz", "qx", "qy", "qz", "P", "Q", "vp", "eps", "del"])}
+TABLES = {n: 20 + i for i, n in enumerate(
+    ["coffx1", "coffx2", "coffy1", "coffy2", "coffz1", "coffz2",
+     "acoffx1", "acoffx2", "acoffy1", "acoffy2", "acoffz1", "acoffz2"])}
+SRCAMP, D0 = 40, 41
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE, "all"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        fp = C.POINTER(C.c_float)
+        dp = C.POINTER(C.c_double)
+        pp = C.POINTER(OrcParams)
+        for sfx, rp in (("_f32", fp), ("_f64", dp)):
+            getattr(L, "orc_create" + sfx).restype = C.c_void_p
+            getattr(L, "orc_create" + sfx).argtypes = [pp, fp, fp, fp]
+            getattr(L, "orc_step" + sfx).argtypes = [C.c_void_p, C.c_int, rp]
+            getattr(L, "orc_run" + sfx).argtypes = [C.c_void_p, rp]
+            getattr(L, "orc_get" + sfx).argtypes = [C.c_void_p, C.c_int, rp]
+            getattr(L, "orc_set" + sfx).argtypes = [C.c_void_p, C.c_int, rp]
+            getattr(L, "orc_destroy" + sfx).argtypes = [C.c_void_p]
+        L.orc_ncell.restype = C.c_long
+        L.orc_ncell.argtypes = [C.c_void_p]
+        L.orc_params_gpu2d.argtypes = [pp] + [C.c_int] * 14
+        L.orc_params_cpu2d.argtypes = [pp] + [C.c_int] * 14
+        L.orc_params_gpu3d.argtypes = [pp] + [C.c_int] * 20
+        L.orc_mute2d.argtypes = [C.c_int, C.c_int] + [C.c_float] * 4 + [C.c_int] * 4 + \
+            [C.c_float, C.c_float, fp, C.c_int]
+        L.orc_mute3d.argtypes = [C.c_int] * 3 + [C.c_float] * 5 + [C.c_int] * 6 + \
+            [C.c_float, C.c_float, fp, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def params_gpu2d(nx, nz, dx, dz, npd, SV, favg, is_, fs, ds, zs, nt, dt_us, rec_depth=1):
+    p = OrcParams()
+    lib().orc_params_gpu2d(C.byref(p), nx, nz, dx, dz, npd, SV, favg, is_, fs, ds, zs,
+                           nt, dt_us, rec_depth)
+    return p
+
+
+def params_cpu2d(nx, nz, dx, dz, npd, SV, favg, is_, fs, ds, zs, nt, dt_us, rec_depth=1):
+    p = OrcParams()
+    lib().orc_params_cpu2d(C.byref(p), nx, nz, dx, dz, npd, SV, favg, is_, fs, ds, zs,
+                           nt, dt_us, rec_depth)
+    return p
+
+
+def params_gpu3d(nx, ny, nz, dx, dy, dz, npml, SV, favg, is_, fsx, dsx, fsy, dsy,
+                 fsz, dsz, nt, dt_us, rec_depth=1, rec_step=1):
+    p = OrcParams()
+    lib().orc_params_gpu3d(C.byref(p), nx, ny, nz, dx, dy, dz, npml, SV, favg, is_,
+                           fsx, dsx, fsy, dsy, fsz, dsz, nt, dt_us, rec_depth, rec_step)
+    return p
+
+
+class Sim:
+    """One shot of the oracle engine in float32 ('f32') or float64 ('f64')."""
+
+    def __init__(self, params, vp, eps, dl, prec="f32"):
+        self.p = params
+        self.prec = prec
+        self.dtype = np.float32 if prec == "f32" else np.float64
+        L = lib()
+        vp = np.ascontiguousarray(vp, dtype=np.float32)
+        eps = np.ascontiguousarray(eps, dtype=np.float32)
+        dl = np.ascontiguousarray(dl, dtype=np.float32)
+        self._keep = (vp, eps, dl)
+        self.h = getattr(L, "orc_create_" + prec)(C.byref(params), _fptr(vp), _fptr(eps), _fptr(dl))
+        if not self.h:
+            raise MemoryError("orc_create failed")
+        np_ = params.npml
+        self.nnx = params.nx + 2 * np_
+        self.nnz = params.nz + 2 * np_
+        self.nny = params.ny + 2 * np_ if params.ndim == 3 else 1
+        self.nrec = params.rec_nx * (params.rec_ny if params.ndim == 3 else 1)
+
+    def _ptr(self, a):
+        return _fptr(a) if self.prec == "f32" else _dptr(a)
+
+    def run(self):
+        shot = np.zeros((self.nrec, self.p.nt), dtype=self.dtype)
+        getattr(lib(), "orc_run_" + self.prec)(self.h, self._ptr(shot))
+        return shot
+
+    def step(self, it, rec=True):
+        row = np.zeros(self.nrec, dtype=self.dtype) if rec else None
+        getattr(lib(), "orc_step_" + self.prec)(self.h, it, self._ptr(row) if rec else None)
+        return row
+
+    def field(self, name):
+        out = np.zeros(self.nny * self.nnx * self.nnz, dtype=self.dtype)
+        getattr(lib(), "orc_get_" + self.prec)(self.h, FIELDS[name], self._ptr(out))
+        shape = (self.nny, self.nnx, self.nnz) if self.p.ndim == 3 else (self.nnx, self.nnz)
+        return out.reshape(shape)
+
+    def set_field(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=self.dtype).ravel()
+        assert a.size == self.nny * self.nnx * self.nnz
+        getattr(lib(), "orc_set_" + self.prec)(self.h, FIELDS[name], self._ptr(a))
+
+    def table(self, name):
+        n = max(self.nnx, self.nny, self.nnz)
+        out = np.zeros(n, dtype=self.dtype)
+        k = getattr(lib(), "orc_get_" + self.prec)(self.h, TABLES[name], self._ptr(out))
+        return out[:k]
+
+    def srcamp(self):
+        out = np.zeros(self.p.nt, dtype=self.dtype)
+        getattr(lib(), "orc_get_" + self.prec)(self.h, SRCAMP, self._ptr(out))
+        return out
+
+    def d0(self):
+        out = np.zeros(1, dtype=self.dtype)
+        getattr(lib(), "orc_get_" + self.prec)(self.h, D0, self._ptr(out))
+        return float(out[0])
+
+    def close(self):
+        if self.h:
+            getattr(lib(), "orc_destroy_" + self.prec)(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mute2d(shot, nt, dt, favg, dx, dz, fs, ds, zs, is_, vp1, eps1, tt=100):
+    """shot: float32 (nx, nt), muted in place."""
+    assert shot.dtype == np.float32 and shot.flags.c_contiguous
+    nx = shot.shape[0]
+    lib().orc_mute2d(nx, nt, dt, favg, dx, dz, fs, ds, zs, is_, vp1, eps1, _fptr(shot), tt)
+    return shot
+
+
+def mute3d(shot, nx, ny, nt, dt, favg, dx, dy, dz, fsx, fsy, dsx, dsy, fsz, is_, vp1, eps1, tt=100):
+    assert shot.dtype == np.float32 and shot.flags.c_contiguous
+    lib().orc_mute3d(nx, ny, nt, dt, favg, dx, dy, dz, fsx, fsy, dsx, dsy, fsz, is_,
+                     vp1, eps1, _fptr(shot), tt)
+    return shot

@@ -1,0 +1,99 @@
+/*
+ * vti_oracle.h — CPU restatement of the reference VTI staggered-grid FD path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This oracle is the checker for the HIP product
+ * in geophysics_amd/: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product never links, calls or falls
+ * back to it.
+ *
+ * Every function restates reference arithmetic (citations are
+ * path:line relative to the reference repository root):
+ *   2D-3D-vti-fd-cuda-gtk-GUI/CPU_vti2dfd_kernels.c   (CPU semantics)
+ *   2D-3D-vti-fd-cuda-gtk-GUI/GPU_vti2dfd_kernels.cu  (GPU 2D semantics)
+ *   2D-3D-vti-fd-cuda-gtk-GUI/GPU_vti3dfd_kernels.cu  (GPU 3D semantics)
+ *   2D-vti-fd+rtm-cuda/Toa_gpu2DvtiFdRtmAdcigsLaplace.cu (RTM imaging)
+ * Two instantiations: *_f32 (float, reference rounding order, no FMA
+ * contraction) and *_f64 (everything in double: the error reference).
+ */
+#ifndef VTI_ORACLE_H
+#define VTI_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Engine-level description of one shot.  All indices are padded-grid
+ * indices (0-based).  Layout of every volume: z fastest, then x, then y
+ * (id = iz + ix*nnz + iy*nnz*nnx), as in the reference. */
+typedef struct orc_params {
+    int ndim;              /* 2 or 3 */
+    int nx, ny, nz;        /* interior sizes; ny = 1 in 2D */
+    int npml;              /* absorbing-layer width (npd / npml) */
+    float dx, dy, dz;      /* metres */
+    float dt;              /* seconds, (float)(dt_us/1e6) */
+    int nt;
+    float favg;            /* Hz */
+    float pfac;            /* source gain */
+    int wtype;             /* 1 Ricker, 2 / 3 Gaussian variants */
+    int d0_kind;           /* 0 GPU (10*vp[N/2]...), 1 CPU (3*vpmax...), 2 none */
+    int acoff_kind;        /* 0 GPU tables, 1 CPU tables (coff1-based acoff2) */
+    int src_kind;          /* 0 point before velocity update, 1 7x7 Gaussian in stress */
+    int t0_kind;           /* 0: first time sample is dt, 1: 0 */
+    int sx, sy, sz;        /* source cell */
+    int sv;                /* SV-suppression taper of eps/delta on/off */
+    int sv_kind;           /* 0 GPU form, 1 CPU form */
+    int sv_R, sv_r;        /* outer radius, inner radius (GPU form) */
+    int sv_cx, sv_cy, sv_cz; /* taper centre */
+    int rec_z, rec_x0, rec_dx, rec_nx, rec_y0, rec_dy, rec_ny;
+    int rec_kind;          /* 0 P, 1 P+P, 2 P+Q */
+} orc_params;
+
+/* Parameter builders restating the reference drivers' conversions. */
+int orc_params_gpu2d(orc_params *p, int nx, int nz, int dx, int dz, int npd,
+                     int SV, int favg, int is, int fs, int ds, int zs,
+                     int nt, int dt_us, int rec_depth);
+int orc_params_cpu2d(orc_params *p, int nx, int nz, int dx, int dz, int npd,
+                     int SV, int favg, int is, int fs, int ds, int zs,
+                     int nt, int dt_us, int rec_depth);
+int orc_params_gpu3d(orc_params *p, int nx, int ny, int nz, int dx, int dy,
+                     int dz, int npml, int SV, int favg, int is, int fsx,
+                     int dsx, int fsy, int dsy, int fsz, int dsz, int nt,
+                     int dt_us, int rec_depth, int rec_step);
+
+/* Field / table selectors for orc_get_*. */
+enum {
+    ORC_U = 0, ORC_V, ORC_W, ORC_PX, ORC_PY, ORC_PZ, ORC_QX, ORC_QY, ORC_QZ,
+    ORC_P, ORC_Q, ORC_VP, ORC_EPS, ORC_DEL,
+    ORC_COFFX1 = 20, ORC_COFFX2, ORC_COFFY1, ORC_COFFY2, ORC_COFFZ1, ORC_COFFZ2,
+    ORC_ACOFFX1, ORC_ACOFFX2, ORC_ACOFFY1, ORC_ACOFFY2, ORC_ACOFFZ1, ORC_ACOFFZ2,
+    ORC_SRCAMP = 40, ORC_D0 = 41
+};
+
+void *orc_create_f32(const orc_params *p, const float *vp, const float *eps,
+                     const float *del);
+void *orc_create_f64(const orc_params *p, const float *vp, const float *eps,
+                     const float *del);
+int orc_step_f32(void *h, int it, float *rec);   /* rec: nrec or NULL */
+int orc_step_f64(void *h, int it, double *rec);
+int orc_run_f32(void *h, float *shot);           /* shot[it + nt*irec] */
+int orc_run_f64(void *h, double *shot);
+int orc_get_f32(void *h, int which, float *out);
+int orc_get_f64(void *h, int which, double *out);
+int orc_set_f32(void *h, int which, const float *in);
+int orc_set_f64(void *h, int which, const double *in);
+void orc_destroy_f32(void *h);
+void orc_destroy_f64(void *h);
+long orc_ncell(void *h32);
+
+/* Direct-wave mute of a finished shot record (float), GPU 2D / 3D forms. */
+void orc_mute2d(int nx, int nt, float dt, float favg, float dx, float dz,
+                int fs, int ds, int zs, int is, float vp1, float eps1,
+                float *shot, int tt);
+void orc_mute3d(int nx, int ny, int nt, float dt, float favg, float dx,
+                float dy, float dz, int fsx, int fsy, int dsx, int dsy,
+                int fsz, int is, float vp1, float eps1, float *shot, int tt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,467 @@
+/*
+ * vti_oracle_impl.h — body of the CPU restatement, included twice by
+ * vti_oracle.c with REAL = float (SFX _f32) and REAL = double (SFX _f64).
+ * TEST INFRASTRUCTURE ONLY (see vti_oracle.h).
+ *
+ * The float instantiation keeps the reference's left-to-right evaluation
+ * order and its float/double promotions; vti_oracle.c is compiled with
+ * -ffp-contract=off so no multiply-add is fused.
+ */
+
+#define CAT2(a, b) a##b
+#define CAT(a, b) CAT2(a, b)
+#define FN(name) CAT(name, SFX)
+#define SIM FN(orc_sim)
+
+typedef struct SIM {
+    orc_params p;
+    int nnx, nny, nnz;
+    long n;
+    REAL *vp, *ep, *de;     /* padded model */
+    REAL *tab[12];          /* coffx1 coffx2 coffy1 coffy2 coffz1 coffz2 acoffx1 .. acoffz2 */
+    REAL *f[11];            /* u v w px py pz qx qy qz P Q */
+    REAL d0;
+    REAL c[4];
+    REAL dtx, dty, dtz;
+    REAL *amp;              /* per-step source gain pfac*source (0 when inactive) */
+    int ng;                 /* Gaussian footprint (CPU source form) */
+    long gidx[49];
+    double gw[49];
+    int tap_n;              /* SV taper factor per integer r^2, 0..tap_n-1 */
+    double *tap;
+} SIM;
+
+/* Ricker / Gaussian wavelet.
+ * GPU_vti2dfd_kernels.cu:183-198, CPU_vti2dfd_kernels.c:30-48. */
+static REAL FN(wavelet)(REAL ts, REAL favg, int wtype)
+{
+    REAL src = 0;
+    if (wtype == 1) {
+        REAL x = favg * ts;
+        REAL xx = x * x;
+        src = (REAL)((1 - 2 * ORC_PI * ORC_PI * (xx)) * exp(-(ORC_PI * ORC_PI * xx)));
+    } else if (wtype == 2) {
+        REAL x = (REAL)((-4) * favg * favg * ORC_PI * ORC_PI / log(0.1));
+        src = (REAL)((-2) * ORC_PI * ORC_PI * ts * exp(-x * ts * ts));
+    } else if (wtype == 3) {
+        REAL x = (REAL)((-1) * favg * favg * ORC_PI * ORC_PI / log(0.1));
+        src = (REAL)exp(-x * ts * ts);
+    }
+    return src;
+}
+
+/* 1-D damping tables for one axis of nn_int interior cells.
+ * GPU form: GPU_vti2dfd_kernels.cu:108-139 (= GPU_vti3dfd_kernels.cu:283-314).
+ * CPU form: CPU_vti2dfd_kernels.c:271-332 — acoff2 is built from coff1
+ * (:307,309,314,319) and the right-side stress position is
+ * (1+i-nx-npd)/npd (:313). */
+static void FN(axis_tables)(int nn_int, int npml, REAL dtd0, int kind,
+                            REAL *c1, REAL *c2, REAL *a1, REAL *a2)
+{
+    int N = nn_int + 2 * npml;
+    for (int id = 0; id < N; id++) {
+        double xv, xs;
+        if (id >= npml && id < npml + nn_int) {
+            c1[id] = 1; c2[id] = 1; a1[id] = 1; a2[id] = 1;
+            continue;
+        }
+        if (id < npml) {
+            xv = (npml - 0.5 - id) / npml;
+            xs = ((npml - id) * 1.0) / npml;
+        } else {
+            xv = (0.5 + id - nn_int - npml) / npml;
+            xs = kind == 0 ? ((id - nn_int - npml) * 1.0) / npml
+                           : ((1 + id - nn_int - npml) * 1.0) / npml;
+        }
+        double ev = (double)dtd0 * pow(xv, 2.0);
+        double es = (double)dtd0 * pow(xs, 2.0);
+        c1[id] = (REAL)(1.0 / (1.0 + ev / 2.0));
+        c2[id] = (REAL)((double)c1[id] * (1.0 - ev / 2.0));
+        a1[id] = (REAL)(1.0 / (1.0 + es / 2.0));
+        a2[id] = kind == 0 ? (REAL)((double)a1[id] * (1.0 - es / 2.0))
+                           : (REAL)((double)c1[id] * (1.0 - es / 2.0));
+    }
+}
+
+void *FN(orc_create)(const orc_params *p, const float *vp, const float *eps,
+                     const float *del)
+{
+    SIM *s = (SIM *)calloc(1, sizeof(SIM));
+    if (!s) return NULL;
+    s->p = *p;
+    int ny = p->ndim == 3 ? p->ny : 1;
+    int np = p->npml;
+    s->nnx = p->nx + 2 * np;
+    s->nnz = p->nz + 2 * np;
+    s->nny = p->ndim == 3 ? ny + 2 * np : 1;
+    s->n = (long)s->nnx * s->nny * s->nnz;
+    long n = s->n;
+    s->vp = (REAL *)malloc(n * sizeof(REAL));
+    s->ep = (REAL *)malloc(n * sizeof(REAL));
+    s->de = (REAL *)malloc(n * sizeof(REAL));
+    for (int k = 0; k < 11; k++) s->f[k] = (REAL *)calloc(n, sizeof(REAL));
+    int nnmax = s->nnx > s->nnz ? s->nnx : s->nnz;
+    if (s->nny > nnmax) nnmax = s->nny;
+    for (int k = 0; k < 12; k++) s->tab[k] = (REAL *)calloc(nnmax, sizeof(REAL));
+    s->amp = (REAL *)calloc(p->nt > 0 ? p->nt : 1, sizeof(REAL));
+
+    /* Edge-replicating pad: pad_vv2d GPU_vti2dfd_kernels.cu:49-74 and
+     * pad_vv3d GPU_vti3dfd_kernels.cu:208-255 copy x, then y, then z;
+     * the result equals clamping each coordinate into the interior.
+     * (CPU_vti2dfd_kernels.c:244 returns early on a zero value and leaves
+     * pad cells uninitialised; that defect is not restated.) */
+    for (int iy = 0; iy < s->nny; iy++)
+        for (int ix = 0; ix < s->nnx; ix++)
+            for (int iz = 0; iz < s->nnz; iz++) {
+                int jx = ix < np ? 0 : (ix >= np + p->nx ? p->nx - 1 : ix - np);
+                int jz = iz < np ? 0 : (iz >= np + p->nz ? p->nz - 1 : iz - np);
+                int jy = 0;
+                if (p->ndim == 3) jy = iy < np ? 0 : (iy >= np + ny ? ny - 1 : iy - np);
+                long src = (long)jz + (long)jx * p->nz + (long)jy * p->nz * p->nx;
+                long id = (long)iz + (long)ix * s->nnz + (long)iy * s->nnz * s->nnx;
+                s->vp[id] = vp[src];
+                s->ep[id] = eps[src];
+                s->de[id] = del[src];
+            }
+
+    /* c[] GPU_vti2dfd_kernels.cu:37, cal_c CPU_vti2dfd_kernels.c:334-343 */
+    s->c[0] = (REAL)1.196289f;
+    s->c[1] = (REAL)-0.0797526f;
+    s->c[2] = (REAL)0.009570313f;
+    s->c[3] = (REAL)-0.0006975447f;
+    if (sizeof(REAL) == sizeof(double)) {
+        s->c[0] = 1.196289; s->c[1] = -0.0797526;
+        s->c[2] = 0.009570313; s->c[3] = -0.0006975447;
+    }
+    REAL dt = p->dt;
+    s->dtx = dt / (REAL)p->dx;
+    s->dty = dt / (REAL)p->dy;
+    s->dtz = dt / (REAL)p->dz;
+
+    /* d0: get_d02d GPU_vti2dfd_kernels.cu:105, get_d03d GPU_vti3dfd_kernels.cu:205,
+     * get_constant CPU_vti2dfd_kernels.c:185-192 */
+    if (p->d0_kind == 0) {
+        if (p->ndim == 2) {
+            REAL sdx = (REAL)p->dx + (REAL)p->dz;
+            s->d0 = (REAL)(10.0 * s->vp[(long)s->nnx * s->nnz / 2] * log(100000.0) /
+                           (2.0 * np * (sdx / 2.0)));
+        } else {
+            REAL sdx = (REAL)p->dx + (REAL)p->dy + (REAL)p->dz;
+            s->d0 = (REAL)(10.0 * s->vp[(long)s->nny * s->nnx * s->nnz / 2] * log(100000.0) /
+                           (2.0 * np * (sdx / 3.0)));
+        }
+    } else if (p->d0_kind == 1) {
+        REAL vmax = s->vp[np];
+        for (long id = np; id < n; id++)
+            if (vmax < s->vp[id]) vmax = s->vp[id];
+        s->d0 = (REAL)(3.0 * vmax * log(100000.0) / (2.0 * np * (REAL)p->dx));
+    } else {
+        s->d0 = 0;
+    }
+    REAL dtd0 = dt * s->d0;
+    FN(axis_tables)(p->nx, np, dtd0, p->acoff_kind, s->tab[0], s->tab[1], s->tab[6], s->tab[7]);
+    FN(axis_tables)(p->nz, np, dtd0, p->acoff_kind, s->tab[4], s->tab[5], s->tab[10], s->tab[11]);
+    if (p->ndim == 3)
+        FN(axis_tables)(ny, np, dtd0, p->acoff_kind, s->tab[2], s->tab[3], s->tab[8], s->tab[9]);
+
+    /* Per-step source gain: add_source2d GPU_vti2dfd_kernels.cu:176-207 with
+     * the float time t (t=dt, t+=dt: :419), ptsource CPU_vti2dfd_kernels.c:50-76. */
+    {
+        REAL t = p->t0_kind == 0 ? dt : (REAL)0;
+        REAL tdelay = (REAL)(1.0 / p->favg);
+        for (int it = 0; it < p->nt; it++) {
+            REAL a = 0;
+            if (t <= 2 * tdelay) {
+                REAL ts = t - tdelay;
+                REAL src = FN(wavelet)(ts, (REAL)p->favg, p->wtype);
+                a = (REAL)p->pfac * src;
+            }
+            s->amp[it] = a;
+            t += dt;
+        }
+    }
+    /* 7x7 Gaussian footprint exp(-z*z-x*x), CPU_vti2dfd_kernels.c:67-74 */
+    s->ng = 0;
+    if (p->src_kind == 1) {
+        for (int j = p->sz - 3; j <= p->sz + 3; j++)
+            for (int i = p->sx - 3; i <= p->sx + 3; i++) {
+                int x = i - p->sx, z = j - p->sz;
+                s->gidx[s->ng] = (long)i * s->nnz + j +
+                                 (long)p->sy * s->nnz * s->nnx * (p->ndim == 3);
+                s->gw[s->ng] = exp(-z * z - x * x);
+                s->ng++;
+            }
+    }
+    /* SV taper factor as a function of the integer r^2.
+     * GPU form GPU_vti2dfd_kernels.cu:260-278 (R=15, r=4) and
+     * Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:342-382 (R=18, r=7);
+     * CPU form CPU_vti2dfd_kernels.c:124,137-155 (CR=15, inner CR*CR/16). */
+    {
+        int R = p->sv_R;
+        s->tap_n = R * R + 1;
+        s->tap = (double *)calloc(s->tap_n, sizeof(double));
+        for (int r2 = 0; r2 <= R * R; r2++) {
+            double f;
+            if (p->sv_kind == 0) {
+                int r = p->sv_r;
+                if (r2 <= r * r) f = 0.0;
+                else {
+                    REAL sr = (REAL)sqrt((REAL)r2);
+                    if (sizeof(REAL) == sizeof(float)) sr = sqrtf((float)r2);
+                    f = 0.5 * (1 - cos(ORC_PI * ((sr - r) * 4.0 / (R * 3.0 - 1))));
+                }
+            } else {
+                int CR = R;
+                if (r2 <= (CR * CR / 16)) f = 0.0;
+                else f = 0.5 * (1 - cos(ORC_PI * ((pow((r2), 0.5) - CR / 4.0) * 4.0 / (CR * 3.0 - 1))));
+            }
+            s->tap[r2] = f;
+        }
+    }
+    return s;
+}
+
+void FN(orc_destroy)(void *h)
+{
+    SIM *s = (SIM *)h;
+    if (!s) return;
+    free(s->vp); free(s->ep); free(s->de);
+    for (int k = 0; k < 11; k++) free(s->f[k]);
+    for (int k = 0; k < 12; k++) free(s->tab[k]);
+    free(s->amp); free(s->tap);
+    free(s);
+}
+
+/* Effective eps/delta after the SV taper. */
+static inline void FN(taper)(const SIM *s, int ix, int iy, int iz, long id, REAL *ee, REAL *dd)
+{
+    *ee = s->ep[id];
+    *dd = s->de[id];
+    if (!s->p.sv) return;
+    int rx = ix - s->p.sv_cx, rz = iz - s->p.sv_cz, ry = 0;
+    if (s->p.ndim == 3) ry = iy - s->p.sv_cy;
+    int r2 = rx * rx + ry * ry + rz * rz;
+    if (r2 <= s->p.sv_R * s->p.sv_R) {
+        *ee = (REAL)(s->tap[r2] * (double)s->ep[id]);
+        *dd = (REAL)(s->tap[r2] * (double)s->de[id]);
+    }
+}
+
+#define SQRT_R(x) (sizeof(REAL) == sizeof(float) ? (REAL)sqrtf((float)(x)) : (REAL)sqrt((double)(x)))
+
+/* Velocity update.  update_vel2d GPU_vti2dfd_kernels.cu:209-238
+ * (= CPU_vti2dfd_kernels.c:79-106); update_vel3d GPU_vti3dfd_kernels.cu:90-129. */
+static void FN(vel)(SIM *s)
+{
+    const int nnx = s->nnx, nny = s->nny, nnz = s->nnz;
+    const long sx = nnz, sy = (long)nnz * nnx;
+    REAL *u = s->f[0], *v = s->f[1], *w = s->f[2];
+    const REAL *P = s->f[9], *Q = s->f[10];
+    const REAL *c = s->c;
+    const REAL *cx1 = s->tab[0], *cx2 = s->tab[1], *cy1 = s->tab[2], *cy2 = s->tab[3];
+    const REAL *cz1 = s->tab[4], *cz2 = s->tab[5];
+    const REAL dtx = s->dtx, dty = s->dty, dtz = s->dtz;
+    if (s->p.ndim == 2) {
+#pragma omp parallel for schedule(static)
+        for (int ix = 4; ix < nnx - 4; ix++)
+            for (int iz = 4; iz < nnz - 4; iz++) {
+                long id = (long)ix * nnz + iz;
+                REAL xx = 0, zz = 0;
+                for (int im = 0; im < 4; im++) {
+                    xx += c[im] * (P[id + (im + 1) * sx] - P[id - im * sx]);
+                    zz += c[im] * (Q[id + im + 1] - Q[id - im]);
+                }
+                u[id] = cx2[ix] * u[id] - cx1[ix] * dtx * xx;
+                w[id] = cz2[iz] * w[id] - cz1[iz] * dtz * zz;
+            }
+    } else {
+#pragma omp parallel for schedule(static)
+        for (int iy = 4; iy < nny - 4; iy++)
+            for (int ix = 4; ix < nnx - 4; ix++)
+                for (int iz = 4; iz < nnz - 4; iz++) {
+                    long id = iz + (long)ix * nnz + (long)iy * sy;
+                    REAL xx = 0, yy = 0, zz = 0;
+                    for (int im = 0; im < 4; im++) {
+                        yy += c[im] * (P[id + (im + 1) * sy] - P[id - im * sy]);
+                        xx += c[im] * (P[id + (im + 1) * sx] - P[id - im * sx]);
+                        zz += c[im] * (Q[id + im + 1] - Q[id - im]);
+                    }
+                    u[id] = cx2[ix] * u[id] - cx1[ix] * dtx * xx;
+                    v[id] = cy2[iy] * v[id] - cy1[iy] * dty * yy;
+                    w[id] = cz2[iz] * w[id] - cz1[iz] * dtz * zz;
+                }
+    }
+}
+
+/* Stress update.  update_stress2d GPU_vti2dfd_kernels.cu:240-299
+ * (= CPU_vti2dfd_kernels.c:108-177 with P=px+pz+s at :173-174);
+ * update_stress3d GPU_vti3dfd_kernels.cu:131-201. */
+static void FN(stress)(SIM *s, int it)
+{
+    const int nnx = s->nnx, nny = s->nny, nnz = s->nnz;
+    const long sx = nnz, sy = (long)nnz * nnx;
+    const REAL *u = s->f[0], *v = s->f[1], *w = s->f[2];
+    REAL *px = s->f[3], *py = s->f[4], *pz = s->f[5];
+    REAL *qx = s->f[6], *qy = s->f[7], *qz = s->f[8];
+    REAL *P = s->f[9], *Q = s->f[10];
+    const REAL *c = s->c, *vp = s->vp;
+    const REAL *ax1 = s->tab[6], *ax2 = s->tab[7], *ay1 = s->tab[8], *ay2 = s->tab[9];
+    const REAL *az1 = s->tab[10], *az2 = s->tab[11];
+    const REAL dtx = s->dtx, dty = s->dty, dtz = s->dtz;
+    if (s->p.ndim == 2) {
+#pragma omp parallel for schedule(static)
+        for (int ix = 4; ix < nnx - 4; ix++)
+            for (int iz = 4; iz < nnz - 4; iz++) {
+                long id = (long)ix * nnz + iz;
+                REAL ee, dd;
+                FN(taper)(s, ix, 0, iz, id, &ee, &dd);
+                REAL xx = 0, zz = 0;
+                for (int im = 0; im < 4; im++) {
+                    xx += c[im] * (u[id + im * sx] - u[id - (im + 1) * sx]);
+                    zz += c[im] * (w[id + im] - w[id - im - 1]);
+                }
+                REAL vv = vp[id];
+                REAL e2 = 1 + 2 * ee;
+                REAL sd = SQRT_R(1 + 2 * dd);
+                px[id] = ax2[ix] * px[id] - ax1[ix] * vv * vv * e2 * dtx * xx;
+                pz[id] = az2[iz] * pz[id] - az1[iz] * vv * vv * sd * dtz * zz;
+                qx[id] = ax2[ix] * qx[id] - ax1[ix] * vv * vv * sd * dtx * xx;
+                qz[id] = az2[iz] * qz[id] - az1[iz] * vv * vv * dtz * zz;
+                P[id] = px[id] + pz[id];
+                Q[id] = qx[id] + qz[id];
+            }
+    } else {
+#pragma omp parallel for schedule(static)
+        for (int iy = 4; iy < nny - 4; iy++)
+            for (int ix = 4; ix < nnx - 4; ix++)
+                for (int iz = 4; iz < nnz - 4; iz++) {
+                    long id = iz + (long)ix * nnz + (long)iy * sy;
+                    REAL ee, dd;
+                    FN(taper)(s, ix, iy, iz, id, &ee, &dd);
+                    REAL xx = 0, yy = 0, zz = 0;
+                    for (int im = 0; im < 4; im++) {
+                        yy += c[im] * (v[id + im * sy] - v[id - (im + 1) * sy]);
+                        xx += c[im] * (u[id + im * sx] - u[id - (im + 1) * sx]);
+                        zz += c[im] * (w[id + im] - w[id - im - 1]);
+                    }
+                    REAL vv = vp[id];
+                    REAL e2 = 1 + 2 * ee;
+                    REAL sd = SQRT_R(1 + 2 * dd);
+                    px[id] = ax2[ix] * px[id] - ax1[ix] * vv * vv * e2 * dtx * xx;
+                    py[id] = ay2[iy] * py[id] - ay1[iy] * vv * vv * e2 * dty * yy;
+                    pz[id] = az2[iz] * pz[id] - az1[iz] * vv * vv * sd * dtz * zz;
+                    qx[id] = ax2[ix] * qx[id] - ax1[ix] * vv * vv * sd * dtx * xx;
+                    qy[id] = ay2[iy] * qy[id] - ay1[iy] * vv * vv * sd * dty * yy;
+                    qz[id] = az2[iz] * qz[id] - az1[iz] * vv * vv * dtz * zz;
+                    P[id] = px[id] + py[id] + pz[id];
+                    Q[id] = qx[id] + qy[id] + qz[id];
+                }
+    }
+    /* In-stress Gaussian source: P = px+pz+s, Q = qx+qz+s (only inside the
+     * updated region, where s is nonzero only on the footprint). */
+    if (s->p.src_kind == 1 && s->amp[it] != 0) {
+        for (int k = 0; k < s->ng; k++) {
+            long id = s->gidx[k];
+            REAL sv = (REAL)((double)s->amp[it] * s->gw[k]);
+            if (s->p.ndim == 2)
+                P[id] = px[id] + pz[id] + sv, Q[id] = qx[id] + qz[id] + sv;
+            else
+                P[id] = px[id] + py[id] + pz[id] + sv, Q[id] = qx[id] + qy[id] + qz[id] + sv;
+        }
+    }
+}
+
+/* Receiver gather after the stress update: shot_record2d
+ * GPU_vti2dfd_kernels.cu:141-149 (P), CPU_vti2dfd_kernels.c:488-491 (P+P),
+ * 3D RTM shot_record Toa_gpu_3dvti_rtm_adcigs_1orderfunction.cu:331-332 (P+Q). */
+static void FN(record)(const SIM *s, REAL *rec)
+{
+    const orc_params *p = &s->p;
+    const REAL *P = s->f[9], *Q = s->f[10];
+    int nry = p->ndim == 3 ? p->rec_ny : 1;
+    for (int iry = 0; iry < nry; iry++)
+        for (int irx = 0; irx < p->rec_nx; irx++) {
+            int ix = p->rec_x0 + irx * p->rec_dx;
+            int iy = p->ndim == 3 ? p->rec_y0 + iry * p->rec_dy : 0;
+            long id = p->rec_z + (long)ix * s->nnz + (long)iy * s->nnz * s->nnx;
+            REAL val = P[id];
+            if (p->rec_kind == 1) val = P[id] + P[id];
+            else if (p->rec_kind == 2) val = P[id] + Q[id];
+            rec[irx + (long)p->rec_nx * iry] = val;
+        }
+}
+
+int FN(orc_step)(void *h, int it, REAL *rec)
+{
+    SIM *s = (SIM *)h;
+    if (!s || it < 0 || it >= s->p.nt) return -1;
+    if (s->p.src_kind == 0) {
+        /* add_source2d GPU_vti2dfd_kernels.cu:200-206: before update_vel */
+        long id = s->p.sz + (long)s->p.sx * s->nnz + (long)s->p.sy * s->nnz * s->nnx * (s->p.ndim == 3);
+        REAL a = s->amp[it];
+        if (a != 0) {
+            s->f[9][id] += a;
+            s->f[10][id] += a;
+        }
+    }
+    FN(vel)(s);
+    FN(stress)(s, it);
+    if (rec) FN(record)(s, rec);
+    return 0;
+}
+
+int FN(orc_run)(void *h, REAL *shot)
+{
+    SIM *s = (SIM *)h;
+    if (!s) return -1;
+    const orc_params *p = &s->p;
+    long nrec = (long)p->rec_nx * (p->ndim == 3 ? p->rec_ny : 1);
+    REAL *row = (REAL *)malloc((nrec > 0 ? nrec : 1) * sizeof(REAL));
+    for (int it = 0; it < p->nt; it++) {
+        FN(orc_step)(h, it, row);
+        if (shot)
+            for (long r = 0; r < nrec; r++) shot[it + (long)p->nt * r] = row[r];
+    }
+    free(row);
+    return 0;
+}
+
+static REAL *FN(select)(SIM *s, int which, long *len)
+{
+    if (which >= ORC_U && which <= ORC_Q) { *len = s->n; return s->f[which]; }
+    if (which == ORC_VP) { *len = s->n; return s->vp; }
+    if (which == ORC_EPS) { *len = s->n; return s->ep; }
+    if (which == ORC_DEL) { *len = s->n; return s->de; }
+    if (which >= ORC_COFFX1 && which <= ORC_ACOFFZ2) {
+        int k = which - ORC_COFFX1;
+        int axis = (k % 6) / 2;   /* 0 x, 1 y, 2 z */
+        *len = axis == 0 ? s->nnx : (axis == 1 ? s->nny : s->nnz);
+        return s->tab[k];
+    }
+    if (which == ORC_SRCAMP) { *len = s->p.nt; return s->amp; }
+    if (which == ORC_D0) { *len = 1; return &s->d0; }
+    *len = 0;
+    return NULL;
+}
+
+int FN(orc_get)(void *h, int which, REAL *out)
+{
+    long len;
+    REAL *src = FN(select)((SIM *)h, which, &len);
+    if (!src) return -1;
+    memcpy(out, src, len * sizeof(REAL));
+    return (int)(len > 0x7fffffff ? 0x7fffffff : len);
+}
+
+int FN(orc_set)(void *h, int which, const REAL *in)
+{
+    long len;
+    REAL *dst = FN(select)((SIM *)h, which, &len);
+    if (!dst) return -1;
+    memcpy(dst, in, len * sizeof(REAL));
+    return 0;
+}
+
+#undef SIM
+#undef FN
+#undef SQRT_R

@@ -1,0 +1,218 @@
+"""GPU parity of the HIP FD path (libvti.so through its C ABI) against the
+oracle and the reference golden vectors.  The bar is bit-exact equality for
+the FP32 path (same float operations in the same order); rel-L2 <= 1e-5 is
+asserted as well and the error against the FP64 oracle is reported."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import load_golden, layered_model, rel_l2, smooth_model3d
+from geophysics_amd import vti
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("name", ["cpu2d_sv0", "cpu2d_sv1", "cpu2d_two_shots", "cpu2d_aniso_grid"])
+def test_cpu_mode_matches_reference_golden_bitwise(name):
+    g = load_golden(name)
+    nx, nz, dx, dz, npd, SV, favg, ns, fs, ds, zs, nt, dt_us = [int(v) for v in g["args"]]
+    _say("case", name, nx, nz, nt)
+    cfg = vti.cfg_ref2d(vti.MODE_CPU_REF, nx, nz, dx, dz, npd, SV, favg, nt, dt_us)
+    with vti.Propagator(cfg, g["vp"], g["eps"], g["dl"]) as prop:
+        for is_ in range(1, ns + 1):
+            prop.set_shot(vti.shot_ref2d(cfg, is_, fs, ds, zs))
+            prop.reset()
+            if is_ == 1:
+                snap_it = int(g["snap_it"])
+                prop.step(0, snap_it + 1)
+                P = prop.field("P")
+                assert np.array_equal(P, g["snap"]), f"snapshot rel-L2 {rel_l2(P, g['snap']):.3e}"
+                prop.step(snap_it + 1, nt - snap_it - 1)
+            else:
+                prop.step(0, nt)
+            rec = prop.record()
+            ref = g["shot"][is_ - 1]
+            err = rel_l2(rec, ref)
+            assert err <= 1e-5, err
+            assert np.array_equal(rec, ref), f"shot {is_} not bitwise: rel-L2 {err:.3e}"
+
+
+@pytest.mark.parametrize("SV", [0, 1])
+def test_gpu_mode_2d_matches_oracle_bitwise(SV):
+    nx, nz, npd, nt = 150, 120, 20, 700
+    vp, e, d = layered_model(nx, nz, seed=11 + SV, elo=0.1, ehi=0.2, dlo=0.0, dhi=0.1)
+    cfg = vti.cfg_ref2d(vti.MODE_GPU_REF, nx, nz, 10, 10, npd, SV, 20, nt, 1000)
+    fs, zs = 70, 3
+    with vti.Propagator(cfg, vp, e, d) as prop:
+        prop.set_shot(vti.shot_ref2d(cfg, 1, fs, 0, zs))
+        prop.run()
+        rec = prop.record()
+        fields = {k: prop.field(k) for k in ("u", "w", "px", "pz", "qx", "qz", "P", "Q")}
+        prop.mute(fs, 0, zs, 1, 100)
+        muted = prop.record()
+    p = po.params_gpu2d(nx, nz, 10, 10, npd, SV, 20, 1, fs, 0, zs, nt, 1000)
+    sim = po.Sim(p, vp, e, d, "f32")
+    ref = sim.run()
+    assert np.array_equal(rec, ref), rel_l2(rec, ref)
+    for k, v in fields.items():
+        assert np.array_equal(v, sim.field(k)), k
+    ref_m = po.mute2d(ref.copy(), nt, p.dt, p.favg, p.dx, p.dz, fs, 0, zs, 1,
+                      float(sim.field("vp")[0, 1]), float(sim.field("eps")[0, 1]), 100)
+    assert np.array_equal(muted, ref_m)
+    s64 = po.Sim(p, vp, e, d, "f64").run()
+    _say("2D GPU-mode SV=%d: rel-L2 vs FP64 oracle %.3e" % (SV, rel_l2(rec, s64)))
+
+
+def test_3d_matches_oracle_bitwise():
+    nx, ny, nz, npml, nt = 44, 40, 36, 8, 160
+    vp, e, d = smooth_model3d(nx, ny, nz, seed=5)
+    cfg = vti.cfg_ref3d(nx, ny, nz, 10, 10, 10, npml, 1, 25, nt, 800)
+    args = (1, nx // 2, 0, ny // 2, 0, 2, 0)
+    with vti.Propagator(cfg, vp, e, d) as prop:
+        prop.set_shot(vti.shot_ref3d(cfg, 0, *args[1:], rec_depth=3, rec_step=2))
+        prop.run()
+        rec = prop.record()
+        fields = {k: prop.field(k) for k in ("u", "v", "w", "px", "py", "pz", "qx", "qy", "qz", "P", "Q")}
+    p = po.params_gpu3d(nx, ny, nz, 10, 10, 10, npml, 1, 25, 0, nx // 2, 0, ny // 2, 0, 2, 0,
+                        nt, 800, 3, 2)
+    sim = po.Sim(p, vp, e, d, "f32")
+    ref = sim.run()
+    assert rec.shape == ref.shape
+    assert np.array_equal(rec, ref), rel_l2(rec, ref)
+    for k, v in fields.items():
+        assert np.array_equal(v, sim.field(k)), k
+    s64 = po.Sim(p, vp, e, d, "f64").run()
+    _say("3D: rel-L2 vs FP64 oracle %.3e" % rel_l2(rec, s64))
+
+
+def test_iso_gives_P_equal_Q():
+    """eps = delta = 0 reduces VTI to ISO: the P and Q systems coincide."""
+    nx, nz = 90, 70
+    vp, _, _ = layered_model(nx, nz, seed=2)
+    z = np.zeros_like(vp)
+    cfg = vti.cfg_ref2d(vti.MODE_GPU_REF, nx, nz, 10, 10, 15, 1, 20, 300, 1000)
+    with vti.Propagator(cfg, vp, z, z) as prop:
+        prop.set_shot(vti.shot_ref2d(cfg, 1, 45, 0, 2))
+        prop.run()
+        assert np.array_equal(prop.field("P"), prop.field("Q"))
+        assert np.abs(prop.field("P")).max() > 0
+    vp3, _, _ = smooth_model3d(30, 28, 26, seed=1)
+    z3 = np.zeros_like(vp3)
+    cfg3 = vti.cfg_ref3d(30, 28, 26, 10, 10, 10, 6, 1, 25, 80, 800)
+    with vti.Propagator(cfg3, vp3, z3, z3) as prop:
+        prop.set_shot(vti.shot_ref3d(cfg3, 0, 15, 0, 14, 0, 2, 0))
+        prop.run()
+        assert np.array_equal(prop.field("P"), prop.field("Q"))
+
+
+def test_device_tables_equal_oracle():
+    nx, nz, npd = 100, 80, 20
+    vp, e, d = layered_model(nx, nz, seed=4)
+    for mode, mk in ((vti.MODE_GPU_REF, po.params_gpu2d), (vti.MODE_CPU_REF, po.params_cpu2d)):
+        cfg = vti.cfg_ref2d(mode, nx, nz, 10, 10, npd, 1, 20, 500, 1000)
+        sim = po.Sim(mk(nx, nz, 10, 10, npd, 1, 20, 1, 50, 0, 2, 500, 1000), vp, e, d, "f32")
+        with vti.Propagator(cfg, vp, e, d) as prop:
+            prop.set_shot(vti.shot_ref2d(cfg, 1, 50, 0, 2))
+            assert np.float32(prop.d0()) == np.float32(sim.d0())
+            for name in ("coffx1", "coffx2", "coffz1", "coffz2", "acoffx1", "acoffx2", "acoffz1", "acoffz2"):
+                assert np.array_equal(prop.table(name), sim.table(name)), name
+            assert np.array_equal(prop.srcamp(), sim.srcamp())
+            for k in ("vp", "eps", "del"):
+                assert np.array_equal(prop.field(k), sim.field(k)), k
+
+
+def test_error_paths():
+    nx, nz = 60, 50
+    vp, e, d = layered_model(nx, nz, seed=3)
+    cfg = vti.cfg_ref2d(vti.MODE_GPU_REF, nx, nz, 10, 10, 10, 0, 20, 50, 1000)
+    with vti.Propagator(cfg) as prop:
+        with pytest.raises(vti.VtiError, match="out of order"):
+            prop.step(0, 1)
+        prop.set_model(vp, e, d)
+        bad = vti.shot_ref2d(cfg, 1, 30, 0, 2)
+        bad.sx = 2
+        with pytest.raises(vti.VtiError, match="source"):
+            prop.set_shot(bad)
+        prop.set_shot(vti.shot_ref2d(cfg, 1, 30, 0, 2))
+        with pytest.raises(vti.VtiError, match="range"):
+            prop.step(40, 20)
+        neg = vp.copy()
+        neg[3, 3] = -1.0
+        with pytest.raises(vti.VtiError, match="positive"):
+            prop.set_model(neg, e, d)
+
+
+def _write(td, **arrs):
+    for k, a in arrs.items():
+        np.ascontiguousarray(a, dtype=np.float32).tofile(os.path.join(td, k))
+    return [os.path.join(td, k) for k in arrs]
+
+
+def test_dropin_cpu_vti2dfd_files_equal_golden(tmp_path):
+    g = load_golden("cpu2d_two_shots")
+    nx, nz, dx, dz, npd, SV, favg, ns, fs, ds, zs, nt, dt_us = [int(v) for v in g["args"]]
+    fv, fe, fd = _write(str(tmp_path), v=g["vp"], e=g["eps"], d=g["dl"])
+    shot, snap = str(tmp_path / "shot"), str(tmp_path / "snap")
+    vti.CPU_vti2dfd(nx, nz, dx, dz, npd, SV, fv, fe, fd, favg, ns, fs, ds, zs, shot, snap, nt, dt_us)
+    out = np.fromfile(shot, dtype=np.float32).reshape(ns, nx, nt)
+    assert np.array_equal(out, g["shot"])
+    frames = np.fromfile(snap, dtype=np.float32).reshape(-1, nx + 2 * npd, nz + 2 * npd)
+    assert np.array_equal(frames[int(g["snap_it"]) // 50], g["snap"])
+
+
+def test_dropin_gpu_vti2dfd_and_3dfd_files(tmp_path):
+    nx, nz, npd, nt = 80, 60, 12, 300
+    vp, e, d = layered_model(nx, nz, seed=8, elo=0.1, dhi=0.1)
+    fv, fe, fd = _write(str(tmp_path), v=vp, e=e, d=d)
+    shot, snap = str(tmp_path / "shot"), str(tmp_path / "snap")
+    vti.GPU_vti2dfd(nx, nz, 10, 10, npd, 1, fv, fe, fd, 20, 2, 20, 30, 2, shot, snap, nt, 1000)
+    out = np.fromfile(shot, dtype=np.float32).reshape(2, nx, nt)
+    for is_ in (1, 2):
+        p = po.params_gpu2d(nx, nz, 10, 10, npd, 1, 20, is_, 20, 30, 2, nt, 1000)
+        sim = po.Sim(p, vp, e, d, "f32")
+        ref = sim.run()
+        po.mute2d(ref, nt, p.dt, p.favg, p.dx, p.dz, 20, 30, 2, is_,
+                  float(sim.field("vp")[0, 1]), float(sim.field("eps")[0, 1]), 100)
+        assert np.array_equal(out[is_ - 1], ref), is_
+    # 3D
+    nx3, ny3, nz3, np3, nt3 = 24, 22, 20, 6, 320
+    vp3, e3, d3 = smooth_model3d(nx3, ny3, nz3, seed=9)
+    fv, fe, fd = _write(str(tmp_path), v3=vp3, e3=e3, d3=d3)
+    vti.GPU_vti3dfd(nx3, ny3, nz3, 10, 10, 10, np3, 1, fv, fe, fd, 25, 1, 12, 0, 11, 0, 2, 0,
+                    shot, snap, nt3, 800)
+    out3 = np.fromfile(shot, dtype=np.float32).reshape(ny3 * nx3, nt3)
+    p3 = po.params_gpu3d(nx3, ny3, nz3, 10, 10, 10, np3, 1, 25, 0, 12, 0, 11, 0, 2, 0, nt3, 800, 1, 1)
+    sim3 = po.Sim(p3, vp3, e3, d3, "f32")
+    ref3 = sim3.run()
+    po.mute3d(ref3, nx3, ny3, nt3, p3.dt, p3.favg, 10, 10, 10, 12, 11, 0, 0, 2, 0,
+              float(sim3.field("vp")[0, 0, 1]), float(sim3.field("eps")[0, 0, 1]), 100)
+    assert np.array_equal(out3, ref3)
+    snap3 = np.fromfile(snap, dtype=np.float32).reshape(-1, ny3, nx3, nz3)
+    assert snap3.shape[0] == 1
+
+
+def test_config1_full_size_parity():
+    """2D benchmark configuration: 400x400, 2000 steps, 20-cell boundary,
+    20 Hz Ricker at (x=200, z=2), receivers at every x at z=2."""
+    nx = nz = 400
+    npd, nt = 20, 2000
+    vp, e, d = layered_model(nx, nz, seed=2024, elo=0.1, ehi=0.2, dlo=0.0, dhi=0.1)
+    cfg = vti.cfg_ref2d(vti.MODE_GPU_REF, nx, nz, 10, 10, npd, 0, 20, nt, 1000)
+    with vti.Propagator(cfg, vp, e, d) as prop:
+        prop.set_shot(vti.shot_ref2d(cfg, 1, 200, 0, 2, rec_depth=2))
+        prop.run()
+        rec = prop.record()
+    p = po.params_gpu2d(nx, nz, 10, 10, npd, 0, 20, 1, 200, 0, 2, nt, 1000, 2)
+    ref = po.Sim(p, vp, e, d, "f32").run()
+    err = rel_l2(rec, ref)
+    _say("config-1 rel-L2 GPU vs FP32 oracle %.3e, bitwise %s" % (err, np.array_equal(rec, ref)))
+    assert err <= 1e-5
+    assert np.array_equal(rec, ref)
+    s64 = po.Sim(p, vp, e, d, "f64").run()
+    _say("config-1 rel-L2 GPU vs FP64 oracle %.3e" % rel_l2(rec, s64))
