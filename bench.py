@@ -36,19 +36,23 @@ def log(*a):
 
 
 def gaussian_smooth(x, sigma):
-    """Separable Gaussian filter (reflect padding) of a (Y, X, Z) cuda tensor."""
+    """Separable Gaussian filter (reflect padding) of a (Y, X, Z) cuda tensor,
+    as a sum of shifted slices (elementwise kernels only)."""
     import torch
-    import torch.nn.functional as F
     r = int(4 * sigma + 0.5)
-    k = torch.arange(-r, r + 1, device=x.device, dtype=torch.float32)
+    k = torch.arange(-r, r + 1, dtype=torch.float64)
     k = torch.exp(-0.5 * (k / sigma) ** 2)
-    k = (k / k.sum()).view(1, 1, -1)
+    k = (k / k.sum()).tolist()
     for dim in (2, 1, 0):
-        y = x.movedim(dim, -1)
-        shp = y.shape
-        y = y.reshape(-1, 1, shp[-1])
-        y = F.conv1d(F.pad(y, (r, r), mode="reflect"), k)
-        x = y.reshape(shp).movedim(-1, dim).contiguous()
+        n = x.shape[dim]
+        lo = x.narrow(dim, 1, r).flip(dim)
+        hi = x.narrow(dim, n - 1 - r, r).flip(dim)
+        xp = torch.cat([lo, x, hi], dim=dim)
+        acc = torch.zeros_like(x)
+        for j in range(2 * r + 1):
+            acc.add_(xp.narrow(dim, j, n), alpha=k[j])
+        del xp
+        x = acc
     return x
 
 
@@ -99,6 +103,30 @@ def cpu_baseline(vp, eps, dl, n, npml, nt_total, budget_s=12.0):
                       f"{steps} steps after the first, {threads} OpenMP threads, {dt:.1f} s"}
 
 
+def copy_calibration(torch, vti, nbytes=4 << 30, reps=20):
+    """Plain copy of nbytes with the library's copy kernel: the achievable
+    HBM rate and the known-size reference for FETCH_SIZE / WRITE_SIZE."""
+    import ctypes
+    src = torch.ones(nbytes // 4, device="cuda", dtype=torch.float32)
+    dst = torch.empty_like(src)
+    s = torch.cuda.current_stream().cuda_stream
+    L = vti.lib()
+    L.vti_calib_copy(ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src.data_ptr()), nbytes,
+                     ctypes.c_void_p(s))
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        L.vti_calib_copy(ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src.data_ptr()), nbytes,
+                         ctypes.c_void_p(s))
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    assert torch.equal(dst[:1024], src[:1024])
+    print(json.dumps({"copy_bytes": nbytes, "ms": round(ms, 4),
+                      "read_plus_write_GBs": round(2 * nbytes / ms / 1e6, 1)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +136,8 @@ def main():
     ap.add_argument("--npml", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2025)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="time the plain device copy kernel (bandwidth / counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -119,6 +149,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
+    if args.copy_calib:
+        copy_calibration(torch, vti)
+        return
     n, npml = args.size, args.npml
     nt = max(args.steps, 2000)
     log(f"[bench] building {n}^3 model on {torch.cuda.get_device_name(0)}")

@@ -21,47 +21,15 @@ const char *last_error_cstr();
 
 using namespace vti;
 
-struct vti_ctx {
-    vti_cfg cfg{};
-    Grid g{};
-    size_t nvol = 0;            // floats per pitched volume
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    int device = 0;
-
-    float *arena = nullptr;     // all volumes
-    float *f[11] = {};          // u v w px py pz qx qy qz P Q (v/py/qy absent in 2D)
-    float *vp = nullptr, *ep = nullptr, *de = nullptr;
-    float *tab = nullptr;       // 12 damping tables, nnmax floats each
-    int nnmax = 0;
-    std::vector<float> htab;    // host copy of the tables
-    float d0 = 0.0f;
-
-    std::vector<float> amp;     // per-step source gain
-    double *dtap = nullptr;     // SV taper factors
-    int tapR = 0;
-    double *dgw = nullptr;      // 7x7 Gaussian weights
-
-    vti_shot shot{};
-    int nrec = 0;
-    float *rec = nullptr;       // [it][irec]
-    float *rec_t = nullptr;     // [irec][it] scratch
-
-    bool have_model = false, have_shot = false;
-    float vp1 = 0, eps1 = 0;    // padded vp[1], eps[1] (mute_directwave2d :167)
-
-    bool prof = false;
-    std::vector<hipEvent_t> ev_pool;
-    size_t ev_used = 0;
-};
-
-namespace {
-
+namespace vti {
 int fail(int code, const std::string &msg)
 {
     set_error(msg);
     return code;
 }
+}  // namespace vti
+
+namespace {
 
 #define HIP_TRY(expr)                                                                   \
     do {                                                                                \
@@ -87,6 +55,124 @@ hipEvent_t next_event(vti_ctx *c)
 }
 
 }  // namespace
+
+namespace vti {
+
+int state_nfield(const vti_ctx *c) { return c->cfg.ndim == 3 ? 11 : 8; }
+
+int state_alloc(vti_ctx *c, VtiState &s)
+{
+    const int nf = state_nfield(c);
+    float *p = nullptr;
+    if (hipMalloc(&p, (size_t)nf * c->nvol * sizeof(float)) != hipSuccess)
+        return fail(VTI_ERR_ALLOC, "state allocation");
+    static const int order3[11] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10};
+    static const int order2[8] = {0, 2, 3, 5, 6, 8, 9, 10};
+    s = VtiState{};
+    const int *ord = nf == 11 ? order3 : order2;
+    for (int k = 0; k < nf; ++k) s.f[ord[k]] = p + (size_t)k * c->nvol;
+    if (launch_zero(p, (size_t)nf * c->nvol, c->stream) != hipSuccess)
+        return fail(VTI_ERR_HIP, "state zero");
+    return VTI_OK;
+}
+
+void state_free(VtiState &s)
+{
+    // the first present field is the base of the single allocation
+    float *base = s.f[0];
+    if (base) (void)hipFree(base);
+    s = VtiState{};
+}
+
+hipError_t state_zero(vti_ctx *c, const VtiState &s)
+{
+    for (int k = 0; k < 11; ++k)
+        if (s.f[k]) {
+            hipError_t e = launch_zero(s.f[k], c->nvol, c->stream);
+            if (e != hipSuccess) return e;
+        }
+    return hipSuccess;
+}
+
+hipError_t state_copy(vti_ctx *c, const VtiState &dst, const VtiState &src)
+{
+    for (int k = 0; k < 11; ++k)
+        if (src.f[k]) {
+            hipError_t e = hipMemcpyAsync(dst.f[k], src.f[k], c->nvol * sizeof(float),
+                                          hipMemcpyDeviceToDevice, c->stream);
+            if (e != hipSuccess) return e;
+        }
+    return hipSuccess;
+}
+
+VtiState main_state(vti_ctx *c)
+{
+    VtiState s;
+    for (int k = 0; k < 11; ++k) s.f[k] = c->f[k];
+    return s;
+}
+
+VelArgs vel_args_for(vti_ctx *c, const VtiState &s, int it, bool inject)
+{
+    const int M = c->nnmax;
+    VelArgs a{};
+    a.g = c->g;
+    a.u = s.f[VTI_F_U]; a.v = s.f[VTI_F_V]; a.w = s.f[VTI_F_W];
+    a.P = s.f[VTI_F_P]; a.Q = s.f[VTI_F_Q];
+    a.cx1 = c->tab + 0 * M; a.cx2 = c->tab + 1 * M;
+    a.cy1 = c->tab + 2 * M; a.cy2 = c->tab + 3 * M;
+    a.cz1 = c->tab + 4 * M; a.cz2 = c->tab + 5 * M;
+    const float dt = c->cfg.dt;
+    a.dtx = dt / c->cfg.dx; a.dty = dt / c->cfg.dy; a.dtz = dt / c->cfg.dz;
+    a.src = -1;
+    a.amp = 0.0f;
+    if (inject && c->cfg.mode == VTI_MODE_GPU_REF && c->amp[it] != 0.0f) {
+        a.src = c->shot.sz + (long long)c->shot.sx * c->g.ldz + (long long)c->shot.sy * c->g.ldxy;
+        a.amp = c->amp[it];
+    }
+    return a;
+}
+
+StressArgs stress_args_for(vti_ctx *c, const VtiState &s, int it, bool inject, bool sv, float *rec_row)
+{
+    const int M = c->nnmax;
+    StressArgs a{};
+    a.g = c->g;
+    a.u = s.f[VTI_F_U]; a.v = s.f[VTI_F_V]; a.w = s.f[VTI_F_W];
+    a.px = s.f[VTI_F_PX]; a.py = s.f[VTI_F_PY]; a.pz = s.f[VTI_F_PZ];
+    a.qx = s.f[VTI_F_QX]; a.qy = s.f[VTI_F_QY]; a.qz = s.f[VTI_F_QZ];
+    a.P = s.f[VTI_F_P]; a.Q = s.f[VTI_F_Q];
+    a.vp = c->vp; a.ep = c->ep; a.de = c->de;
+    a.ax1 = c->tab + 6 * M; a.ax2 = c->tab + 7 * M;
+    a.ay1 = c->tab + 8 * M; a.ay2 = c->tab + 9 * M;
+    a.az1 = c->tab + 10 * M; a.az2 = c->tab + 11 * M;
+    const float dt = c->cfg.dt;
+    a.dtx = dt / c->cfg.dx; a.dty = dt / c->cfg.dy; a.dtz = dt / c->cfg.dz;
+    a.sv = sv ? 1 : 0;
+    a.svR = c->cfg.sv_R;
+    a.cx = c->shot.sv_cx; a.cy = c->shot.sv_cy; a.cz = c->shot.sv_cz;
+    a.tap = c->dtap;
+    a.gsrc = (inject && c->cfg.mode == VTI_MODE_CPU_REF && c->amp[it] != 0.0f) ? 1 : 0;
+    a.gx = c->shot.sx; a.gy = c->shot.sy; a.gz = c->shot.sz;
+    a.gamp = c->amp[it];
+    a.gw = c->dgw;
+    a.rec = rec_row;
+    a.rec_kind = c->cfg.rec_kind;
+    a.rz = c->shot.rec_z;
+    a.rx0 = c->shot.rec_x0; a.rdx = c->shot.rec_dx; a.rnx = c->shot.rec_nx;
+    a.ry0 = c->shot.rec_y0; a.rdy = c->shot.rec_dy; a.rny = c->shot.rec_ny;
+    return a;
+}
+
+}  // namespace vti
+
+static VelArgs vel_args(vti_ctx *c, int it) { return vel_args_for(c, main_state(c), it, true); }
+
+static StressArgs stress_args(vti_ctx *c, int it)
+{
+    float *row = c->nrec > 0 ? c->rec + (size_t)it * c->nrec : nullptr;
+    return stress_args_for(c, main_state(c), it, true, c->cfg.sv != 0, row);
+}
 
 extern "C" {
 
@@ -467,58 +553,6 @@ int vti_reset(vti_ctx *c)
     return VTI_OK;
 }
 
-static VelArgs vel_args(vti_ctx *c, int it)
-{
-    const int M = c->nnmax;
-    VelArgs a{};
-    a.g = c->g;
-    a.u = c->f[VTI_F_U]; a.v = c->f[VTI_F_V]; a.w = c->f[VTI_F_W];
-    a.P = c->f[VTI_F_P]; a.Q = c->f[VTI_F_Q];
-    a.cx1 = c->tab + 0 * M; a.cx2 = c->tab + 1 * M;
-    a.cy1 = c->tab + 2 * M; a.cy2 = c->tab + 3 * M;
-    a.cz1 = c->tab + 4 * M; a.cz2 = c->tab + 5 * M;
-    const float dt = c->cfg.dt;
-    a.dtx = dt / c->cfg.dx; a.dty = dt / c->cfg.dy; a.dtz = dt / c->cfg.dz;
-    a.src = -1;
-    a.amp = 0.0f;
-    if (c->cfg.mode == VTI_MODE_GPU_REF && c->amp[it] != 0.0f) {
-        a.src = c->shot.sz + (long long)c->shot.sx * c->g.ldz + (long long)c->shot.sy * c->g.ldxy;
-        a.amp = c->amp[it];
-    }
-    return a;
-}
-
-static StressArgs stress_args(vti_ctx *c, int it)
-{
-    const int M = c->nnmax;
-    StressArgs a{};
-    a.g = c->g;
-    a.u = c->f[VTI_F_U]; a.v = c->f[VTI_F_V]; a.w = c->f[VTI_F_W];
-    a.px = c->f[VTI_F_PX]; a.py = c->f[VTI_F_PY]; a.pz = c->f[VTI_F_PZ];
-    a.qx = c->f[VTI_F_QX]; a.qy = c->f[VTI_F_QY]; a.qz = c->f[VTI_F_QZ];
-    a.P = c->f[VTI_F_P]; a.Q = c->f[VTI_F_Q];
-    a.vp = c->vp; a.ep = c->ep; a.de = c->de;
-    a.ax1 = c->tab + 6 * M; a.ax2 = c->tab + 7 * M;
-    a.ay1 = c->tab + 8 * M; a.ay2 = c->tab + 9 * M;
-    a.az1 = c->tab + 10 * M; a.az2 = c->tab + 11 * M;
-    const float dt = c->cfg.dt;
-    a.dtx = dt / c->cfg.dx; a.dty = dt / c->cfg.dy; a.dtz = dt / c->cfg.dz;
-    a.sv = c->cfg.sv ? 1 : 0;
-    a.svR = c->cfg.sv_R;
-    a.cx = c->shot.sv_cx; a.cy = c->shot.sv_cy; a.cz = c->shot.sv_cz;
-    a.tap = c->dtap;
-    a.gsrc = (c->cfg.mode == VTI_MODE_CPU_REF && c->amp[it] != 0.0f) ? 1 : 0;
-    a.gx = c->shot.sx; a.gy = c->shot.sy; a.gz = c->shot.sz;
-    a.gamp = c->amp[it];
-    a.gw = c->dgw;
-    a.rec = c->nrec > 0 ? c->rec + (size_t)it * c->nrec : nullptr;
-    a.rec_kind = c->cfg.rec_kind;
-    a.rz = c->shot.rec_z;
-    a.rx0 = c->shot.rec_x0; a.rdx = c->shot.rec_dx; a.rnx = c->shot.rec_nx;
-    a.ry0 = c->shot.rec_y0; a.rdy = c->shot.rec_dy; a.rny = c->shot.rec_ny;
-    return a;
-}
-
 static int check_step(vti_ctx *c, int it)
 {
     if (!c) return fail(VTI_ERR_ARG, "null ctx");
@@ -586,30 +620,40 @@ float *vti_record_device(vti_ctx *c) { return c ? c->rec : nullptr; }
 
 // Mutes operate on the [it][irec] buffer by transposing to [irec][it],
 // muting there with the reference index arithmetic, and transposing back.
-static int mute_common(vti_ctx *c, bool three, int a0, int a1, int a2, int a3, int a4, int is, int tt)
+static int mute_common(vti_ctx *c, int form, int a0, int a1, int a2, int a3, int a4, int is, int tt)
 {
     if (!c || !c->rec) return fail(VTI_ERR_STATE, "vti_mute: no receiver buffer");
-    const int nt = c->cfg.nt, nrec = c->nrec;
+    const int nt = c->cfg.nt, nrec = c->nrec, np = c->cfg.npml;
+    const vti_shot &s = c->shot;
     HIP_TRY(launch_transpose(c->rec, c->rec_t, nt, nrec, c->stream));
-    if (!three)
+    if (form == 0)
         HIP_TRY(launch_mute2d(c->rec_t, nrec, nt, c->cfg.dt, c->cfg.favg, c->cfg.dx, c->cfg.dz,
-                              a0, a1, a2, is, c->vp1, c->eps1, tt, c->stream));
+                              a0, a1, a2, is, c->vp1, c->eps1, tt, s.rec_x0 - np, s.rec_dx, c->stream));
     else
-        HIP_TRY(launch_mute3d(c->rec_t, c->shot.rec_nx, c->shot.rec_ny, nt, c->cfg.dt, c->cfg.favg,
+        HIP_TRY(launch_mute3d(c->rec_t, s.rec_nx, s.rec_ny, nt, c->cfg.dt, c->cfg.favg,
                               c->cfg.dx, c->cfg.dy, c->cfg.dz, a0, a1, a2, a3, a4, is, c->vp1,
-                              c->eps1, tt, c->stream));
+                              c->eps1, tt, form == 2, s.rec_x0 - np, s.rec_dx, s.rec_y0 - np,
+                              s.rec_dy, c->stream));
     HIP_TRY(launch_transpose(c->rec_t, c->rec, nrec, nt, c->stream));
     return VTI_OK;
 }
 
 int vti_mute(vti_ctx *c, int fs, int ds, int zs, int is, int tt)
 {
-    return mute_common(c, false, fs, ds, zs, 0, 0, is, tt);
+    if (c && c->cfg.ndim != 2) return fail(VTI_ERR_ARG, "vti_mute: 2D only");
+    return mute_common(c, 0, fs, ds, zs, 0, 0, is, tt);
 }
 
 int vti_mute3d(vti_ctx *c, int fsx, int dsx, int fsy, int dsy, int fsz, int is, int tt)
 {
-    return mute_common(c, true, fsx, dsx, fsy, dsy, fsz, is, tt);
+    if (c && c->cfg.ndim != 3) return fail(VTI_ERR_ARG, "vti_mute3d: 3D only");
+    return mute_common(c, 1, fsx, dsx, fsy, dsy, fsz, is, tt);
+}
+
+int vti_mute3d_window(vti_ctx *c, int fsx, int dsx, int fsy, int dsy, int fsz, int is, int tt)
+{
+    if (c && c->cfg.ndim != 3) return fail(VTI_ERR_ARG, "vti_mute3d_window: 3D only");
+    return mute_common(c, 2, fsx, dsx, fsy, dsy, fsz, is, tt);
 }
 
 int vti_get_record(vti_ctx *c, float *dst)

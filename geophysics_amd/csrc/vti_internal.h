@@ -67,11 +67,12 @@ hipError_t launch_transpose(const float *in, float *out, int rows, int cols,
                             hipStream_t s);
 hipError_t launch_mute2d(float *rec_t, int nrec, int nt, float dt, float favg,
                          float dx, float dz, int fs, int ds, int zs, int is,
-                         float vp1, float eps1, int tt, hipStream_t s);
-hipError_t launch_mute3d(float *rec_t, int nx, int ny, int nt, float dt,
+                         float vp1, float eps1, int tt, int x0, int rdx, hipStream_t s);
+hipError_t launch_mute3d(float *rec_t, int nrx, int nry, int nt, float dt,
                          float favg, float dx, float dy, float dz, int fsx,
                          int dsx, int fsy, int dsy, int fsz, int is,
-                         float vp1, float eps1, int tt, hipStream_t s);
+                         float vp1, float eps1, int tt, int window,
+                         int x0, int rdx, int y0, int rdy, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, size_t nbytes, hipStream_t s);
 
 // ---- host table construction (vti_tables.cpp) ---------------------------
@@ -83,5 +84,68 @@ void taper_table(int cpu_form, int R, int r, std::vector<double> &tap);
 void gaussian_weights(double w[49]);
 
 void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
 
+}  // namespace vti
+
+// One set of wavefield volumes: u v w px py pz qx qy qz P Q (v/py/qy are
+// null in 2D), each Grid-pitched.
+struct VtiState {
+    float *f[11] = {};
+};
+
+// Context: model, tables, the main wavefield state and per-shot buffers.
+struct vti_ctx {
+    vti_cfg cfg{};
+    vti::Grid g{};
+    size_t nvol = 0;            // floats per pitched volume
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = 0;
+
+    float *arena = nullptr;     // model + main state volumes
+    float *f[11] = {};          // main state: u v w px py pz qx qy qz P Q
+    float *vp = nullptr, *ep = nullptr, *de = nullptr;
+    float *tab = nullptr;       // 12 damping tables, nnmax floats each
+    int nnmax = 0;
+    std::vector<float> htab;    // host copy of the tables
+    float d0 = 0.0f;
+
+    std::vector<float> amp;     // per-step source gain
+    double *dtap = nullptr;     // SV taper factors
+    int tapR = 0;
+    double *dgw = nullptr;      // 7x7 Gaussian weights
+
+    vti_shot shot{};
+    int nrec = 0;
+    float *rec = nullptr;       // [it][irec]
+    float *rec_t = nullptr;     // [irec][it] scratch
+
+    bool have_model = false, have_shot = false;
+    float vp1 = 0, eps1 = 0;    // padded vp[1], eps[1] (mute_directwave2d :167)
+
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+
+    // imaging (RTM): interior-sized image and illumination, accumulated over shots
+    float *img = nullptr, *illum = nullptr;
+    size_t nint = 0;            // nx*ny*nz
+    int rtm_illum_mode = 0;
+};
+
+namespace vti {
+// number of state volumes per state (8 in 2D, 11 in 3D)
+int state_nfield(const vti_ctx *c);
+// allocate / free an extra state (zeroed)
+int state_alloc(vti_ctx *c, VtiState &s);
+void state_free(VtiState &s);
+hipError_t state_zero(vti_ctx *c, const VtiState &s);
+hipError_t state_copy(vti_ctx *c, const VtiState &dst, const VtiState &src);
+VtiState main_state(vti_ctx *c);
+// kernel arguments for one step of state s; inject: apply the shot's source
+// (point before the velocity update in GPU mode, footprint in the stress
+// update in CPU mode); sv: SV taper; rec_row: receiver row of this step or null
+VelArgs vel_args_for(vti_ctx *c, const VtiState &s, int it, bool inject);
+StressArgs stress_args_for(vti_ctx *c, const VtiState &s, int it, bool inject, bool sv, float *rec_row);
 }  // namespace vti

@@ -291,13 +291,16 @@ __global__ void transpose_kernel(const float *__restrict__ in, float *__restrict
     }
 }
 
-// mute_directwave2d GPU_vti2dfd_kernels.cu:151-174 on a [irec][it] record.
-__global__ void mute2d_kernel(float *shot, int nx, int nt, float dt, float favg, float dx,
-                              float dz, int fs, int ds, int zs, int is, float vp1, float eps1, int tt)
+// mute_directwave2d GPU_vti2dfd_kernels.cu:151-174 on a [irec][it] record;
+// receiver r sits at model column x0 + r*rdx (the reference: x0 = 0, rdx = 1).
+__global__ void mute2d_kernel(float *shot, int nrec, int nt, float dt, float favg, float dx,
+                              float dz, int fs, int ds, int zs, int is, float vp1, float eps1, int tt,
+                              int x0, int rdx)
 {
     const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (id >= (long long)nx * nt) return;
-    const int ix = (int)(id / nt), it = (int)(id % nt);
+    if (id >= (long long)nrec * nt) return;
+    const int r = (int)(id / nt), it = (int)(id % nt);
+    const int ix = x0 + r * rdx;
     const float mu_x = dx * abs(ix - fs - (is - 1) * ds);
     const float mu_z = dz * zs;
     const float mu_t0 = sqrtf(mu_x * mu_x + mu_z * mu_z) / (vp1 * sqrtf(1 + 2 * eps1));
@@ -306,23 +309,28 @@ __global__ void mute2d_kernel(float *shot, int nx, int nt, float dt, float favg,
     if ((it > (int)(mu_t0 / dt) - tt) && it < mu_nt) shot[id] = 0.0f;
 }
 
-// mute_directwave3d GPU_vti3dfd_kernels.cu:342-369 on a [iy][ix][it] record.
-__global__ void mute3d_kernel(float *shot, int nx, int ny, int nt, float dt, float favg,
+// mute_directwave3d GPU_vti3dfd_kernels.cu:342-369 (window = 0: every sample
+// before the cut) and the windowed form of the 3D RTM main
+// (Toa_gpu_3dvti_rtm_adcigs_1orderfunction.cu:371-380, window = 1), on a
+// [iry][irx][it] record; receiver (irx, iry) at model (x0+irx*rdx, y0+iry*rdy).
+__global__ void mute3d_kernel(float *shot, int nrx, int nry, int nt, float dt, float favg,
                               float dx, float dy, float dz, int fsx, int dsx, int fsy,
-                              int dsy, int fsz, int is, float vp1, float eps1, int tt)
+                              int dsy, int fsz, int is, float vp1, float eps1, int tt, int window,
+                              int x0, int rdx, int y0, int rdy)
 {
     const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (id >= (long long)nx * ny * nt) return;
+    if (id >= (long long)nrx * nry * nt) return;
     const int it = (int)(id % nt);
     const long long r = id / nt;
-    const int ix = (int)(r % nx), iy = (int)(r / nx);
+    const int ix = x0 + (int)(r % nrx) * rdx, iy = y0 + (int)(r / nrx) * rdy;
     const float mu_x = dx * abs(ix - fsx - is * dsx);
     const float mu_y = dy * abs(iy - fsy - is * dsy);
     const float mu_z = dz * fsz;
     const float mu_t0 = sqrtf(mu_x * mu_x + mu_y * mu_y + mu_z * mu_z) / (vp1 * sqrtf(1 + 2 * eps1));
     const int mu_t = (int)(2.0 / (dt * favg));
     const int mu_nt = (int)(mu_t0 / dt) + mu_t + tt;
-    if (it < mu_nt) shot[id] = 0.0f;
+    const bool lo = !window || (it > (int)(mu_t0 / dt) - tt);
+    if (lo && it < mu_nt) shot[id] = 0.0f;
 }
 
 __global__ void copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n4)
@@ -396,22 +404,23 @@ hipError_t launch_transpose(const float *in, float *out, int rows, int cols, hip
 
 hipError_t launch_mute2d(float *rec_t, int nrec, int nt, float dt, float favg, float dx,
                          float dz, int fs, int ds, int zs, int is, float vp1, float eps1,
-                         int tt, hipStream_t s)
+                         int tt, int x0, int rdx, hipStream_t s)
 {
     const long long n = (long long)nrec * nt;
     hipLaunchKernelGGL(mute2d_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                       rec_t, nrec, nt, dt, favg, dx, dz, fs, ds, zs, is, vp1, eps1, tt);
+                       rec_t, nrec, nt, dt, favg, dx, dz, fs, ds, zs, is, vp1, eps1, tt, x0, rdx);
     return hipGetLastError();
 }
 
-hipError_t launch_mute3d(float *rec_t, int nx, int ny, int nt, float dt, float favg,
+hipError_t launch_mute3d(float *rec_t, int nrx, int nry, int nt, float dt, float favg,
                          float dx, float dy, float dz, int fsx, int dsx, int fsy, int dsy,
-                         int fsz, int is, float vp1, float eps1, int tt, hipStream_t s)
+                         int fsz, int is, float vp1, float eps1, int tt, int window,
+                         int x0, int rdx, int y0, int rdy, hipStream_t s)
 {
-    const long long n = (long long)nx * ny * nt;
+    const long long n = (long long)nrx * nry * nt;
     hipLaunchKernelGGL(mute3d_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                       rec_t, nx, ny, nt, dt, favg, dx, dy, dz, fsx, dsx, fsy, dsy, fsz, is,
-                       vp1, eps1, tt);
+                       rec_t, nrx, nry, nt, dt, favg, dx, dy, dz, fsx, dsx, fsy, dsy, fsz, is,
+                       vp1, eps1, tt, window, x0, rdx, y0, rdy);
     return hipGetLastError();
 }
 

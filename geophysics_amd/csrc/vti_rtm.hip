@@ -1,0 +1,417 @@
+// vti_rtm.hip — reverse-time migration of one shot at a time on top of the
+// FD kernels: forward source wavefield, receiver back-propagation of the
+// observed record, zero-lag cross-correlation imaging and illumination
+// (RTM() 2D-vti-fd+rtm-cuda/Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:1184-1566;
+//  3D: 3D-vti-fd+rtm-cuda/Toa_gpu_3dvti_rtm_adcigs_1orderfunction.cu:686-1041).
+//
+// Source wavefield.  The reference "reconstructs" the source field in the
+// backward loop by re-running the forward operator from the final state
+// with a one-cell boundary ring re-injected (:1423-1437), which is not a
+// time reversal.  Here the backward loop receives the exact forward field
+// P(t) of every step it images, recomputed from full-state checkpoints with
+// the binomial (revolve) schedule: with s checkpoint slots a shot of nt
+// steps costs at most t*nt forward steps, t the smallest integer with
+// C(s+t, s) >= nt.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "vti_internal.h"
+#include "vti_rtm.h"
+
+namespace vti {
+namespace {
+
+// ------------------------------------------------------------ kernels ----
+// shot_record(load) Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:525-528: the observed
+// samples of step it overwrite P and Q on the injection row.
+__global__ void inject_rec_kernel(Grid g, float *P, float *Q, const float *__restrict__ row, int iz,
+                                  int rx0, int rdx, int rnx, int ry0, int rdy, int rny)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rnx * rny) return;
+    const int ix = rx0 + (r % rnx) * rdx, iy = ry0 + (r / rnx) * rdy;
+    const long long c = iz + (long long)ix * g.ldz + (long long)iy * g.ldxy;
+    const float v = row[r];
+    P[c] = v;
+    Q[c] = v;
+}
+
+// cal_migration :679-698 (img += s*g) and cal_illumination :654-677
+// (illum += P*P + Q*Q, clamped to 1 where <= 0 when clamp != 0) over the
+// interior nx x ny x nz, image layout z fastest.
+__global__ void image_kernel(Grid g, int np, int nx, int ny, int nz, const float *__restrict__ sP,
+                             const float *__restrict__ gP, float *img)
+{
+    const int iz = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ix = blockIdx.y, iy = blockIdx.z;
+    if (iz >= nz) return;
+    const long long c = (iz + np) + (long long)(ix + np) * g.ldz + (long long)(iy + (g.nny > 1 ? np : 0)) * g.ldxy;
+    const long long o = iz + (long long)ix * nz + (long long)iy * nz * nx;
+    img[o] += sP[c] * gP[c];
+}
+
+__global__ void illum_kernel(Grid g, int np, int nx, int ny, int nz, const float *__restrict__ P,
+                             const float *__restrict__ Q, float *illum, int clamp)
+{
+    const int iz = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ix = blockIdx.y, iy = blockIdx.z;
+    if (iz >= nz) return;
+    const long long c = (iz + np) + (long long)(ix + np) * g.ldz + (long long)(iy + (g.nny > 1 ? np : 0)) * g.ldxy;
+    const long long o = iz + (long long)ix * nz + (long long)iy * nz * nx;
+    const float p = P[c], q = Q[c];
+    float v = illum[o] + (p * p + q * q);
+    if (clamp && v <= 0.0f) v = 1.0f;
+    illum[o] = v;
+}
+
+// migration_illum :700-715 (2D: divide) / 3D :539-553 (divide where nonzero)
+__global__ void divide_kernel(float *img, const float *__restrict__ illum, size_t n, int guard)
+{
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float d = illum[i];
+    if (!guard || d != 0.0f) img[i] /= d;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// Binomial checkpointing: beta(s, t) = C(s+t, s) states reversible with s
+// slots and t sweeps (Griewank & Walther, revolve).
+static double beta(int s, int t)
+{
+    double b = 1.0;
+    for (int i = 1; i <= s; ++i) b = b * (t + i) / i;
+    return b;
+}
+
+static int reps_for(int n, int s)
+{
+    int t = 0;
+    while (beta(s, t) < n) ++t;
+    return t;
+}
+
+// Left part length of the split of n steps with s free slots.
+static int split_len(int n, int s)
+{
+    const int t = reps_for(n, s);
+    const double right = beta(s - 1, t);
+    int m = (int)std::max(1.0, n - right);
+    return std::min(m, n - 1);
+}
+
+int reverse_sweep(int nt, int nslot, const SweepOps &op, long long *fwd_steps)
+{
+    // states X_0..X_nt; X_k = state before step k.  consume(k) receives X_k
+    // in the working state for k = nt .. 1.
+    long long steps = 0;
+    std::vector<int> slot_pos(nslot, -1);   // position stored in each slot
+    auto alloc_slot = [&](int pos) -> int {
+        for (int i = 0; i < nslot; ++i)
+            if (slot_pos[i] < 0) { slot_pos[i] = pos; return i; }
+        return -1;
+    };
+    auto slot_of = [&](int pos) -> int {
+        for (int i = 0; i < nslot; ++i)
+            if (slot_pos[i] == pos) return i;
+        return -1;
+    };
+    auto restore = [&](int pos) -> int {
+        if (pos == 0) return op.zero();
+        const int i = slot_of(pos);
+        if (i < 0) return fail(VTI_ERR_STATE, "reverse_sweep: missing checkpoint");
+        return op.restore(i);
+    };
+    auto advance = [&](int from, int to) -> int {
+        for (int it = from; it < to; ++it) {
+            int st = op.step(it);
+            if (st) return st;
+            ++steps;
+        }
+        return VTI_OK;
+    };
+    // chain of right-recursion splits, stored during the first full sweep
+    std::vector<int> chain;
+    {
+        int l = 0, s = nslot;
+        while (nt - l > 1 && s > 0) {
+            const int m = l + split_len(nt - l, s);
+            chain.push_back(m);
+            l = m;
+            --s;
+        }
+    }
+    // first sweep: 0 -> nt (the caller's forward pass: record, illumination)
+    int st = op.zero();
+    if (st) return st;
+    size_t ci = 0;
+    for (int it = 0; it < nt; ++it) {
+        if (ci < chain.size() && chain[ci] == it) {
+            const int i = alloc_slot(it);
+            if (i < 0) return fail(VTI_ERR_STATE, "reverse_sweep: slot overflow");
+            if ((st = op.store(i))) return st;
+            ++ci;
+        }
+        if ((st = op.first_step(it))) return st;
+        ++steps;
+    }
+    // X_nt is live now: consume it, then reverse the rest
+    if (op.after_first && (st = op.after_first())) return st;
+    if ((st = op.consume(nt))) return st;
+    std::function<int(int, int, int)> rev = [&](int l, int r, int s) -> int {
+        // consume X_r .. X_{l+1}; X_l restorable
+        const int n = r - l;
+        if (n <= 0) return VTI_OK;
+        if (n == 1) {
+            int e = restore(l);
+            if (e) return e;
+            if ((e = advance(l, l + 1))) return e;
+            return op.consume(l + 1);
+        }
+        if (s <= 0) {
+            for (int k = r; k > l; --k) {
+                int e = restore(l);
+                if (e) return e;
+                if ((e = advance(l, k))) return e;
+                if ((e = op.consume(k))) return e;
+            }
+            return VTI_OK;
+        }
+        const int m = l + split_len(n, s);
+        int e = restore(l);
+        if (e) return e;
+        if ((e = advance(l, m))) return e;
+        const int i = alloc_slot(m);
+        if (i < 0) return fail(VTI_ERR_STATE, "reverse_sweep: slot overflow");
+        if ((e = op.store(i))) return e;
+        if ((e = rev(m, r, s - 1))) return e;
+        slot_pos[i] = -1;
+        return rev(l, m, s);
+    };
+    // unwind the chain: innermost segment first (X_nt already consumed)
+    int r = nt - 1;
+    int s_left = nslot - (int)chain.size();
+    for (int k = (int)chain.size() - 1; k >= -1; --k) {
+        const int l = k >= 0 ? chain[k] : 0;
+        if ((st = rev(l, r, s_left))) return st;
+        if (k >= 0) {
+            slot_pos[slot_of(l)] = -1;
+            ++s_left;
+        }
+        r = l;
+    }
+    if (fwd_steps) *fwd_steps = steps;
+    return VTI_OK;
+}
+
+hipError_t launch_inject_rec(vti_ctx *c, const VtiState &s, const float *row, int iz)
+{
+    const vti_shot &sh = c->shot;
+    const int n = sh.rec_nx * (c->cfg.ndim == 3 ? sh.rec_ny : 1);
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(inject_rec_kernel, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->g,
+                       s.f[VTI_F_P], s.f[VTI_F_Q], row, iz, sh.rec_x0, sh.rec_dx, sh.rec_nx,
+                       sh.rec_y0, c->cfg.ndim == 3 ? sh.rec_dy : 1, c->cfg.ndim == 3 ? sh.rec_ny : 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_image(vti_ctx *c, const float *sP, const float *gP, float *img)
+{
+    const vti_cfg &k = c->cfg;
+    const int ny = k.ndim == 3 ? k.ny : 1;
+    dim3 blk(64), grd((k.nz + 63) / 64, k.nx, ny);
+    hipLaunchKernelGGL(image_kernel, grd, blk, 0, c->stream, c->g, k.npml, k.nx, ny, k.nz, sP, gP, img);
+    return hipGetLastError();
+}
+
+hipError_t launch_illum(vti_ctx *c, const float *P, const float *Q, float *illum, int clamp)
+{
+    const vti_cfg &k = c->cfg;
+    const int ny = k.ndim == 3 ? k.ny : 1;
+    dim3 blk(64), grd((k.nz + 63) / 64, k.nx, ny);
+    hipLaunchKernelGGL(illum_kernel, grd, blk, 0, c->stream, c->g, k.npml, k.nx, ny, k.nz, P, Q, illum, clamp);
+    return hipGetLastError();
+}
+
+hipError_t launch_divide(vti_ctx *c, float *img, const float *illum, size_t n, int guard)
+{
+    hipLaunchKernelGGL(divide_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, img, illum, n, guard);
+    return hipGetLastError();
+}
+
+// laplace_filter(adj=1) Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:150-202 and the 3D
+// form Toa_gpu_3dvti_rtm_adcigs_1orderfunction.cu:555-616, in the same
+// scatter order.
+void laplace_adj(int nz, int nx, int ny, const float *in, float *out)
+{
+    const size_t n = (size_t)nz * nx * ny;
+    for (size_t j = 0; j < n; ++j) out[j] = 0.0f;
+    const size_t sx = nz, sy = (size_t)nz * nx;
+    for (int iy = 0; iy < ny; ++iy)
+        for (int ix = 0; ix < nx; ++ix)
+            for (int iz = 0; iz < nz; ++iz) {
+                const size_t j = iz + ix * sx + iy * sy;
+                if (iz > 0) { out[j - 1] -= in[j]; out[j] += in[j]; }
+                if (iz < nz - 1) { out[j + 1] -= in[j]; out[j] += in[j]; }
+                if (ix > 0) { out[j - sx] -= in[j]; out[j] += in[j]; }
+                if (ix < nx - 1) { out[j + sx] -= in[j]; out[j] += in[j]; }
+                if (ny > 1) {
+                    if (iy > 0) { out[j - sy] -= in[j]; out[j] += in[j]; }
+                    if (iy < ny - 1) { out[j + sy] -= in[j]; out[j] += in[j]; }
+                }
+            }
+}
+
+}  // namespace vti
+
+// ------------------------------------------------------------ C ABI ----
+using namespace vti;
+
+namespace {
+
+struct RtmCache {
+    VtiState g;                  // receiver wavefield
+    std::vector<VtiState> slots; // checkpoints
+};
+
+}  // namespace
+
+extern "C" {
+
+int vti_rtm_defaults(vti_rtm_opts *o, int ndim)
+{
+    if (!o) return fail(VTI_ERR_ARG, "vti_rtm_defaults: null");
+    std::memset(o, 0, sizeof(*o));
+    o->sv_forward = 1;                 // Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:1387 (true)
+    o->sv_backward = 0;                // :1460 (false)
+    o->mute_tt = ndim == 3 ? 70 : 20;  // :1409 (20); 3D :mute_directwave(..., 70)
+    o->inject_z = -1;                  // the record row (2D :527); 3D reference: npml-1 (:334)
+    o->illum_mode = ndim == 3 ? 1 : 0; // 2D: source energy with the <=0 -> 1 clamp; 3D: source+receiver
+    o->image_every = 1;
+    o->nsnap = 0;
+    return VTI_OK;
+}
+
+int vti_rtm_reset(vti_ctx *c)
+{
+    if (!c) return fail(VTI_ERR_ARG, "vti_rtm_reset: null ctx");
+    const size_t n = (size_t)c->cfg.nx * (c->cfg.ndim == 3 ? c->cfg.ny : 1) * c->cfg.nz;
+    if (!c->img) {
+        if (hipMalloc(&c->img, n * sizeof(float)) != hipSuccess ||
+            hipMalloc(&c->illum, n * sizeof(float)) != hipSuccess)
+            return fail(VTI_ERR_ALLOC, "vti_rtm_reset: image buffers");
+        c->nint = n;
+    }
+    if (launch_zero(c->img, n, c->stream) != hipSuccess || launch_zero(c->illum, n, c->stream) != hipSuccess)
+        return fail(VTI_ERR_HIP, "vti_rtm_reset: zero");
+    return VTI_OK;
+}
+
+int vti_rtm_shot(vti_ctx *c, const vti_rtm_opts *o, const float *obs, long long *fwd_steps)
+{
+    if (!c || !o) return fail(VTI_ERR_ARG, "vti_rtm_shot: null argument");
+    if (!c->have_model || !c->have_shot) return fail(VTI_ERR_STATE, "vti_rtm_shot: model and shot first");
+    if (c->nrec <= 0) return fail(VTI_ERR_STATE, "vti_rtm_shot: the shot has no receivers");
+    if (o->image_every < 1 || o->nsnap < 0) return fail(VTI_ERR_ARG, "vti_rtm_shot: bad options");
+    int st;
+    if (!c->img && (st = vti_rtm_reset(c))) return st;
+    const int nt = c->cfg.nt, nrec = c->nrec;
+    const int inj_z = o->inject_z >= 0 ? o->inject_z : c->shot.rec_z;
+    if (inj_z < 4 || inj_z >= c->g.nnz - 4) return fail(VTI_ERR_ARG, "vti_rtm_shot: injection row outside the updated region");
+    const int nslot = o->nsnap > 0 ? std::min(o->nsnap, nt) : nt;
+    RtmCache rc;
+    struct Guard {
+        RtmCache &r;
+        ~Guard() { state_free(r.g); for (auto &s : r.slots) state_free(s); }
+    } guard{rc};
+    if ((st = state_alloc(c, rc.g))) return st;
+    rc.slots.resize(nslot);
+    for (int i = 0; i < nslot; ++i)
+        if ((st = state_alloc(c, rc.slots[i]))) return st;
+    const VtiState W = main_state(c);
+    const int nd = c->cfg.ndim;
+    const bool svf = o->sv_forward != 0 && c->cfg.sv != 0;
+    const bool svb = o->sv_backward != 0 && c->cfg.sv != 0;
+    const int clamp = o->illum_mode == 0;
+    auto hip = [](hipError_t e, const char *w) { return e == hipSuccess ? VTI_OK : fail(VTI_ERR_HIP, w); };
+
+    SweepOps op;
+    op.zero = [&]() { return hip(state_zero(c, W), "zero"); };
+    op.store = [&](int i) { return hip(state_copy(c, rc.slots[i], W), "store"); };
+    op.restore = [&](int i) { return hip(state_copy(c, W, rc.slots[i]), "restore"); };
+    op.step = [&](int it) {
+        if (launch_vel(nd, vel_args_for(c, W, it, true), c->stream) != hipSuccess) return fail(VTI_ERR_HIP, "vel");
+        return hip(launch_stress(nd, stress_args_for(c, W, it, true, svf, nullptr), c->stream), "stress");
+    };
+    op.first_step = [&](int it) {
+        if (launch_vel(nd, vel_args_for(c, W, it, true), c->stream) != hipSuccess) return fail(VTI_ERR_HIP, "vel");
+        float *row = c->rec + (size_t)it * nrec;
+        if (launch_stress(nd, stress_args_for(c, W, it, true, svf, row), c->stream) != hipSuccess)
+            return fail(VTI_ERR_HIP, "stress");
+        return hip(launch_illum(c, W.f[VTI_F_P], W.f[VTI_F_Q], c->illum, clamp), "illum");
+    };
+    op.after_first = [&]() {
+        if (obs) {   // observed record [irec][it] replaces the modelled one (:1411-1415)
+            if (hipMemcpyAsync(c->rec_t, obs, (size_t)nrec * nt * sizeof(float), hipMemcpyHostToDevice,
+                               c->stream) != hipSuccess)
+                return fail(VTI_ERR_HIP, "obs upload");
+            return hip(launch_transpose(c->rec_t, c->rec, nrec, nt, c->stream), "transpose");
+        }
+        if (o->mute_tt >= 0) {
+            const vti_shot &s = c->shot;
+            const int np = c->cfg.npml;
+            if (nd == 2) return vti_mute(c, s.sx - np + 1, 0, s.sz - np + 1, 1, o->mute_tt);
+            return vti_mute3d_window(c, s.sx - np + 1, 0, s.sy - np + 1, 0, s.sz - np + 1, 0, o->mute_tt);
+        }
+        return VTI_OK;
+    };
+    op.consume = [&](int k) {
+        const int it = k - 1;
+        if (launch_inject_rec(c, rc.g, c->rec + (size_t)it * nrec, inj_z) != hipSuccess)
+            return fail(VTI_ERR_HIP, "inject");
+        if (launch_vel(nd, vel_args_for(c, rc.g, it, false), c->stream) != hipSuccess)
+            return fail(VTI_ERR_HIP, "vel");
+        if (launch_stress(nd, stress_args_for(c, rc.g, it, false, svb, nullptr), c->stream) != hipSuccess)
+            return fail(VTI_ERR_HIP, "stress");
+        if (o->illum_mode == 1 && launch_illum(c, rc.g.f[VTI_F_P], rc.g.f[VTI_F_Q], c->illum, 0) != hipSuccess)
+            return fail(VTI_ERR_HIP, "illum");
+        if (it % o->image_every == 0)
+            return hip(launch_image(c, W.f[VTI_F_P], rc.g.f[VTI_F_P], c->img), "image");
+        return VTI_OK;
+    };
+    c->rtm_illum_mode = o->illum_mode;
+    if ((st = reverse_sweep(nt, nslot, op, fwd_steps))) return st;
+    return hip(hipStreamSynchronize(c->stream), "sync");
+}
+
+int vti_rtm_image(vti_ctx *c, int finalize, float *img, float *illum)
+{
+    if (!c || !c->img) return fail(VTI_ERR_STATE, "vti_rtm_image: no image");
+    const size_t n = c->nint;
+    std::vector<float> hi(n), hl(n);
+    if (hipStreamSynchronize(c->stream) != hipSuccess ||
+        hipMemcpy(hi.data(), c->img, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(hl.data(), c->illum, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(VTI_ERR_HIP, "vti_rtm_image: copy");
+    if (illum) std::memcpy(illum, hl.data(), n * sizeof(float));
+    if (!img) return VTI_OK;
+    if (!finalize) {
+        std::memcpy(img, hi.data(), n * sizeof(float));
+        return VTI_OK;
+    }
+    // migration_illum then laplace_filter(adj=1) (:1486-1493)
+    const bool guard = c->rtm_illum_mode == 1;
+    for (size_t i = 0; i < n; ++i)
+        if (!guard || hl[i] != 0.0f) hi[i] /= hl[i];
+    laplace_adj(c->cfg.nz, c->cfg.nx, c->cfg.ndim == 3 ? c->cfg.ny : 1, hi.data(), img);
+    return VTI_OK;
+}
+
+}  // extern "C"

@@ -52,6 +52,11 @@ class Shot(C.Structure):
         "rec_x0", "rec_dx", "rec_nx", "rec_y0", "rec_dy", "rec_ny")]
 
 
+class RtmOpts(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("sv_forward", "sv_backward", "mute_tt", "inject_z",
+                                        "illum_mode", "image_every", "nsnap")]
+
+
 _lib = None
 FP = C.POINTER(C.c_float)
 DP = C.POINTER(C.c_double)
@@ -92,6 +97,18 @@ def _declare(L):
         "vti_calib_copy": (C.c_int, [VP, VP, C.c_size_t, VP]),
         "vti_sync": (C.c_int, [VP]),
         "vti_dropin_status": (C.c_int, []),
+        "vti_rtm_defaults": (C.c_int, [C.POINTER(RtmOpts), C.c_int]),
+        "vti_rtm_reset": (C.c_int, [VP]),
+        "vti_rtm_shot": (C.c_int, [VP, C.POINTER(RtmOpts), FP, C.POINTER(C.c_longlong)]),
+        "vti_rtm_image": (C.c_int, [VP, C.c_int, FP, FP]),
+        "vti_mute3d_window": (C.c_int, [VP] + [C.c_int] * 7),
+        "vti_grad_create": (C.c_int, [C.POINTER(Cfg), C.c_int, C.POINTER(VP)]),
+        "vti_grad_destroy": (C.c_int, [VP]),
+        "vti_grad_set_model": (C.c_int, [VP, DP, DP, DP, C.c_int]),
+        "vti_grad_set_shot": (C.c_int, [VP, C.POINTER(Shot)]),
+        "vti_grad_set_source": (C.c_int, [VP, DP]),
+        "vti_grad_forward": (C.c_int, [VP, DP]),
+        "vti_grad_run": (C.c_int, [VP, DP, DP, C.c_int, DP, DP, DP, DP, C.POINTER(C.c_longlong)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -274,6 +291,37 @@ class Propagator:
         check(lib().vti_get_table(self.h, T_D0, out.ctypes.data_as(FP)), "get_table")
         return float(out[0])
 
+    # ---- reverse-time migration (vti_rtm_*) ----
+    def rtm_defaults(self):
+        o = RtmOpts()
+        check(lib().vti_rtm_defaults(C.byref(o), self.cfg.ndim), "rtm_defaults")
+        return o
+
+    def rtm_reset(self):
+        check(lib().vti_rtm_reset(self.h), "rtm_reset")
+
+    def rtm_shot(self, opts=None, obs=None):
+        """Migrate the current shot; obs: observed record (nrec, nt) or None.
+        Returns the number of forward steps taken."""
+        o = opts if opts is not None else self.rtm_defaults()
+        steps = C.c_longlong(0)
+        if obs is not None:
+            a, p = _f32(obs)
+            if a.shape != (self.nrec, self.cfg.nt):
+                raise VtiError(f"obs must be shaped ({self.nrec}, {self.cfg.nt})")
+        else:
+            p = None
+        check(lib().vti_rtm_shot(self.h, C.byref(o), p, C.byref(steps)), "rtm_shot")
+        return steps.value
+
+    def rtm_image(self, finalize=True):
+        shape = (self.cfg.ny, self.cfg.nx, self.cfg.nz) if self.cfg.ndim == 3 else (self.cfg.nx, self.cfg.nz)
+        img = np.zeros(shape, dtype=np.float32)
+        ill = np.zeros(shape, dtype=np.float32)
+        check(lib().vti_rtm_image(self.h, 1 if finalize else 0, img.ctypes.data_as(FP),
+                                  ill.ctypes.data_as(FP)), "rtm_image")
+        return img, ill
+
     def prof_enable(self, on=True):
         check(lib().vti_prof_enable(self.h, 1 if on else 0), "prof_enable")
 
@@ -326,3 +374,89 @@ def GPU_vti3dfd(nx, ny, nz, dx, dy, dz, npml, SV, FNv, FNe, FNd, favg, ns, fsx, 
     lib().GPU_vti3dfd(nx, ny, nz, dx, dy, dz, npml, SV, _b(FNv), _b(FNe), _b(FNd), favg, ns,
                       fsx, dsx, fsy, dsy, fsz, dsz, _b(FNshot), _b(FNsnap), nt, dt, run_count)
     check(lib().vti_dropin_status(), "GPU_vti3dfd")
+
+
+def _f64(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(DP)
+
+
+class Gradient:
+    """Forward, exact discrete adjoint and vp-gradient of the FD path
+    (vti_grad_*), FP32 or FP64.  Point-source (GPU) semantics."""
+
+    def __init__(self, cfg, fp64=False):
+        self.cfg = cfg
+        h = VP()
+        check(lib().vti_grad_create(C.byref(cfg), 1 if fp64 else 0, C.byref(h)), "vti_grad_create")
+        self.h = h
+        self.fp64 = fp64
+        self.shot = None
+
+    @property
+    def model_shape(self):
+        c = self.cfg
+        return (c.ny, c.nx, c.nz) if c.ndim == 3 else (c.nx, c.nz)
+
+    @property
+    def nrec(self):
+        s = self.shot
+        return s.rec_nx * (s.rec_ny if self.cfg.ndim == 3 else 1)
+
+    def set_model(self, vp, eps, dl, keep_tables=False):
+        a, pa = _f64(vp)
+        b, pb = _f64(eps)
+        d, pd = _f64(dl)
+        check(lib().vti_grad_set_model(self.h, pa, pb, pd, 1 if keep_tables else 0), "grad_set_model")
+
+    def set_shot(self, shot):
+        check(lib().vti_grad_set_shot(self.h, C.byref(shot)), "grad_set_shot")
+        self.shot = shot
+
+    def set_source(self, amp=None):
+        if amp is None:
+            check(lib().vti_grad_set_source(self.h, None), "grad_set_source")
+        else:
+            a, p = _f64(amp)
+            check(lib().vti_grad_set_source(self.h, p), "grad_set_source")
+
+    def forward(self):
+        d = np.zeros((self.nrec, self.cfg.nt))
+        check(lib().vti_grad_forward(self.h, d.ctypes.data_as(DP)), "grad_forward")
+        return d
+
+    def run(self, dobs=None, resid=None, nsnap=0):
+        """Returns dict(misfit, grad_vp, grad_src, d, fwd_steps)."""
+        shape = (self.nrec, self.cfg.nt)
+        po = pr = None
+        if dobs is not None:
+            dobs, po = _f64(dobs)
+            assert dobs.shape == shape
+        if resid is not None:
+            resid, pr = _f64(resid)
+            assert resid.shape == shape
+        J = C.c_double()
+        g = np.zeros(self.model_shape)
+        gs = np.zeros(self.cfg.nt)
+        d = np.zeros(shape)
+        steps = C.c_longlong()
+        check(lib().vti_grad_run(self.h, po, pr, nsnap, C.byref(J), g.ctypes.data_as(DP),
+                                 gs.ctypes.data_as(DP), d.ctypes.data_as(DP), C.byref(steps)), "grad_run")
+        return {"misfit": J.value, "grad_vp": g, "grad_src": gs, "d": d, "fwd_steps": steps.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vti_grad_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

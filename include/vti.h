@@ -174,6 +174,70 @@ int vti_get_table(vti_ctx *ctx, int which, float *dst);
 /* Padded sizes (nnx, nny, nnz; nny = 1 in 2D). */
 int vti_padded_dims(const vti_ctx *ctx, int *nnx, int *nny, int *nnz);
 
+/* ---- reverse-time migration -------------------------------------------- */
+/* Per-shot RTM on the context's model and shot: forward source wavefield
+ * (record + illumination), receiver back-propagation of the observed record
+ * and zero-lag cross-correlation imaging, following RTM()
+ * 2D-vti-fd+rtm-cuda/Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:1184-1566 and the 3D
+ * main Toa_gpu_3dvti_rtm_adcigs_1orderfunction.cu:686-1041.  The source
+ * wavefield of every imaged step is the exact forward field, recomputed
+ * from nsnap full-state checkpoints with the binomial (revolve) schedule;
+ * the reference's boundary-ring reconstruction (:1423-1437) is replaced. */
+typedef struct vti_rtm_opts {
+    int sv_forward;    /* SV taper in the source propagation (reference: on)  */
+    int sv_backward;   /* SV taper in the receiver propagation (reference: off) */
+    int mute_tt;       /* windowed direct-wave mute of the modelled record when
+                          no observed record is given; <0 = none (2D ref: 20) */
+    int inject_z;      /* receiver injection row, padded; <0 = the record row  */
+    int illum_mode;    /* 0: source energy, <=0 clamped to 1 while accumulating,
+                          divide (2D ref :654-677, :700-715); 1: source +
+                          receiver energy, divide where nonzero (3D ref)       */
+    int image_every;   /* cross-correlate every k-th step                      */
+    int nsnap;         /* full-state checkpoint slots; 0 = one per step        */
+} vti_rtm_opts;
+int vti_rtm_defaults(vti_rtm_opts *opts, int ndim);
+/* Zero the accumulated image and illumination (interior nx*ny*nz). */
+int vti_rtm_reset(vti_ctx *ctx);
+/* One shot; obs = observed record [irec][it] in host memory, or NULL to
+ * migrate the modelled (muted) record.  *fwd_steps = forward steps taken. */
+int vti_rtm_shot(vti_ctx *ctx, const vti_rtm_opts *opts, const float *obs,
+                 long long *fwd_steps);
+/* Image and illumination to host memory; finalize = divide by the
+ * illumination and apply the adjoint Laplacian filter (:1486-1493). */
+int vti_rtm_image(vti_ctx *ctx, int finalize, float *img, float *illum);
+/* Windowed 3D direct-wave mute of the RTM main
+ * (Toa_gpu_3dvti_rtm_adcigs_1orderfunction.cu:353-381). */
+int vti_mute3d_window(vti_ctx *ctx, int fsx, int dsx, int fsy, int dsy,
+                      int fsz, int is, int tt);
+
+/* ---- adjoint and FWI gradient -------------------------------------------- */
+/* No counterpart in the reference for VTI; the reference's FWI gradient is
+ * the ISO one of 3D-iso-fd-rtm-fwi-cuda/fwi3d/Toa_gpu_3dFWI.cu:295-311,
+ * 474-494.  A vti_grad context runs the forward step of the point-source
+ * (GPU) form in FP32 (bit-identical to vti_step) or FP64, its exact
+ * discrete adjoint, and the gradient of J = 1/2 sum (d - dobs)^2 with
+ * respect to the unpadded vp (absorbing-layer tables held fixed).  Host
+ * arrays at this boundary are double. */
+typedef struct vti_grad vti_grad;
+int vti_grad_create(const vti_cfg *cfg, int fp64, vti_grad **out);
+int vti_grad_destroy(vti_grad *g);
+/* keep_tables: reuse the damping tables of the previous model (so that a
+ * perturbed model sees the same absorbing layer). */
+int vti_grad_set_model(vti_grad *g, const double *vp, const double *eps,
+                       const double *del, int keep_tables);
+int vti_grad_set_shot(vti_grad *g, const vti_shot *shot);
+/* Per-step source gain (nt values) replacing pfac*wavelet; NULL restores it. */
+int vti_grad_set_source(vti_grad *g, const double *amp);
+/* Forward only: record d [irec][it]. */
+int vti_grad_forward(vti_grad *g, double *d);
+/* Forward + adjoint sweep with nsnap checkpoint slots (0 = every step).
+ * The residual is d - dobs, or resid when non-NULL.  Outputs (each may be
+ * NULL): misfit, grad_vp (unpadded model size), grad_src (nt: gradient
+ * with respect to the per-step source gain), d_out (record), fwd_steps. */
+int vti_grad_run(vti_grad *g, const double *dobs, const double *resid, int nsnap,
+                 double *misfit, double *grad_vp, double *grad_src, double *d_out,
+                 long long *fwd_steps);
+
 /* ---- measurement ------------------------------------------------------- */
 /* When on, vti_step brackets every velocity and stress launch with HIP
  * events on the context stream; vti_prof_read sums them (ms). */

@@ -65,9 +65,15 @@ def lib():
         L.orc_params_cpu2d.argtypes = [pp] + [C.c_int] * 14
         L.orc_params_gpu3d.argtypes = [pp] + [C.c_int] * 20
         L.orc_mute2d.argtypes = [C.c_int, C.c_int] + [C.c_float] * 4 + [C.c_int] * 4 + \
-            [C.c_float, C.c_float, fp, C.c_int]
+            [C.c_float, C.c_float, fp, C.c_int, C.c_int, C.c_int]
         L.orc_mute3d.argtypes = [C.c_int] * 3 + [C.c_float] * 5 + [C.c_int] * 6 + \
-            [C.c_float, C.c_float, fp, C.c_int]
+            [C.c_float, C.c_float, fp] + [C.c_int] * 6
+        for sfx, rp in (("_f32", fp), ("_f64", dp)):
+            getattr(L, "orc_rtm" + sfx).argtypes = [C.c_void_p, C.c_void_p] + [C.c_int] * 5 + [rp] * 4
+        L.orc_laplace_adj.argtypes = [C.c_int] * 3 + [fp, fp]
+        for sfx, rp in (("_f32", fp), ("_f64", dp)):
+            getattr(L, "orc_fwi_grad" + sfx).argtypes = [C.c_void_p] + [rp] * 6
+            getattr(L, "orc_set_source" + sfx).argtypes = [C.c_void_p, rp]
         _lib = L
     return _lib
 
@@ -175,16 +181,93 @@ class Sim:
             pass
 
 
-def mute2d(shot, nt, dt, favg, dx, dz, fs, ds, zs, is_, vp1, eps1, tt=100):
-    """shot: float32 (nx, nt), muted in place."""
+def mute2d(shot, nt, dt, favg, dx, dz, fs, ds, zs, is_, vp1, eps1, tt=100, x0=0, rdx=1):
+    """shot: float32 (nrec, nt), muted in place (mute_directwave2d)."""
     assert shot.dtype == np.float32 and shot.flags.c_contiguous
-    nx = shot.shape[0]
-    lib().orc_mute2d(nx, nt, dt, favg, dx, dz, fs, ds, zs, is_, vp1, eps1, _fptr(shot), tt)
+    nrec = shot.shape[0]
+    lib().orc_mute2d(nrec, nt, dt, favg, dx, dz, fs, ds, zs, is_, vp1, eps1, _fptr(shot), tt, x0, rdx)
     return shot
 
 
-def mute3d(shot, nx, ny, nt, dt, favg, dx, dy, dz, fsx, fsy, dsx, dsy, fsz, is_, vp1, eps1, tt=100):
+def mute3d(shot, nrx, nry, nt, dt, favg, dx, dy, dz, fsx, fsy, dsx, dsy, fsz, is_, vp1, eps1, tt=100,
+           window=0, x0=0, rdx=1, y0=0, rdy=1):
     assert shot.dtype == np.float32 and shot.flags.c_contiguous
-    lib().orc_mute3d(nx, ny, nt, dt, favg, dx, dy, dz, fsx, fsy, dsx, dsy, fsz, is_,
-                     vp1, eps1, _fptr(shot), tt)
+    lib().orc_mute3d(nrx, nry, nt, dt, favg, dx, dy, dz, fsx, fsy, dsx, dsy, fsz, is_,
+                     vp1, eps1, _fptr(shot), tt, window, x0, rdx, y0, rdy)
     return shot
+
+
+def laplace_adj(img):
+    """laplace_filter(adj=1) of an interior image shaped (nx, nz) or (ny, nx, nz)."""
+    a = np.ascontiguousarray(img, dtype=np.float32)
+    out = np.zeros_like(a)
+    if a.ndim == 2:
+        nx, nz, ny = a.shape[0], a.shape[1], 1
+    else:
+        ny, nx, nz = a.shape
+    lib().orc_laplace_adj(nz, nx, ny, _fptr(a), _fptr(out))
+    return out
+
+
+def rtm(params, vp, eps, dl, sv_fwd=1, sv_bwd=0, inject_z=None, illum_mode=0, image_every=1,
+        obs=None, mute=None, prec="f32"):
+    """One-shot RTM with every forward snapshot kept (no checkpointing).
+
+    mute: callable(modelled_record) applied in place before it is used as
+    data when obs is None.  Returns (img, illum, modelled) shaped like the
+    interior ((nx, nz) or (ny, nx, nz)) and (nrec, nt)."""
+    s = Sim(params, vp, eps, dl, prec)
+    g = Sim(params, vp, eps, dl, prec)
+    dt = s.dtype
+    shape = (params.ny, params.nx, params.nz) if params.ndim == 3 else (params.nx, params.nz)
+    img = np.zeros(shape, dtype=dt)
+    illum = np.zeros(shape, dtype=dt)
+    modelled = np.zeros((s.nrec, params.nt), dtype=dt)
+    iz = params.rec_z if inject_z is None else inject_z
+    if obs is None and mute is not None:
+        # the mute applies to the modelled record between the passes: run the
+        # forward once to get it, then migrate it as observed data
+        tmp = Sim(params, vp, eps, dl, prec)
+        rec = tmp.run()
+        tmp.close()
+        mute(rec)
+        obs = rec
+    o = None if obs is None else np.ascontiguousarray(obs, dtype=dt)
+    fn = getattr(lib(), "orc_rtm_" + prec)
+    fn(s.h, g.h, sv_fwd, sv_bwd, iz, illum_mode, image_every,
+       s._ptr(o) if o is not None else None, s._ptr(modelled), s._ptr(img), s._ptr(illum))
+    s.close()
+    g.close()
+    return img, illum, modelled
+
+
+def copy_tables(dst, src):
+    """Give Sim dst the damping tables of Sim src (fixed absorbing layer)."""
+    for name in TABLES:
+        t = np.zeros(dst.nny * dst.nnx * dst.nnz, dtype=dst.dtype)
+        k = getattr(lib(), "orc_get_" + src.prec)(src.h, TABLES[name], src._ptr(t))
+        if k > 0:
+            getattr(lib(), "orc_set_" + dst.prec)(dst.h, TABLES[name], dst._ptr(t))
+
+
+def fwi_grad(sim, dobs=None, resid=None):
+    """Misfit, vp-gradient (interior shape), source gradient (nt), record.
+    sim must be fresh or is reset (its fields are overwritten)."""
+    p = sim.p
+    shape = (p.ny, p.nx, p.nz) if p.ndim == 3 else (p.nx, p.nz)
+    dt = sim.dtype
+    J = np.zeros(1, dtype=dt)
+    g = np.zeros(shape, dtype=dt)
+    gs = np.zeros(p.nt, dtype=dt)
+    d = np.zeros((sim.nrec, p.nt), dtype=dt)
+    o = None if dobs is None else np.ascontiguousarray(dobs, dtype=dt)
+    r = None if resid is None else np.ascontiguousarray(resid, dtype=dt)
+    getattr(lib(), "orc_fwi_grad_" + sim.prec)(
+        sim.h, sim._ptr(o) if o is not None else None, sim._ptr(r) if r is not None else None,
+        sim._ptr(J), sim._ptr(g), sim._ptr(gs), sim._ptr(d))
+    return float(J[0]), g, gs, d
+
+
+def set_source(sim, amp):
+    a = np.ascontiguousarray(amp, dtype=sim.dtype)
+    getattr(lib(), "orc_set_source_" + sim.prec)(sim.h, sim._ptr(a))

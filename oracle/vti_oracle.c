@@ -126,29 +126,34 @@ int orc_params_gpu3d(orc_params *p, int nx, int ny, int nz, int dx, int dy,
     return 0;
 }
 
-/* mute_directwave2d GPU_vti2dfd_kernels.cu:151-174 on shot[ix*nt+it]. */
-void orc_mute2d(int nx, int nt, float dt, float favg, float dx, float dz,
+/* mute_directwave2d GPU_vti2dfd_kernels.cu:151-174 on shot[r*nt+it]. */
+void orc_mute2d(int nrec, int nt, float dt, float favg, float dx, float dz,
                 int fs, int ds, int zs, int is, float vp1, float eps1,
-                float *shot, int tt)
+                float *shot, int tt, int x0, int rdx)
 {
-    for (int ix = 0; ix < nx; ix++) {
+    for (int r = 0; r < nrec; r++) {
+        int ix = x0 + r * rdx;
         float mu_x = dx * abs(ix - fs - (is - 1) * ds);
         float mu_z = dz * zs;
         float mu_t0 = sqrtf(mu_x * mu_x + mu_z * mu_z) / (vp1 * sqrtf(1 + 2 * eps1));
         int mu_t = (int)(2.0 / (dt * favg));
         int mu_nt = (int)(mu_t0 / dt) + mu_t + tt;
         for (int it = 0; it < nt; it++)
-            if ((it > (int)(mu_t0 / dt) - tt) && it < mu_nt) shot[(long)ix * nt + it] = 0.0f;
+            if ((it > (int)(mu_t0 / dt) - tt) && it < mu_nt) shot[(long)r * nt + it] = 0.0f;
     }
 }
 
-/* mute_directwave3d GPU_vti3dfd_kernels.cu:342-369 on shot[it+nt*ix+nt*nx*iy]. */
-void orc_mute3d(int nx, int ny, int nt, float dt, float favg, float dx,
+/* mute_directwave3d GPU_vti3dfd_kernels.cu:342-369 (window = 0) and the
+ * RTM main's windowed form Toa_gpu_3dvti_rtm_adcigs_1orderfunction.cu:371-380
+ * (window = 1) on shot[it + nt*(irx + nrx*iry)]. */
+void orc_mute3d(int nrx, int nry, int nt, float dt, float favg, float dx,
                 float dy, float dz, int fsx, int fsy, int dsx, int dsy,
-                int fsz, int is, float vp1, float eps1, float *shot, int tt)
+                int fsz, int is, float vp1, float eps1, float *shot, int tt,
+                int window, int x0, int rdx, int y0, int rdy)
 {
-    for (int iy = 0; iy < ny; iy++)
-        for (int ix = 0; ix < nx; ix++) {
+    for (int iry = 0; iry < nry; iry++)
+        for (int irx = 0; irx < nrx; irx++) {
+            int ix = x0 + irx * rdx, iy = y0 + iry * rdy;
             float mu_x = dx * abs(ix - fsx - is * dsx);
             float mu_y = dy * abs(iy - fsy - is * dsy);
             float mu_z = dz * fsz;
@@ -156,7 +161,28 @@ void orc_mute3d(int nx, int ny, int nt, float dt, float favg, float dx,
                           (vp1 * sqrtf(1 + 2 * eps1));
             int mu_t = (int)(2.0 / (dt * favg));
             int mu_nt = (int)(mu_t0 / dt) + mu_t + tt;
-            for (int it = 0; it < nt; it++)
-                if (it < mu_nt) shot[it + (long)ix * nt + (long)iy * nx * nt] = 0.0f;
+            for (int it = 0; it < nt; it++) {
+                int lo = !window || (it > (int)(mu_t0 / dt) - tt);
+                if (lo && it < mu_nt) shot[it + (long)nt * (irx + (long)nrx * iry)] = 0.0f;
+            }
         }
+}
+
+void orc_laplace_adj(int nz, int nx, int ny, const float *in, float *out)
+{
+    long n = (long)nz * nx * ny, sx = nz, sy = (long)nz * nx;
+    for (long j = 0; j < n; j++) out[j] = 0.0f;
+    for (int iy = 0; iy < ny; iy++)
+        for (int ix = 0; ix < nx; ix++)
+            for (int iz = 0; iz < nz; iz++) {
+                long j = iz + ix * sx + iy * sy;
+                if (iz > 0) { out[j - 1] -= in[j]; out[j] += in[j]; }
+                if (iz < nz - 1) { out[j + 1] -= in[j]; out[j] += in[j]; }
+                if (ix > 0) { out[j - sx] -= in[j]; out[j] += in[j]; }
+                if (ix < nx - 1) { out[j + sx] -= in[j]; out[j] += in[j]; }
+                if (ny > 1) {
+                    if (iy > 0) { out[j - sy] -= in[j]; out[j] += in[j]; }
+                    if (iy < ny - 1) { out[j + sy] -= in[j]; out[j] += in[j]; }
+                }
+            }
 }

@@ -85,13 +85,32 @@ void orc_destroy_f32(void *h);
 void orc_destroy_f64(void *h);
 long orc_ncell(void *h32);
 
-/* Direct-wave mute of a finished shot record (float), GPU 2D / 3D forms. */
-void orc_mute2d(int nx, int nt, float dt, float favg, float dx, float dz,
+/* RTM of one shot (see vti_oracle_impl.h). */
+int orc_rtm_f32(void *hs, void *hr, int sv_fwd, int sv_bwd, int inject_z, int illum_mode,
+                int image_every, const float *obs, float *modelled, float *img, float *illum);
+int orc_rtm_f64(void *hs, void *hr, int sv_fwd, int sv_bwd, int inject_z, int illum_mode,
+                int image_every, const double *obs, double *modelled, double *img, double *illum);
+/* Misfit gradient with respect to vp (see vti_oracle_impl.h). */
+int orc_fwi_grad_f32(void *h, const float *dobs, const float *resid, float *misfit,
+                     float *grad_int, float *grad_src, float *d_out);
+int orc_fwi_grad_f64(void *h, const double *dobs, const double *resid, double *misfit,
+                     double *grad_int, double *grad_src, double *d_out);
+int orc_set_source_f32(void *h, const float *amp);
+int orc_set_source_f64(void *h, const double *amp);
+/* laplace_filter(adj=1), Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:150-202 (ny=1)
+ * and the 3D form Toa_gpu_3dvti_rtm_adcigs_1orderfunction.cu:555-616. */
+void orc_laplace_adj(int nz, int nx, int ny, const float *in, float *out);
+
+/* Direct-wave mute of a finished shot record (float), GPU 2D / 3D forms;
+ * receiver r at model column x0 + r*rdx (3D: x0+irx*rdx, y0+iry*rdy);
+ * window: 3D RTM form with a lower bound. */
+void orc_mute2d(int nrec, int nt, float dt, float favg, float dx, float dz,
                 int fs, int ds, int zs, int is, float vp1, float eps1,
-                float *shot, int tt);
-void orc_mute3d(int nx, int ny, int nt, float dt, float favg, float dx,
+                float *shot, int tt, int x0, int rdx);
+void orc_mute3d(int nrx, int nry, int nt, float dt, float favg, float dx,
                 float dy, float dz, int fsx, int fsy, int dsx, int dsy,
-                int fsz, int is, float vp1, float eps1, float *shot, int tt);
+                int fsz, int is, float vp1, float eps1, float *shot, int tt,
+                int window, int x0, int rdx, int y0, int rdy);
 
 #ifdef __cplusplus
 }

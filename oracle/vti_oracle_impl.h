@@ -233,11 +233,11 @@ void FN(orc_destroy)(void *h)
 }
 
 /* Effective eps/delta after the SV taper. */
-static inline void FN(taper)(const SIM *s, int ix, int iy, int iz, long id, REAL *ee, REAL *dd)
+static inline void FN(taper)(const SIM *s, int sv, int ix, int iy, int iz, long id, REAL *ee, REAL *dd)
 {
     *ee = s->ep[id];
     *dd = s->de[id];
-    if (!s->p.sv) return;
+    if (!sv) return;
     int rx = ix - s->p.sv_cx, rz = iz - s->p.sv_cz, ry = 0;
     if (s->p.ndim == 3) ry = iy - s->p.sv_cy;
     int r2 = rx * rx + ry * ry + rz * rz;
@@ -296,7 +296,7 @@ static void FN(vel)(SIM *s)
 /* Stress update.  update_stress2d GPU_vti2dfd_kernels.cu:240-299
  * (= CPU_vti2dfd_kernels.c:108-177 with P=px+pz+s at :173-174);
  * update_stress3d GPU_vti3dfd_kernels.cu:131-201. */
-static void FN(stress)(SIM *s, int it)
+static void FN(stress)(SIM *s, int it, int inject, int sv)
 {
     const int nnx = s->nnx, nny = s->nny, nnz = s->nnz;
     const long sx = nnz, sy = (long)nnz * nnx;
@@ -314,7 +314,7 @@ static void FN(stress)(SIM *s, int it)
             for (int iz = 4; iz < nnz - 4; iz++) {
                 long id = (long)ix * nnz + iz;
                 REAL ee, dd;
-                FN(taper)(s, ix, 0, iz, id, &ee, &dd);
+                FN(taper)(s, sv, ix, 0, iz, id, &ee, &dd);
                 REAL xx = 0, zz = 0;
                 for (int im = 0; im < 4; im++) {
                     xx += c[im] * (u[id + im * sx] - u[id - (im + 1) * sx]);
@@ -337,7 +337,7 @@ static void FN(stress)(SIM *s, int it)
                 for (int iz = 4; iz < nnz - 4; iz++) {
                     long id = iz + (long)ix * nnz + (long)iy * sy;
                     REAL ee, dd;
-                    FN(taper)(s, ix, iy, iz, id, &ee, &dd);
+                    FN(taper)(s, sv, ix, iy, iz, id, &ee, &dd);
                     REAL xx = 0, yy = 0, zz = 0;
                     for (int im = 0; im < 4; im++) {
                         yy += c[im] * (v[id + im * sy] - v[id - (im + 1) * sy]);
@@ -359,7 +359,7 @@ static void FN(stress)(SIM *s, int it)
     }
     /* In-stress Gaussian source: P = px+pz+s, Q = qx+qz+s (only inside the
      * updated region, where s is nonzero only on the footprint). */
-    if (s->p.src_kind == 1 && s->amp[it] != 0) {
+    if (inject && s->p.src_kind == 1 && s->amp[it] != 0) {
         for (int k = 0; k < s->ng; k++) {
             long id = s->gidx[k];
             REAL sv = (REAL)((double)s->amp[it] * s->gw[k]);
@@ -391,11 +391,10 @@ static void FN(record)(const SIM *s, REAL *rec)
         }
 }
 
-int FN(orc_step)(void *h, int it, REAL *rec)
+static int FN(step_ex)(SIM *s, int it, int inject, int sv, REAL *rec)
 {
-    SIM *s = (SIM *)h;
     if (!s || it < 0 || it >= s->p.nt) return -1;
-    if (s->p.src_kind == 0) {
+    if (inject && s->p.src_kind == 0) {
         /* add_source2d GPU_vti2dfd_kernels.cu:200-206: before update_vel */
         long id = s->p.sz + (long)s->p.sx * s->nnz + (long)s->p.sy * s->nnz * s->nnx * (s->p.ndim == 3);
         REAL a = s->amp[it];
@@ -405,9 +404,16 @@ int FN(orc_step)(void *h, int it, REAL *rec)
         }
     }
     FN(vel)(s);
-    FN(stress)(s, it);
+    FN(stress)(s, it, inject, sv);
     if (rec) FN(record)(s, rec);
     return 0;
+}
+
+int FN(orc_step)(void *h, int it, REAL *rec)
+{
+    SIM *s = (SIM *)h;
+    if (!s) return -1;
+    return FN(step_ex)(s, it, 1, s->p.sv, rec);
 }
 
 int FN(orc_run)(void *h, REAL *shot)
@@ -459,6 +465,259 @@ int FN(orc_set)(void *h, int which, const REAL *in)
     REAL *dst = FN(select)((SIM *)h, which, &len);
     if (!dst) return -1;
     memcpy(dst, in, len * sizeof(REAL));
+    return 0;
+}
+
+/* RTM of one shot, restating RTM() Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:1378-1484
+ * with the exact forward field of every step (all P snapshots kept in
+ * memory).  hs: the shot's simulation (source side), fresh; hr: a second
+ * simulation of the same model and shot (receiver side), fresh.
+ * obs: [irec][it] record or NULL (use the modelled record, muted by the
+ * caller-provided callback values already applied: pass the muted record
+ * as obs).  img, illum: interior nx*ny*nz, accumulated. */
+int FN(orc_rtm)(void *hs, void *hr, int sv_fwd, int sv_bwd, int inject_z, int illum_mode,
+                int image_every, const REAL *obs, REAL *modelled, REAL *img, REAL *illum)
+{
+    SIM *s = (SIM *)hs, *g = (SIM *)hr;
+    if (!s || !g || image_every < 1) return -1;
+    const orc_params *p = &s->p;
+    const int np = p->npml, nt = p->nt;
+    const int ny = p->ndim == 3 ? p->ny : 1;
+    const long nint = (long)p->nx * ny * p->nz;
+    const long nrec = (long)p->rec_nx * (p->ndim == 3 ? p->rec_ny : 1);
+    REAL *snap = (REAL *)malloc((size_t)nt * nint * sizeof(REAL));
+    REAL *row = (REAL *)malloc((nrec > 0 ? nrec : 1) * sizeof(REAL));
+    if (!snap || !row) { free(snap); free(row); return -2; }
+    const int yo = p->ndim == 3 ? np : 0;
+    for (int it = 0; it < nt; it++) {
+        FN(step_ex)(s, it, 1, sv_fwd, row);
+        for (long r = 0; r < nrec; r++) modelled[it + (long)nt * r] = row[r];
+        REAL *P = s->f[9], *Q = s->f[10];
+        for (int iy = 0; iy < ny; iy++)
+            for (int ix = 0; ix < p->nx; ix++)
+                for (int iz = 0; iz < p->nz; iz++) {
+                    long c = (iz + np) + (long)(ix + np) * s->nnz + (long)(iy + yo) * s->nnz * s->nnx;
+                    long o = iz + (long)ix * p->nz + (long)iy * p->nz * p->nx;
+                    snap[(size_t)it * nint + o] = P[c];
+                    /* cal_illumination :671-675 */
+                    REAL v = illum[o] + (P[c] * P[c] + Q[c] * Q[c]);
+                    if (illum_mode == 0 && v <= 0) v = 1;
+                    illum[o] = v;
+                }
+    }
+    const REAL *data = obs ? obs : modelled;
+    for (int it = nt - 1; it >= 0; it--) {
+        /* shot_record(load) :525-528 into g_P and g_Q */
+        for (long r = 0; r < nrec; r++) {
+            int irx = (int)(r % p->rec_nx), iry = (int)(r / p->rec_nx);
+            int ix = p->rec_x0 + irx * p->rec_dx;
+            int iy = p->ndim == 3 ? p->rec_y0 + iry * p->rec_dy : 0;
+            long c = inject_z + (long)ix * g->nnz + (long)iy * g->nnz * g->nnx;
+            g->f[9][c] = data[it + (long)nt * r];
+            g->f[10][c] = data[it + (long)nt * r];
+        }
+        FN(step_ex)(g, it, 0, sv_bwd, NULL);
+        REAL *gP = g->f[9], *gQ = g->f[10];
+        for (int iy = 0; iy < ny; iy++)
+            for (int ix = 0; ix < p->nx; ix++)
+                for (int iz = 0; iz < p->nz; iz++) {
+                    long c = (iz + np) + (long)(ix + np) * g->nnz + (long)(iy + yo) * g->nnz * g->nnx;
+                    long o = iz + (long)ix * p->nz + (long)iy * p->nz * p->nx;
+                    if (illum_mode == 1) illum[o] = illum[o] + (gP[c] * gP[c] + gQ[c] * gQ[c]);
+                    if (it % image_every == 0) img[o] += snap[(size_t)it * nint + o] * gP[c];
+                }
+    }
+    free(snap);
+    free(row);
+    return 0;
+}
+
+/* Misfit gradient J = 1/2 sum (d - dobs)^2 with respect to vp, by the
+ * transpose of every line of the forward step written in scatter form
+ * (the forward: vel then stress then record, GPU or CPU source form).
+ * The absorbing-layer tables are held fixed.  All forward velocity states
+ * are kept in memory (small grids).  resid: if non-NULL it replaces
+ * d - dobs (the adjoint of the linear source-to-data map then follows).
+ * Outputs: *misfit, grad_int (interior nx*ny*nz), grad_src (nt: adjoint
+ * of the per-step source gain, point source form only), d_out ([irec][it]). */
+int FN(orc_fwi_grad)(void *h, const REAL *dobs, const REAL *resid, REAL *misfit, REAL *grad_int,
+                     REAL *grad_src, REAL *d_out)
+{
+    SIM *s = (SIM *)h;
+    if (!s) return -1;
+    const orc_params *p = &s->p;
+    const int nt = p->nt, np = p->npml, three = p->ndim == 3;
+    const int nnx = s->nnx, nny = s->nny, nnz = s->nnz;
+    const long n = s->n, sx = nnz, sy = (long)nnz * nnx;
+    const long nrec = (long)p->rec_nx * (three ? p->rec_ny : 1);
+    const int nvf = three ? 3 : 2;
+    REAL *hist = (REAL *)malloc((size_t)nt * nvf * n * sizeof(REAL));
+    REAL *row = (REAL *)malloc((nrec > 0 ? nrec : 1) * sizeof(REAL));
+    REAL *d = (REAL *)malloc((size_t)(nrec > 0 ? nrec : 1) * nt * sizeof(REAL));
+    if (!hist || !row || !d) { free(hist); free(row); free(d); return -2; }
+    /* forward: fresh state, keep u (v) w after every step */
+    for (int k = 0; k < 11; k++) memset(s->f[k], 0, n * sizeof(REAL));
+    for (int it = 0; it < nt; it++) {
+        FN(step_ex)(s, it, 1, p->sv, row);
+        for (long r = 0; r < nrec; r++) d[it + (long)nt * r] = row[r];
+        memcpy(hist + ((size_t)it * nvf + 0) * n, s->f[0], n * sizeof(REAL));
+        memcpy(hist + ((size_t)it * nvf + 1) * n, s->f[2], n * sizeof(REAL));
+        if (three) memcpy(hist + ((size_t)it * nvf + 2) * n, s->f[1], n * sizeof(REAL));
+    }
+    double J = 0;
+    for (long i = 0; i < nrec * nt; i++) {
+        REAL rr = resid ? resid[i] : d[i] - dobs[i];
+        J += 0.5 * (double)rr * (double)rr;
+    }
+    if (misfit) *misfit = (REAL)J;
+    if (d_out) memcpy(d_out, d, (size_t)nrec * nt * sizeof(REAL));
+    /* per-cell stress coefficients (taper applied) and their vp-derivatives */
+    REAL *C[6], *dC[6];   /* px py pz qx qy qz */
+    for (int k = 0; k < 6; k++) { C[k] = (REAL *)calloc(n, sizeof(REAL)); dC[k] = (REAL *)calloc(n, sizeof(REAL)); }
+    const REAL *ax1 = s->tab[6], *ay1 = s->tab[8], *az1 = s->tab[10];
+    for (int iy = 0; iy < nny; iy++)
+        for (int ix = 0; ix < nnx; ix++)
+            for (int iz = 0; iz < nnz; iz++) {
+                long id = iz + ix * sx + iy * sy;
+                REAL ee, dd;
+                FN(taper)(s, p->sv, ix, iy, iz, id, &ee, &dd);
+                REAL vv = s->vp[id], e2 = 1 + 2 * ee, sd = SQRT_R(1 + 2 * dd);
+                REAL a1y = three ? ay1[iy] : 0;
+                C[0][id] = ax1[ix] * vv * vv * e2 * s->dtx;  dC[0][id] = 2 * ax1[ix] * vv * e2 * s->dtx;
+                C[1][id] = a1y * vv * vv * e2 * s->dty;      dC[1][id] = 2 * a1y * vv * e2 * s->dty;
+                C[2][id] = az1[iz] * vv * vv * sd * s->dtz;  dC[2][id] = 2 * az1[iz] * vv * sd * s->dtz;
+                C[3][id] = ax1[ix] * vv * vv * sd * s->dtx;  dC[3][id] = 2 * ax1[ix] * vv * sd * s->dtx;
+                C[4][id] = a1y * vv * vv * sd * s->dty;      dC[4][id] = 2 * a1y * vv * sd * s->dty;
+                C[5][id] = az1[iz] * vv * vv * s->dtz;       dC[5][id] = 2 * az1[iz] * vv * s->dtz;
+            }
+    /* adjoint state (zero outside the updated region R) */
+    REAL *A[11];   /* u v w px py pz qx qy qz P Q */
+    for (int k = 0; k < 11; k++) A[k] = (REAL *)calloc(n, sizeof(REAL));
+    REAL *g = (REAL *)calloc(n, sizeof(REAL));
+    REAL *ua = A[0], *va = A[1], *wa = A[2], *Pa = A[9], *Qa = A[10];
+    REAL *sa[6] = {A[3], A[4], A[5], A[6], A[7], A[8]};
+    const REAL *c = s->c;
+    const REAL *cx1 = s->tab[0], *cx2 = s->tab[1], *cy1 = s->tab[2], *cy2 = s->tab[3];
+    const REAL *cz1 = s->tab[4], *cz2 = s->tab[5];
+    const REAL *ax2 = s->tab[7], *ay2 = s->tab[9], *az2 = s->tab[11];
+    const int y0 = three ? 4 : 0, y1 = three ? nny - 4 : 1;
+#define INR(ix_, iy_, iz_) ((ix_) >= 4 && (ix_) < nnx - 4 && (iz_) >= 4 && (iz_) < nnz - 4 && \
+                            (!three || ((iy_) >= 4 && (iy_) < nny - 4)))
+    for (int it = nt - 1; it >= 0; it--) {
+        const REAL *uh = hist + ((size_t)it * nvf + 0) * n, *wh = hist + ((size_t)it * nvf + 1) * n;
+        const REAL *vh = three ? hist + ((size_t)it * nvf + 2) * n : NULL;
+        /* record^T */
+        for (long r = 0; r < nrec; r++) {
+            int irx = (int)(r % p->rec_nx), iry = (int)(r / p->rec_nx);
+            long cc = p->rec_z + (long)(p->rec_x0 + irx * p->rec_dx) * sx +
+                      (three ? (long)(p->rec_y0 + iry * p->rec_dy) * sy : 0);
+            REAL rr = resid ? resid[it + (long)nt * r] : d[it + (long)nt * r] - dobs[it + (long)nt * r];
+            if (p->rec_kind == 0) Pa[cc] += rr;
+            else if (p->rec_kind == 1) Pa[cc] += rr + rr;
+            else { Pa[cc] += rr; Qa[cc] += rr; }
+        }
+        /* stress^T: P = sum of p-splits, Q = sum of q-splits, then the splits */
+        for (int iy = y0; iy < y1; iy++)
+            for (int ix = 4; ix < nnx - 4; ix++)
+                for (int iz = 4; iz < nnz - 4; iz++) {
+                    long id = iz + ix * sx + iy * sy;
+                    sa[0][id] += Pa[id]; sa[2][id] += Pa[id]; if (three) sa[1][id] += Pa[id];
+                    sa[3][id] += Qa[id]; sa[5][id] += Qa[id]; if (three) sa[4][id] += Qa[id];
+                    Pa[id] = 0; Qa[id] = 0;
+                }
+        for (int iy = y0; iy < y1; iy++)
+            for (int ix = 4; ix < nnx - 4; ix++)
+                for (int iz = 4; iz < nnz - 4; iz++) {
+                    long id = iz + ix * sx + iy * sy;
+                    REAL xx = 0, yy = 0, zz = 0;
+                    for (int m = 0; m < 4; m++) {
+                        xx += c[m] * (uh[id + m * sx] - uh[id - (m + 1) * sx]);
+                        zz += c[m] * (wh[id + m] - wh[id - m - 1]);
+                        if (three) yy += c[m] * (vh[id + m * sy] - vh[id - (m + 1) * sy]);
+                    }
+                    g[id] -= sa[0][id] * dC[0][id] * xx + sa[3][id] * dC[3][id] * xx +
+                             sa[2][id] * dC[2][id] * zz + sa[5][id] * dC[5][id] * zz;
+                    if (three) g[id] -= sa[1][id] * dC[1][id] * yy + sa[4][id] * dC[4][id] * yy;
+                    REAL fx = sa[0][id] * C[0][id] + sa[3][id] * C[3][id];
+                    REAL fz = sa[2][id] * C[2][id] + sa[5][id] * C[5][id];
+                    REAL fy = three ? sa[1][id] * C[1][id] + sa[4][id] * C[4][id] : 0;
+                    for (int m = 0; m < 4; m++) {
+                        /* px[x] -= C c_m (u[x+m] - u[x-m-1]) */
+                        if (INR(ix + m, iy, iz)) ua[id + m * sx] -= c[m] * fx;
+                        if (INR(ix - m - 1, iy, iz)) ua[id - (m + 1) * sx] += c[m] * fx;
+                        if (INR(ix, iy, iz + m)) wa[id + m] -= c[m] * fz;
+                        if (INR(ix, iy, iz - m - 1)) wa[id - m - 1] += c[m] * fz;
+                        if (three) {
+                            if (INR(ix, iy + m, iz)) va[id + m * sy] -= c[m] * fy;
+                            if (INR(ix, iy - m - 1, iz)) va[id - (m + 1) * sy] += c[m] * fy;
+                        }
+                    }
+                }
+        for (int iy = y0; iy < y1; iy++)
+            for (int ix = 4; ix < nnx - 4; ix++)
+                for (int iz = 4; iz < nnz - 4; iz++) {
+                    long id = iz + ix * sx + iy * sy;
+                    sa[0][id] *= ax2[ix]; sa[3][id] *= ax2[ix];
+                    sa[2][id] *= az2[iz]; sa[5][id] *= az2[iz];
+                    if (three) { sa[1][id] *= ay2[iy]; sa[4][id] *= ay2[iy]; }
+                }
+        /* vel^T: u[x] -= cx1 dtx sum c_m (P[x+m+1] - P[x-m]) */
+        for (int iy = y0; iy < y1; iy++)
+            for (int ix = 4; ix < nnx - 4; ix++)
+                for (int iz = 4; iz < nnz - 4; iz++) {
+                    long id = iz + ix * sx + iy * sy;
+                    REAL hx = cx1[ix] * s->dtx * ua[id];
+                    REAL hz = cz1[iz] * s->dtz * wa[id];
+                    REAL hy = three ? cy1[iy] * s->dty * va[id] : 0;
+                    for (int m = 0; m < 4; m++) {
+                        if (INR(ix + m + 1, iy, iz)) Pa[id + (m + 1) * sx] -= c[m] * hx;
+                        if (INR(ix - m, iy, iz)) Pa[id - m * sx] += c[m] * hx;
+                        if (INR(ix, iy, iz + m + 1)) Qa[id + m + 1] -= c[m] * hz;
+                        if (INR(ix, iy, iz - m)) Qa[id - m] += c[m] * hz;
+                        if (three) {
+                            if (INR(ix, iy + m + 1, iz)) Pa[id + (m + 1) * sy] -= c[m] * hy;
+                            if (INR(ix, iy - m, iz)) Pa[id - m * sy] += c[m] * hy;
+                        }
+                    }
+                }
+        for (int iy = y0; iy < y1; iy++)
+            for (int ix = 4; ix < nnx - 4; ix++)
+                for (int iz = 4; iz < nnz - 4; iz++) {
+                    long id = iz + ix * sx + iy * sy;
+                    ua[id] *= cx2[ix]; wa[id] *= cz2[iz];
+                    if (three) va[id] *= cy2[iy];
+                }
+        /* source^T (point form: P[src] += a, Q[src] += a before the velocity update) */
+        if (grad_src) {
+            long sid = p->sz + (long)p->sx * sx + (three ? (long)p->sy * sy : 0);
+            grad_src[it] = p->src_kind == 0 ? Pa[sid] + Qa[sid] : 0;
+        }
+    }
+#undef INR
+    /* fold the padded-grid gradient onto the interior cells it copies */
+    const int ny = three ? p->ny : 1;
+    for (long j = 0; j < (long)p->nx * ny * p->nz; j++) grad_int[j] = 0;
+    for (int iy = 0; iy < nny; iy++)
+        for (int ix = 0; ix < nnx; ix++)
+            for (int iz = 0; iz < nnz; iz++) {
+                int jx = ix < np ? 0 : (ix >= np + p->nx ? p->nx - 1 : ix - np);
+                int jz = iz < np ? 0 : (iz >= np + p->nz ? p->nz - 1 : iz - np);
+                int jy = 0;
+                if (three) jy = iy < np ? 0 : (iy >= np + ny ? ny - 1 : iy - np);
+                grad_int[jz + (long)jx * p->nz + (long)jy * p->nz * p->nx] += g[iz + ix * sx + iy * sy];
+            }
+    for (int k = 0; k < 6; k++) { free(C[k]); free(dC[k]); }
+    for (int k = 0; k < 11; k++) free(A[k]);
+    free(g); free(hist); free(row); free(d);
+    return 0;
+}
+
+/* Forward with a caller-given source gain per step (linear source map). */
+int FN(orc_set_source)(void *h, const REAL *amp)
+{
+    SIM *s = (SIM *)h;
+    if (!s) return -1;
+    memcpy(s->amp, amp, s->p.nt * sizeof(REAL));
     return 0;
 }
 

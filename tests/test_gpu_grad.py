@@ -1,0 +1,146 @@
+"""HIP adjoint and vp-gradient (vti_grad_*) against the adjoint identity,
+the oracle's FP64 gradient, finite differences and the production forward."""
+import numpy as np
+import pytest
+
+from conftest import layered_model, rel_l2, smooth_model3d
+from geophysics_amd import vti
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _case(ndim, nt=None):
+    if ndim == 2:
+        nx, nz, npd = 40, 34, 8
+        nt = nt or 160
+        vp, e, d = layered_model(nx, nz, seed=3, elo=0.1, ehi=0.2, dlo=0.0, dhi=0.1)
+        cfg = vti.cfg_ref2d(vti.MODE_GPU_REF, nx, nz, 10, 10, npd, 1, 25, nt, 1000)
+        shot = vti.shot_ref2d(cfg, 1, 20, 0, 3, rec_depth=2)
+        p = po.params_gpu2d(nx, nz, 10, 10, npd, 1, 25, 1, 20, 0, 3, nt, 1000, 2)
+    else:
+        nx, ny, nz, npml = 16, 14, 15, 5
+        nt = nt or 70
+        vp, e, d = smooth_model3d(nx, ny, nz, seed=2, sigma=2.0)
+        cfg = vti.cfg_ref3d(nx, ny, nz, 10, 10, 10, npml, 1, 30, nt, 800)
+        shot = vti.shot_ref3d(cfg, 0, 8, 0, 7, 0, 3, 0, rec_depth=2, rec_step=2)
+        p = po.params_gpu3d(nx, ny, nz, 10, 10, 10, npml, 1, 30, 0, 8, 0, 7, 0, 3, 0, nt, 800, 2, 2)
+    return cfg, shot, p, vp, e, d
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_adjoint_dot_product_fp64(ndim):
+    cfg, shot, p, vp, e, d = _case(ndim)
+    rng = np.random.default_rng(7)
+    with vti.Gradient(cfg, fp64=True) as G:
+        G.set_model(vp, e, d)
+        G.set_shot(shot)
+        a = rng.standard_normal(cfg.nt)
+        G.set_source(a)
+        La = G.forward()
+        r = rng.standard_normal(La.shape)
+        LTr = G.run(resid=r, nsnap=4)["grad_src"]
+    lhs, rhs = float(np.sum(La * r)), float(np.sum(a * LTr))
+    rel = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    _say(f"{ndim}D dot test: <La,r>={lhs:.15e} <a,L^T r>={rhs:.15e} rel {rel:.2e}")
+    assert rel <= 1e-6
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_gradient_fp64_matches_oracle(ndim):
+    cfg, shot, p, vp, e, d = _case(ndim)
+    rng = np.random.default_rng(11)
+    vtrue = vp * (1.0 + 0.03 * rng.standard_normal(vp.shape)).astype(np.float32)
+    dobs = po.Sim(p, vtrue, e, d, "f64").run()
+    J0, g_ref, _, _ = po.fwi_grad(po.Sim(p, vp, e, d, "f64"), dobs=dobs)
+    with vti.Gradient(cfg, fp64=True) as G:
+        G.set_model(vp, e, d)
+        G.set_shot(shot)
+        out = G.run(dobs=dobs, nsnap=3)
+    err = rel_l2(out["grad_vp"], g_ref)
+    _say(f"{ndim}D FP64 gradient vs oracle FP64: rel-L2 {err:.2e}, misfit {out['misfit']:.6e} / {J0:.6e}")
+    assert abs(out["misfit"] - J0) <= 1e-10 * J0
+    assert err <= 1e-9
+    with vti.Gradient(cfg, fp64=False) as G32:
+        G32.set_model(vp, e, d)
+        G32.set_shot(shot)
+        out32 = G32.run(dobs=dobs, nsnap=3)
+    err32 = rel_l2(out32["grad_vp"], g_ref)
+    _say(f"{ndim}D FP32 gradient vs oracle FP64: rel-L2 {err32:.2e}")
+    assert err32 <= 1e-3
+
+
+def test_fp32_forward_is_the_production_forward():
+    cfg, shot, p, vp, e, d = _case(2, nt=300)
+    with vti.Gradient(cfg, fp64=False) as G:
+        G.set_model(vp, e, d)
+        G.set_shot(shot)
+        dg = G.forward()
+    with vti.Propagator(cfg, vp, e, d) as prop:
+        prop.set_shot(shot)
+        prop.run()
+        rec = prop.record()
+    assert np.array_equal(dg.astype(np.float32), rec)
+
+
+def test_revolve_schedule_does_not_change_the_gradient():
+    cfg, shot, p, vp, e, d = _case(2, nt=200)
+    rng = np.random.default_rng(3)
+    with vti.Gradient(cfg, fp64=True) as G:
+        G.set_model(vp, e, d)
+        G.set_shot(shot)
+        r = rng.standard_normal((G.nrec, cfg.nt))
+        outs = [G.run(resid=r, nsnap=s) for s in (0, 1, 2, 7)]
+    for o in outs[1:]:
+        assert np.array_equal(o["grad_vp"], outs[0]["grad_vp"])
+        assert o["fwd_steps"] > cfg.nt
+    _say("forward steps for nsnap 0,1,2,7:", [o["fwd_steps"] for o in outs])
+
+
+def test_gradient_finite_differences_32_perturbations():
+    """Central differences of the misfit along 32 random smooth model
+    perturbations against <grad, dm> (FP64 HIP path, tables held fixed)."""
+    from scipy.ndimage import gaussian_filter
+    cfg, shot, p, vp, e, d = _case(2, nt=150)
+    rng = np.random.default_rng(2024)
+    vtrue = vp * (1.0 + 0.03 * rng.standard_normal(vp.shape))
+    with vti.Gradient(cfg, fp64=True) as G:
+        G.set_model(vtrue, e, d)
+        G.set_shot(shot)
+        dobs = G.forward()
+        G.set_model(vp, e, d)
+        g = G.run(dobs=dobs, nsnap=4)["grad_vp"]
+        h = 0.005
+        worst = 0.0
+        for k in range(32):
+            dm = gaussian_filter(rng.standard_normal(vp.shape), 2.0)
+            dm /= np.abs(dm).max()
+            Js = []
+            for sgn in (1, -1):
+                G.set_model(vp.astype(np.float64) + sgn * h * dm, e, d, keep_tables=True)
+                Js.append(0.5 * float(np.sum((G.forward() - dobs) ** 2)))
+            fd = (Js[0] - Js[1]) / (2 * h)
+            an = float(np.sum(g * dm))
+            rel = abs(fd - an) / max(abs(an), 1e-30)
+            worst = max(worst, rel)
+            assert rel <= 1e-6, (k, fd, an)
+    _say(f"32 finite-difference checks: worst relative error {worst:.2e}")
+
+
+def test_torch_gradcheck_over_the_c_abi():
+    import torch
+    from geophysics_amd.autograd import VtiModeling
+    nx, nz, npd, nt = 12, 11, 5, 40
+    vp, e, d = layered_model(nx, nz, seed=1, elo=0.1, ehi=0.2, dlo=0.0, dhi=0.1)
+    cfg = vti.cfg_ref2d(vti.MODE_GPU_REF, nx, nz, 10, 10, npd, 0, 30, nt, 1000)
+    shot = vti.shot_ref2d(cfg, 1, 6, 0, 2, rec_depth=2)
+    with vti.Gradient(cfg, fp64=True) as G:
+        G.set_model(vp, e, d)
+        G.set_shot(shot)
+        x = torch.tensor(vp.astype(np.float64), requires_grad=True)
+        f = lambda v: VtiModeling.apply(v, G, e, d, 3)
+        assert torch.autograd.gradcheck(f, (x,), eps=1e-3, atol=1e-8, rtol=1e-4, fast_mode=True)
