@@ -226,7 +226,7 @@ def main():
             "vel_kernel_ms": round(vel_avg * 1e3, 4), "stress_kernel_ms": round(str_avg * 1e3, 4),
         },
         "roofline": {
-            "bound": "hbm", "kernel": "stress3d_kernel",
+            "bound": "hbm", "kernel": "vti::s3d::stress3d<64, true>",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
         },

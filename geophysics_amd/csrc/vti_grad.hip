@@ -649,7 +649,7 @@ struct Grad : vti_grad {
         if (!have_model || !have_shot) return fail(VTI_ERR_STATE, "vti_grad: model and shot first");
         if (!dobs && !resid) return fail(VTI_ERR_ARG, "vti_grad: observed data or residual needed");
         const int nt = cfg.nt;
-        const int nslot = nsnap > 0 ? std::min(nsnap, nt) : nt;
+        const int nslot = std::max(0, nsnap > 0 ? std::min(nsnap, nt - 1) : nt - 1);
         while ((int)slots.size() < nslot) {
             T *s = nullptr;
             if (hipMalloc(&s, (size_t)nstate() * nvol * sizeof(T)) != hipSuccess)

@@ -11,16 +11,16 @@
 // Work layout.  Every volume is z-fastest with a 128-byte-aligned row pitch
 // (Grid::ldz).  A workgroup is 64 (z) x 4 (x) threads: one wavefront covers
 // 64 consecutive z cells of one x row, so every load and store of a row is a
-// single 256-byte coalesced access.  In 3D each thread owns one (z,x)
-// column and marches the slowest axis y, keeping the 8-deep y-stencil of P
-// (velocity kernel) or v (stress kernel) in a register queue, so each value
-// of those fields is fetched once per kernel; x- and z-neighbours are a few
-// KB away and are served by L1/L2.  Workgroup ids are remapped so that each
-// XCD (workgroups are dealt round-robin over the 8 XCDs) receives a
-// contiguous slab of x-rows and halo rows stay inside one L2.
+// single 256-byte coalesced access.  The 3D kernels (vti_stencil3d.h) give
+// each thread one (z,x) column and a chunk of the slowest axis y, keeping
+// the 8-deep y-stencil of P (velocity kernel) or v (stress kernel) in a
+// register queue; x- and z-neighbours are a few KB away and are served by
+// L1/L2, and workgroup ids are remapped so that each XCD (workgroups are
+// dealt round-robin over the 8 XCDs) receives a contiguous slab of x-rows.
 #include <hip/hip_runtime.h>
 
 #include "vti_internal.h"
+#include "vti_stencil3d.h"
 
 namespace vti {
 namespace {
@@ -34,16 +34,6 @@ __device__ __forceinline__ float cst(int m)
 
 constexpr int TZ = 64;    // z cells per workgroup (one wavefront)
 constexpr int TX = 4;     // x rows per workgroup
-constexpr int NXCD = 8;   // MI355X: 8 XCDs, workgroups dealt round-robin
-
-// Map the hardware workgroup id to a logical tile so that XCD k owns the
-// contiguous logical range [k*per, (k+1)*per).
-__device__ __forceinline__ int xcd_tile(int b, int ntile_padded)
-{
-    const int per = ntile_padded / NXCD;
-    return (b % NXCD) * per + (b / NXCD);
-}
-
 // ---------------------------------------------------------------- 2D ----
 template <bool INJ>
 __global__ __launch_bounds__(256) void vel2d_kernel(VelArgs a)
@@ -126,119 +116,6 @@ __global__ __launch_bounds__(256) void stress2d_kernel(StressArgs a)
             const float r = a.rec_kind == 0 ? Pn : (a.rec_kind == 1 ? Pn + Pn : Pn + Qn);
             a.rec[k / a.rdx] = r;
         }
-    }
-}
-
-// ---------------------------------------------------------------- 3D ----
-template <bool INJ>
-__global__ __launch_bounds__(256) void vel3d_kernel(VelArgs a, int nzt, int ntile, int ntile_p)
-{
-    const Grid g = a.g;
-    const int t = xcd_tile(blockIdx.x, ntile_p);
-    if (t >= ntile) return;
-    const int iz = (t % nzt) * TZ + threadIdx.x;
-    const int ix = (t / nzt) * TX + threadIdx.y;
-    if (iz < 4 || iz >= g.nnz - 4 || ix < 4 || ix >= g.nnx - 4) return;
-    const long long col = iz + (long long)ix * g.ldz;
-    const long long L = g.ldxy;
-    auto ld = [&](const float *f, long long i) -> float {
-        float v = f[i];
-        if (INJ && i == a.src) v = v + a.amp;   // add_source3d GPU_vti3dfd_kernels.cu:85-86
-        return v;
-    };
-    // q[k] = P at y-3+k for the current y
-    float q[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) q[k] = ld(a.P, col + (long long)(1 + k) * L);
-    const int yend = g.nny - 4;
-    for (int iy = 4; iy < yend; ++iy) {
-        const long long id = col + (long long)iy * L;
-        float xx = 0.0f, yy = 0.0f, zz = 0.0f;
-#pragma unroll
-        for (int im = 0; im < 4; ++im) {
-            yy += cst(im) * (q[3 + im + 1] - q[3 - im]);
-            const float p1 = ld(a.P, id + (long long)(im + 1) * g.ldz);
-            const float p0 = im == 0 ? q[3] : ld(a.P, id - (long long)im * g.ldz);
-            xx += cst(im) * (p1 - p0);
-            zz += cst(im) * (ld(a.Q, id + im + 1) - ld(a.Q, id - im));
-        }
-        a.u[id] = a.cx2[ix] * a.u[id] - a.cx1[ix] * a.dtx * xx;
-        a.v[id] = a.cy2[iy] * a.v[id] - a.cy1[iy] * a.dty * yy;
-        a.w[id] = a.cz2[iz] * a.w[id] - a.cz1[iz] * a.dtz * zz;
-#pragma unroll
-        for (int k = 0; k < 7; ++k) q[k] = q[k + 1];
-        if (iy + 1 < yend) q[7] = ld(a.P, id + 5 * L);
-    }
-}
-
-__global__ __launch_bounds__(256) void stress3d_kernel(StressArgs a, int nzt, int ntile, int ntile_p)
-{
-    const Grid g = a.g;
-    const int t = xcd_tile(blockIdx.x, ntile_p);
-    if (t >= ntile) return;
-    const int iz = (t % nzt) * TZ + threadIdx.x;
-    const int ix = (t / nzt) * TX + threadIdx.y;
-    if (iz < 4 || iz >= g.nnz - 4 || ix < 4 || ix >= g.nnx - 4) return;
-    const long long col = iz + (long long)ix * g.ldz;
-    const long long L = g.ldxy;
-    // per-column flags for the rare fused work
-    const bool sv_col = a.sv && abs(ix - a.cx) <= a.svR && abs(iz - a.cz) <= a.svR;
-    const bool src_col = a.gsrc && abs(ix - a.gx) <= 3 && abs(iz - a.gz) <= 3;
-    int rk = -1;
-    if (a.rec && iz == a.rz) {
-        const int k = ix - a.rx0;
-        if (k >= 0 && k % a.rdx == 0 && k / a.rdx < a.rnx) rk = k / a.rdx;
-    }
-    const float ax1 = a.ax1[ix], ax2 = a.ax2[ix], az1 = a.az1[iz], az2 = a.az2[iz];
-    // vq[k] = v at y-4+k for the current y
-    float vq[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) vq[k] = a.v[col + (long long)k * L];
-    const int yend = g.nny - 4;
-    for (int iy = 4; iy < yend; ++iy) {
-        const long long id = col + (long long)iy * L;
-        float xx = 0.0f, yy = 0.0f, zz = 0.0f;
-#pragma unroll
-        for (int im = 0; im < 4; ++im) {
-            yy += cst(im) * (vq[4 + im] - vq[3 - im]);
-            xx += cst(im) * (a.u[id + (long long)im * g.ldz] - a.u[id - (long long)(im + 1) * g.ldz]);
-            zz += cst(im) * (a.w[id + im] - a.w[id - im - 1]);
-        }
-        const float vv = a.vp[id];
-        float ee = a.ep[id], dd = a.de[id];
-        if (sv_col) {
-            const int rx = ix - a.cx, ry = iy - a.cy, rz = iz - a.cz;
-            if (ry <= a.svR && ry >= -a.svR) taper(a, rx * rx + ry * ry + rz * rz, ee, dd);
-        }
-        const float e2 = 1.0f + 2.0f * ee;
-        const float sd = sqrtf(1.0f + 2.0f * dd);
-        const float ay1 = a.ay1[iy], ay2 = a.ay2[iy];
-        const float px = ax2 * a.px[id] - ax1 * vv * vv * e2 * a.dtx * xx;
-        const float py = ay2 * a.py[id] - ay1 * vv * vv * e2 * a.dty * yy;
-        const float pz = az2 * a.pz[id] - az1 * vv * vv * sd * a.dtz * zz;
-        const float qx = ax2 * a.qx[id] - ax1 * vv * vv * sd * a.dtx * xx;
-        const float qy = ay2 * a.qy[id] - ay1 * vv * vv * sd * a.dty * yy;
-        const float qz = az2 * a.qz[id] - az1 * vv * vv * a.dtz * zz;
-        a.px[id] = px; a.py[id] = py; a.pz[id] = pz;
-        a.qx[id] = qx; a.qy[id] = qy; a.qz[id] = qz;
-        float Pn = px + py + pz, Qn = qx + qy + qz;
-        if (src_col && iy == a.gy) {
-            const float s = (float)((double)a.gamp * a.gw[(ix - a.gx + 3) * 7 + (iz - a.gz + 3)]);
-            Pn = px + py + pz + s;
-            Qn = qx + qy + qz + s;
-        }
-        a.P[id] = Pn;
-        a.Q[id] = Qn;
-        if (rk >= 0) {
-            const int j = iy - a.ry0;
-            if (j >= 0 && j % a.rdy == 0 && j / a.rdy < a.rny) {
-                const float r = a.rec_kind == 0 ? Pn : (a.rec_kind == 1 ? Pn + Pn : Pn + Qn);
-                a.rec[rk + (long long)a.rnx * (j / a.rdy)] = r;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 7; ++k) vq[k] = vq[k + 1];
-        if (iy + 1 < yend) vq[7] = a.v[id + 4 * L];
     }
 }
 
@@ -343,6 +220,11 @@ __global__ void copy_kernel(const float4 *__restrict__ src, float4 *__restrict__
 }  // namespace
 
 // ---------------------------------------------------------- launchers ----
+// 3D launches march y in chunks of 64 planes per workgroup (4x more
+// workgroups than whole-column marching at 512^3, 8 halo planes re-read
+// per chunk) with non-temporal access to the fields touched once.
+constexpr int kYChunk = 64;
+
 hipError_t launch_vel(int ndim, const VelArgs &a, hipStream_t s)
 {
     const bool inj = a.src >= 0;
@@ -351,11 +233,9 @@ hipError_t launch_vel(int ndim, const VelArgs &a, hipStream_t s)
         if (inj) hipLaunchKernelGGL(vel2d_kernel<true>, grd, blk, 0, s, a);
         else hipLaunchKernelGGL(vel2d_kernel<false>, grd, blk, 0, s, a);
     } else {
-        const int nzt = (a.g.nnz + TZ - 1) / TZ, nxt = (a.g.nnx + TX - 1) / TX;
-        const int nt = nzt * nxt, ntp = (nt + NXCD - 1) / NXCD * NXCD;
-        dim3 blk(TZ, TX), grd(ntp);
-        if (inj) hipLaunchKernelGGL(vel3d_kernel<true>, grd, blk, 0, s, a, nzt, nt, ntp);
-        else hipLaunchKernelGGL(vel3d_kernel<false>, grd, blk, 0, s, a, nzt, nt, ntp);
+        int nzt, nt, ntp;
+        const int grd = s3d::grid3d(a.g, kYChunk, nzt, nt, ntp);
+        hipLaunchKernelGGL((s3d::vel3d<kYChunk, true>), dim3(grd), dim3(s3d::TZ, s3d::TX), 0, s, a, nzt, nt, ntp);
     }
     return hipGetLastError();
 }
@@ -366,10 +246,9 @@ hipError_t launch_stress(int ndim, const StressArgs &a, hipStream_t s)
         dim3 blk(TZ, TX), grd((a.g.nnz + TZ - 1) / TZ, (a.g.nnx + TX - 1) / TX);
         hipLaunchKernelGGL(stress2d_kernel, grd, blk, 0, s, a);
     } else {
-        const int nzt = (a.g.nnz + TZ - 1) / TZ, nxt = (a.g.nnx + TX - 1) / TX;
-        const int nt = nzt * nxt, ntp = (nt + NXCD - 1) / NXCD * NXCD;
-        dim3 blk(TZ, TX), grd(ntp);
-        hipLaunchKernelGGL(stress3d_kernel, grd, blk, 0, s, a, nzt, nt, ntp);
+        int nzt, nt, ntp;
+        const int grd = s3d::grid3d(a.g, kYChunk, nzt, nt, ntp);
+        hipLaunchKernelGGL((s3d::stress3d<kYChunk, true>), dim3(grd), dim3(s3d::TZ, s3d::TX), 0, s, a, nzt, nt, ntp);
     }
     return hipGetLastError();
 }

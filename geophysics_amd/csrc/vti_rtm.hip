@@ -81,136 +81,6 @@ __global__ void divide_kernel(float *img, const float *__restrict__ illum, size_
 
 }  // namespace
 
-// ---------------------------------------------------------------------
-// Binomial checkpointing: beta(s, t) = C(s+t, s) states reversible with s
-// slots and t sweeps (Griewank & Walther, revolve).
-static double beta(int s, int t)
-{
-    double b = 1.0;
-    for (int i = 1; i <= s; ++i) b = b * (t + i) / i;
-    return b;
-}
-
-static int reps_for(int n, int s)
-{
-    int t = 0;
-    while (beta(s, t) < n) ++t;
-    return t;
-}
-
-// Left part length of the split of n steps with s free slots.
-static int split_len(int n, int s)
-{
-    const int t = reps_for(n, s);
-    const double right = beta(s - 1, t);
-    int m = (int)std::max(1.0, n - right);
-    return std::min(m, n - 1);
-}
-
-int reverse_sweep(int nt, int nslot, const SweepOps &op, long long *fwd_steps)
-{
-    // states X_0..X_nt; X_k = state before step k.  consume(k) receives X_k
-    // in the working state for k = nt .. 1.
-    long long steps = 0;
-    std::vector<int> slot_pos(nslot, -1);   // position stored in each slot
-    auto alloc_slot = [&](int pos) -> int {
-        for (int i = 0; i < nslot; ++i)
-            if (slot_pos[i] < 0) { slot_pos[i] = pos; return i; }
-        return -1;
-    };
-    auto slot_of = [&](int pos) -> int {
-        for (int i = 0; i < nslot; ++i)
-            if (slot_pos[i] == pos) return i;
-        return -1;
-    };
-    auto restore = [&](int pos) -> int {
-        if (pos == 0) return op.zero();
-        const int i = slot_of(pos);
-        if (i < 0) return fail(VTI_ERR_STATE, "reverse_sweep: missing checkpoint");
-        return op.restore(i);
-    };
-    auto advance = [&](int from, int to) -> int {
-        for (int it = from; it < to; ++it) {
-            int st = op.step(it);
-            if (st) return st;
-            ++steps;
-        }
-        return VTI_OK;
-    };
-    // chain of right-recursion splits, stored during the first full sweep
-    std::vector<int> chain;
-    {
-        int l = 0, s = nslot;
-        while (nt - l > 1 && s > 0) {
-            const int m = l + split_len(nt - l, s);
-            chain.push_back(m);
-            l = m;
-            --s;
-        }
-    }
-    // first sweep: 0 -> nt (the caller's forward pass: record, illumination)
-    int st = op.zero();
-    if (st) return st;
-    size_t ci = 0;
-    for (int it = 0; it < nt; ++it) {
-        if (ci < chain.size() && chain[ci] == it) {
-            const int i = alloc_slot(it);
-            if (i < 0) return fail(VTI_ERR_STATE, "reverse_sweep: slot overflow");
-            if ((st = op.store(i))) return st;
-            ++ci;
-        }
-        if ((st = op.first_step(it))) return st;
-        ++steps;
-    }
-    // X_nt is live now: consume it, then reverse the rest
-    if (op.after_first && (st = op.after_first())) return st;
-    if ((st = op.consume(nt))) return st;
-    std::function<int(int, int, int)> rev = [&](int l, int r, int s) -> int {
-        // consume X_r .. X_{l+1}; X_l restorable
-        const int n = r - l;
-        if (n <= 0) return VTI_OK;
-        if (n == 1) {
-            int e = restore(l);
-            if (e) return e;
-            if ((e = advance(l, l + 1))) return e;
-            return op.consume(l + 1);
-        }
-        if (s <= 0) {
-            for (int k = r; k > l; --k) {
-                int e = restore(l);
-                if (e) return e;
-                if ((e = advance(l, k))) return e;
-                if ((e = op.consume(k))) return e;
-            }
-            return VTI_OK;
-        }
-        const int m = l + split_len(n, s);
-        int e = restore(l);
-        if (e) return e;
-        if ((e = advance(l, m))) return e;
-        const int i = alloc_slot(m);
-        if (i < 0) return fail(VTI_ERR_STATE, "reverse_sweep: slot overflow");
-        if ((e = op.store(i))) return e;
-        if ((e = rev(m, r, s - 1))) return e;
-        slot_pos[i] = -1;
-        return rev(l, m, s);
-    };
-    // unwind the chain: innermost segment first (X_nt already consumed)
-    int r = nt - 1;
-    int s_left = nslot - (int)chain.size();
-    for (int k = (int)chain.size() - 1; k >= -1; --k) {
-        const int l = k >= 0 ? chain[k] : 0;
-        if ((st = rev(l, r, s_left))) return st;
-        if (k >= 0) {
-            slot_pos[slot_of(l)] = -1;
-            ++s_left;
-        }
-        r = l;
-    }
-    if (fwd_steps) *fwd_steps = steps;
-    return VTI_OK;
-}
-
 hipError_t launch_inject_rec(vti_ctx *c, const VtiState &s, const float *row, int iz)
 {
     const vti_shot &sh = c->shot;
@@ -325,7 +195,7 @@ int vti_rtm_shot(vti_ctx *c, const vti_rtm_opts *o, const float *obs, long long 
     const int nt = c->cfg.nt, nrec = c->nrec;
     const int inj_z = o->inject_z >= 0 ? o->inject_z : c->shot.rec_z;
     if (inj_z < 4 || inj_z >= c->g.nnz - 4) return fail(VTI_ERR_ARG, "vti_rtm_shot: injection row outside the updated region");
-    const int nslot = o->nsnap > 0 ? std::min(o->nsnap, nt) : nt;
+    const int nslot = std::max(0, o->nsnap > 0 ? std::min(o->nsnap, nt - 1) : nt - 1);
     RtmCache rc;
     struct Guard {
         RtmCache &r;

@@ -102,6 +102,18 @@ def _declare(L):
         "vti_rtm_shot": (C.c_int, [VP, C.POINTER(RtmOpts), FP, C.POINTER(C.c_longlong)]),
         "vti_rtm_image": (C.c_int, [VP, C.c_int, FP, FP]),
         "vti_mute3d_window": (C.c_int, [VP] + [C.c_int] * 7),
+        "vti_batch_create": (C.c_int, [C.c_int, C.POINTER(Cfg), C.POINTER(Shot), C.POINTER(FP),
+                                       C.POINTER(FP), C.POINTER(FP), C.POINTER(VP)]),
+        "vti_batch_destroy": (C.c_int, [VP]),
+        "vti_sweep_check": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_int)]),
+        "vti_batch_config": (C.c_int, [VP, C.c_int, C.c_int, C.c_int]),
+        "vti_batch_get_config": (C.c_int, [VP] + [C.POINTER(C.c_int)] * 4),
+        "vti_batch_autotune": (C.c_int, [VP, C.c_int, DP]),
+        "vti_batch_cells": (C.c_longlong, [VP]),
+        "vti_batch_forward": (C.c_int, [VP, C.c_int, C.c_int]),
+        "vti_batch_rtm": (C.c_int, [VP, C.POINTER(RtmOpts), C.POINTER(FP), C.POINTER(C.c_longlong)]),
+        "vti_batch_image": (C.c_int, [VP, C.c_int, C.c_int, FP, FP]),
+        "vti_batch_record": (C.c_int, [VP, C.c_int, FP]),
         "vti_grad_create": (C.c_int, [C.POINTER(Cfg), C.c_int, C.POINTER(VP)]),
         "vti_grad_destroy": (C.c_int, [VP]),
         "vti_grad_set_model": (C.c_int, [VP, DP, DP, DP, C.c_int]),
@@ -158,6 +170,19 @@ def cfg_ref3d(nx, ny, nz, dx, dy, dz, npml, SV, favg, nt, dt_us):
     c = Cfg()
     check(lib().vti_cfg_ref3d(C.byref(c), nx, ny, nz, dx, dy, dz, npml, SV, favg, nt, dt_us), "cfg_ref3d")
     return c
+
+
+def sweep_check(nt, nslot):
+    """Host-only run of the checkpoint schedule: (forward steps, slots touched)."""
+    steps, slots = C.c_longlong(), C.c_int()
+    check(lib().vti_sweep_check(nt, nslot, C.byref(steps), C.byref(slots)), "sweep_check")
+    return steps.value, slots.value
+
+
+def rtm_defaults(ndim=2):
+    o = RtmOpts()
+    check(lib().vti_rtm_defaults(C.byref(o), ndim), "rtm_defaults")
+    return o
 
 
 def shot_ref2d(cfg, is_, fs, ds, zs, rec_depth=1):
@@ -447,6 +472,91 @@ class Gradient:
     def close(self):
         if getattr(self, "h", None):
             lib().vti_grad_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Independent 2D shots of uneven grids advanced together (vti_batch_*)."""
+
+    SHAPES = [(64, 4), (64, 2), (64, 8), (128, 2), (32, 8)]
+
+    def __init__(self, cfgs, shots, models):
+        """models: list of (vp, eps, dl) arrays shaped (nx, nz) per shot."""
+        n = len(cfgs)
+        assert len(shots) == n and len(models) == n
+        self.cfgs, self.shots = list(cfgs), list(shots)
+        ca = (Cfg * n)(*cfgs)
+        sa = (Shot * n)(*shots)
+        self._keep = []
+        ptrs = []
+        for k in range(3):
+            arrs = [np.ascontiguousarray(m[k], dtype=np.float32) for m in models]
+            self._keep.append(arrs)
+            ptrs.append((FP * n)(*[a.ctypes.data_as(FP) for a in arrs]))
+        h = VP()
+        check(lib().vti_batch_create(n, ca, sa, ptrs[0], ptrs[1], ptrs[2], C.byref(h)), "vti_batch_create")
+        self.h = h
+        self.n = n
+
+    @property
+    def cells(self):
+        return lib().vti_batch_cells(self.h)
+
+    def config(self, shape=0, grid_mult=0, graph_steps=0):
+        check(lib().vti_batch_config(self.h, shape, grid_mult, graph_steps), "batch_config")
+
+    def get_config(self):
+        v = [C.c_int() for _ in range(4)]
+        check(lib().vti_batch_get_config(self.h, *[C.byref(x) for x in v]), "batch_get_config")
+        return {"tile": (v[0].value, v[1].value), "grid_mult": v[2].value, "graph_steps": v[3].value}
+
+    def autotune(self, steps=4):
+        ms = C.c_double()
+        shape = check(lib().vti_batch_autotune(self.h, steps, C.byref(ms)), "batch_autotune")
+        return shape, ms.value
+
+    def forward(self, nsteps, record=True):
+        check(lib().vti_batch_forward(self.h, nsteps, 1 if record else 0), "batch_forward")
+
+    def rtm(self, opts, obs=None):
+        steps = C.c_longlong()
+        if obs is None:
+            p = None
+        else:
+            arrs = [np.ascontiguousarray(o, dtype=np.float32) for o in obs]
+            self._obs = arrs
+            p = (FP * self.n)(*[a.ctypes.data_as(FP) for a in arrs])
+        check(lib().vti_batch_rtm(self.h, C.byref(opts), p, C.byref(steps)), "batch_rtm")
+        return steps.value
+
+    def image(self, s, finalize=False):
+        c = self.cfgs[s]
+        img = np.zeros((c.nx, c.nz), np.float32)
+        ill = np.zeros((c.nx, c.nz), np.float32)
+        check(lib().vti_batch_image(self.h, s, 1 if finalize else 0, img.ctypes.data_as(FP),
+                                    ill.ctypes.data_as(FP)), "batch_image")
+        return img, ill
+
+    def record(self, s):
+        out = np.zeros((self.shots[s].rec_nx, self.cfgs[s].nt), np.float32)
+        check(lib().vti_batch_record(self.h, s, out.ctypes.data_as(FP)), "batch_record")
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vti_batch_destroy(self.h)
             self.h = None
 
     def __enter__(self):

@@ -209,6 +209,47 @@ int vti_rtm_image(vti_ctx *ctx, int finalize, float *img, float *illum);
  * (Toa_gpu_3dvti_rtm_adcigs_1orderfunction.cu:353-381). */
 int vti_mute3d_window(vti_ctx *ctx, int fsx, int dsx, int fsy, int dsy,
                       int fsz, int is, int tt);
+/* The checkpoint schedule of vti_rtm_shot / vti_batch_rtm / vti_grad_run
+ * run on positions instead of wavefields (host only, no device needed):
+ * VTI_OK when the reverse sweep receives X_nt .. X_1 in order with at most
+ * nslot slots; *fwd_steps the forward steps it takes, *max_slots the slots
+ * it touched.  Replaces the reference's boundary re-injection
+ * (Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:1423-1437), which has no schedule. */
+int vti_sweep_check(int nt, int nslot, long long *fwd_steps, int *max_slots);
+
+/* ---- batches of independent 2D shots ------------------------------------- */
+/* Many 2D shots, each with its own grid, model and geometry (GPU form),
+ * advanced together: one persistent work-list launch per phase covers every
+ * tile of every shot; the time index lives on the device so a block of steps
+ * is a replayable hipGraph; the tile shape and grid size come from an
+ * autotuner.  Replaces the reference's serial shot loop with ~15 memsets per
+ * shot (GPU_vti2dfd_kernels.cu:404-444) for the RTM of
+ * Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:1350-1484.  Every shot's record and
+ * image equal the single-shot path's bit for bit. */
+typedef struct vti_batch vti_batch;
+/* cfgs/shots/vp/eps/del: nshot entries; models unpadded, host memory. */
+int vti_batch_create(int nshot, const vti_cfg *cfgs, const vti_shot *shots,
+                     const float *const *vp, const float *const *eps,
+                     const float *const *del, vti_batch **out);
+int vti_batch_destroy(vti_batch *b);
+/* tile_shape: 0..4 = 64x4, 64x2, 64x8, 128x2, 32x8 (z x x); grid_mult:
+ * persistent grid of grid_mult*256 workgroups (0 = one per tile);
+ * graph_steps: steps per captured hipGraph block (0/1 = direct launches). */
+int vti_batch_config(vti_batch *b, int tile_shape, int grid_mult, int graph_steps);
+int vti_batch_get_config(vti_batch *b, int *tile_z, int *tile_x, int *grid_mult,
+                         int *graph_steps);
+/* Times `steps` forward steps for every shape and grid size, keeps the
+ * fastest; returns the chosen shape index, *best_ms per step. */
+int vti_batch_autotune(vti_batch *b, int steps, double *best_ms);
+/* Updated cells per step summed over the shots. */
+long long vti_batch_cells(const vti_batch *b);
+/* nsteps forward steps of every shot from rest; record: keep the records. */
+int vti_batch_forward(vti_batch *b, int nsteps, int record);
+/* RTM of all shots in lockstep (source illumination, illum_mode 0). */
+int vti_batch_rtm(vti_batch *b, const vti_rtm_opts *opts, const float *const *obs,
+                  long long *fwd_steps);
+int vti_batch_image(vti_batch *b, int shot, int finalize, float *img, float *illum);
+int vti_batch_record(vti_batch *b, int shot, float *dst);
 
 /* ---- adjoint and FWI gradient -------------------------------------------- */
 /* No counterpart in the reference for VTI; the reference's FWI gradient is

@@ -98,6 +98,8 @@ def test_revolve_schedule_does_not_change_the_gradient():
     for o in outs[1:]:
         assert np.array_equal(o["grad_vp"], outs[0]["grad_vp"])
         assert o["fwd_steps"] > cfg.nt
+    assert outs[0]["fwd_steps"] == cfg.nt
+    assert [o["fwd_steps"] for o in outs] == [vti.sweep_check(cfg.nt, s)[0] for s in (cfg.nt, 1, 2, 7)]
     _say("forward steps for nsnap 0,1,2,7:", [o["fwd_steps"] for o in outs])
 
 

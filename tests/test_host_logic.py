@@ -105,3 +105,44 @@ def test_shot_helpers_equal_oracle_3d():
     for k in ("sx", "sy", "sz", "sv_cx", "sv_cy", "sv_cz", "rec_z", "rec_x0", "rec_dx", "rec_nx",
               "rec_y0", "rec_dy", "rec_ny"):
         assert getattr(s, k) == getattr(p, k), k
+
+
+def _revolve_min_steps(nt, s):
+    """Least forward steps to hand X_nt .. X_1 to the reverse sweep with s
+    slots and X_0 free: Griewank & Walther's bound for nt+1 states and s+1
+    checkpoints, r*(nt+1) - C(s+2+r-1, s+2), r least with C(s+1+r, s+1) >= nt+1."""
+    from math import comb
+    l, c = nt + 1, s + 1
+    r = 0
+    while comb(c + r, c) < l:
+        r += 1
+    return r * l - (comb(c + 1 + r - 1, c + 1) if r >= 1 else 0)
+
+
+def _dp_min_steps(nmax, smax):
+    """Exhaustive cost of the checkpoint recursion: advance j steps, keep
+    X_j in a slot, reverse the right part with one slot fewer, consume X_j,
+    reverse the left part; with no slot left, recompute each state from X_l."""
+    cost = [[0] * (smax + 1) for _ in range(nmax + 1)]
+    for n in range(1, nmax + 1):
+        cost[n][0] = n * (n + 1) // 2
+        for s in range(1, smax + 1):
+            cost[n][s] = min(j + (cost[n - j][s - 1] if j < n else 0) + cost[j - 1][s] for j in range(1, n + 1))
+    return cost
+
+
+def test_checkpoint_schedule_is_optimal_and_ordered():
+    """The reverse sweep hands over X_nt .. X_1 in order (checked inside
+    vti_sweep_check) within the slot budget, at the least possible number
+    of forward steps."""
+    dp = _dp_min_steps(90, 6)
+    for s in range(0, 7):
+        for nt in range(1, 91):
+            steps, used = vti.sweep_check(nt, s)
+            assert used <= s
+            assert steps == dp[nt][s] == _revolve_min_steps(nt, s), (nt, s)
+    for nt, s in [(400, 3), (2000, 40), (4000, 40), (4000, 100), (20000, 20), (3000, 3000)]:
+        steps, used = vti.sweep_check(nt, s)
+        assert steps == _revolve_min_steps(nt, min(s, nt - 1)) and used <= s, (nt, s)
+    assert vti.sweep_check(400, 400)[0] == 400      # every state kept: no recomputation
+    assert vti.sweep_check(4000, 40)[0] == 11057
