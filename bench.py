@@ -127,6 +127,214 @@ def copy_calibration(torch, vti, nbytes=4 << 30, reps=20):
                       "read_plus_write_GBs": round(2 * nbytes / ms / 1e6, 1)}), flush=True)
 
 
+def layered_fault_model(nx, nz, rng):
+    """Flat layers (vp increasing with depth, 1500..3500 m/s; eps in
+    [0, 0.2], delta in [0, min(eps, 0.1)]) cut by one dipping normal fault
+    with a throw of 5..25 cells.  (nx, nz) arrays, z fastest."""
+    nl = int(rng.integers(4, 8))
+    tops = np.sort(rng.uniform(0.08, 0.95, nl - 1)) * nz
+    vp_l = np.sort(rng.uniform(1500.0, 3500.0, nl))
+    ep_l = rng.uniform(0.0, 0.2, nl)
+    dl_l = np.minimum(rng.uniform(0.0, 0.1, nl), ep_l)
+    z = np.arange(nz, dtype=np.float64)[None, :]
+    x = np.arange(nx, dtype=np.float64)[:, None]
+    x0 = rng.uniform(0.3, 0.7) * nx
+    dip = np.tan(np.radians(rng.uniform(50.0, 75.0)))
+    throw = rng.uniform(5.0, 25.0)
+    zz = np.where(x > x0 + (z / dip), z - throw, z)          # hanging wall shifted up
+    lay = np.searchsorted(tops, np.clip(zz, 0, nz - 1))
+    f = lambda v: np.ascontiguousarray(v[lay].astype(np.float32))
+    return f(vp_l), f(ep_l), f(dl_l)
+
+
+def bench_batch_rtm(args, torch, vti):
+    """Batched 2D VTI RTM: independent shots of uneven grids in one
+    persistent launch per phase (nx in [300, 1200], nz in [200, 800],
+    dx 10 m, dt 1 ms, 20-cell boundary, Ricker 10-25 Hz, every surface cell
+    a receiver), source field from binomial checkpoints, receiver
+    back-propagation, zero-lag imaging with source illumination."""
+    nshot = args.shots or 256
+    nt = args.nt or 3000
+    rng = np.random.default_rng(args.seed)
+    cfgs, shots, models = [], [], []
+    for k in range(nshot):
+        nx, nz = int(rng.integers(300, 1201)), int(rng.integers(200, 801))
+        favg = float(rng.uniform(10.0, 25.0))
+        cfg = vti.cfg_ref2d(vti.MODE_GPU_REF, nx, nz, 10, 10, 20, 1, favg, nt, 1000)
+        cfgs.append(cfg)
+        shots.append(vti.shot_ref2d(cfg, 1, nx // 2, 0, 2, rec_depth=2))
+        models.append(layered_fault_model(nx, nz, rng))
+    log(f"[bench] batch RTM: {nshot} shots, nt {nt}")
+    B = vti.Batch(cfgs, shots, models)
+    shape, ms_tune = B.autotune(steps=min(10, nt))
+    cfgd = B.get_config()
+    log(f"[bench] autotuned tile {cfgd['tile']} grid_mult {cfgd['grid_mult']}: {ms_tune:.3f} ms/step")
+    o = vti.rtm_defaults(2)
+    o.nsnap = 32
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fwd = B.rtm(o, None)
+    t1 = time.perf_counter()
+    secs = t1 - t0
+    cells = B.cells
+    updates = cells * (fwd + nt)           # forward (incl. recomputation) + receiver steps
+    out = {
+        "metric": "batched 2D VTI RTM throughput (Gcell-updates/s over forward, recomputation and receiver steps)",
+        "value": round(updates / secs / 1e9, 3), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": nt,
+        "warmup": 0, "ms_per_step": round(secs / nt * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{nshot} independent 2D shots, nx in [300,1200], nz in [200,800], dx 10 m, "
+                               f"nt {nt}, dt 1 ms, Ricker 10-25 Hz, layered models with a dipping fault",
+                   "shots": nshot, "cells_per_step": cells, "forward_steps": fwd, "checkpoint_slots": o.nsnap,
+                   "tile": list(cfgd["tile"]), "grid_mult": cfgd["grid_mult"], "seconds": round(secs, 3),
+                   "shots_per_s": round(nshot / secs, 3)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    B.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_fwi_grad(args, torch, vti):
+    """2D VTI FWI gradient (FP32 hand-written adjoint, 40 checkpoint slots):
+    1000 x 300 grid, dx 12.5 m, 4000 steps, dt 1 ms, shots 15 cells apart;
+    true model layered with two Gaussian anomalies (+-300 m/s, radius 20
+    cells), starting model the true one smoothed with sigma 15."""
+    from scipy.ndimage import gaussian_filter
+    nshot = args.shots or 64
+    nt = args.nt or 4000
+    nx, nz, dx = 1000, 300, 12.5
+    rng = np.random.default_rng(args.seed)
+    vp, ep, dl = layered_fault_model(nx, nz, rng)
+    X, Z = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+    for sgn, (cx, cz) in ((1.0, (0.35 * nx, 0.5 * nz)), (-1.0, (0.65 * nx, 0.6 * nz))):
+        vp = vp + (sgn * 300.0 * np.exp(-((X - cx) ** 2 + (Z - cz) ** 2) / (2 * 20.0 ** 2))).astype(np.float32)
+    v0 = gaussian_filter(vp.astype(np.float64), 15.0, mode="nearest")
+    cfg = vti.cfg_ref2d(vti.MODE_GPU_REF, nx, nz, dx, dx, 20, 0, 10, nt, 1000)
+    log(f"[bench] FWI gradient: {nshot} shots, nt {nt}, grid {nx}x{nz}")
+    G = vti.Gradient(cfg, fp64=False)
+    total = np.zeros((nx, nz))
+    misfit = 0.0
+    t_obs = t_grad = 0.0
+    steps = 0
+    for k in range(nshot):
+        sx = 20 + 15 * k
+        if sx >= nx - 20:
+            break
+        shot = vti.shot_ref2d(cfg, 1, sx, 0, 2, rec_depth=2)
+        G.set_model(vp, ep, dl)
+        G.set_shot(shot)
+        t0 = time.perf_counter()
+        dobs = G.forward()
+        t1 = time.perf_counter()
+        G.set_model(v0, ep, dl)
+        r = G.run(dobs=dobs, nsnap=40)
+        t2 = time.perf_counter()
+        t_obs += t1 - t0
+        t_grad += t2 - t1
+        total += r["grad_vp"]
+        misfit += r["misfit"]
+        steps = r["fwd_steps"]
+        log(f"[bench] shot {k}: misfit {r['misfit']:.4e}, {t2 - t1:.2f} s")
+    G.close()
+    done = k + 1
+    cells = (nx + 40 - 8) * (nz + 40 - 8)
+    out = {
+        "metric": "2D VTI FWI gradient time per shot (s), FP32 adjoint with 40 checkpoint slots",
+        "value": round(t_grad / done, 4), "unit": "s/shot", "n_gpus": 1, "steps": nt, "warmup": 0,
+        "ms_per_step": round(t_grad / done / nt * 1e3, 4), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{nx}x{nz} grid, dx {dx} m, {nt} steps, dt 1 ms, {done} shots 15 cells apart, "
+                               "layered model with two Gaussian anomalies, start = smoothed (sigma 15)",
+                   "shots": done, "forward_steps_per_shot": steps, "misfit": misfit,
+                   "observed_data_s_per_shot": round(t_obs / done, 4),
+                   "cells_per_step": cells, "grad_norm": float(np.linalg.norm(total))},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_rtm3d_learned(args, torch, vti):
+    """3D RTM with learned snapshot compression against plain checkpoint
+    recomputation: 384^3 grid, dx 10 m, 4000 steps, dt 0.8 ms, smoothed
+    random vp / eps / delta as the headline workload; snapshots every 4
+    steps through the autoencoder, trained online on the first 3 shots'
+    snapshots, applied to the following shots; per-shot wall time and image
+    error against the plain path imaging the same steps."""
+    n = args.size if args.size != 512 else 384
+    nt = args.nt or 4000
+    nshot = args.shots or 19
+    npml = args.npml
+    log(f"[bench] learned RTM: {n}^3, nt {nt}, {nshot} shots")
+    vp, eps, dl = make_model(n, args.seed, torch.device("cuda:0"))
+    torch.cuda.synchronize()
+    cfg = vti.cfg_ref3d(n, n, n, 10, 10, 10, npml, 1, 15, nt, 800)
+    prop = vti.Propagator(cfg)
+    prop.set_model(vp.data_ptr(), eps.data_ptr(), dl.data_ptr(), on_device=True)
+    del vp, eps, dl
+    torch.cuda.empty_cache()
+    rng = np.random.default_rng(args.seed)
+    pos = [(int(rng.integers(n // 4, 3 * n // 4)), int(rng.integers(n // 4, 3 * n // 4))) for _ in range(nshot)]
+    A = vti.Autoencoder(crop=64, batch=4, seed=args.seed)
+    A.set_lr(1e-3)
+    # plain path: checkpoint slots in the same memory as the latent store
+    lat_bytes = (nt // 4) * vti.Autoencoder.latent_elems(n) * 2
+    state_bytes = 11 * 4 * (n + 2 * npml) ** 2 * ((n + 2 * npml + 31) // 32 * 32)
+    nsnap_equal = max(1, int(lat_bytes // state_bytes))
+    rows = []
+    t_train = 0.0
+    ntrain = min(3, nshot)
+    for k, (cx, cy) in enumerate(pos):
+        prop.set_shot(vti.shot_ref3d(cfg, 0, cx, 0, cy, 0, 1, 0, rec_depth=3, rec_step=2))
+        if k < ntrain:
+            t0 = time.perf_counter()
+            loss = prop.ae_train_shot(A, save_every=4, nsteps=2)
+            t_train += time.perf_counter() - t0
+            log(f"[bench] training shot {k}: mean loss {loss:.4e}")
+            continue
+        o = prop.rtm_defaults()
+        o.image_every = 4
+        o.nsnap = nsnap_equal
+        prop.rtm_reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fsteps = prop.rtm_shot(o)
+        prop.sync()
+        t_plain = time.perf_counter() - t0
+        img_p, _ = prop.rtm_image(finalize=False)
+        lo = vti.learned_defaults()
+        lo.check_every, lo.ck_every = 16, 0
+        prop.rtm_reset()
+        t0 = time.perf_counter()
+        st = prop.rtm_shot_learned(A, prop.rtm_defaults(), lo, None)
+        t_learn = time.perf_counter() - t0
+        img_l, _ = prop.rtm_image(finalize=False)
+        err = float(np.linalg.norm(img_l - img_p) / max(np.linalg.norm(img_p), 1e-30))
+        rows.append((t_plain, t_learn, err, fsteps, st))
+        log(f"[bench] shot {k}: plain {t_plain:.2f} s ({fsteps} fwd steps, {nsnap_equal} slots), "
+            f"learned {t_learn:.2f} s, image rel-L2 {err:.3e}, snapshot err {st['mean_checked_err']:.3e}")
+    prop.close()
+    A.close()
+    tp = float(np.mean([r[0] for r in rows])) if rows else 0.0
+    tl = float(np.mean([r[1] for r in rows])) if rows else 0.0
+    out = {
+        "metric": "3D RTM shot time, learned snapshot compression vs plain checkpoint recomputation at equal memory",
+        "value": round(tp / tl, 3) if tl else None, "unit": "speedup (plain s / learned s)", "n_gpus": 1,
+        "steps": nt, "warmup": 0, "ms_per_step": round(tl / nt * 1e3, 4) if rows else None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16", "data": "synthetic",
+        "config": {"workload": f"3D VTI RTM {n}^3, {nt} steps, dt 0.8 ms, {ntrain} training shots then "
+                               f"{len(rows)} shots imaged both ways",
+                   "plain_s_per_shot": round(tp, 3), "learned_s_per_shot": round(tl, 3),
+                   "image_rel_l2_max": max((r[2] for r in rows), default=None),
+                   "image_rel_l2_mean": float(np.mean([r[2] for r in rows])) if rows else None,
+                   "plain_checkpoint_slots": nsnap_equal, "plain_forward_steps": rows[0][3] if rows else None,
+                   "latent_store_GB": round(lat_bytes / 1e9, 3), "training_s": round(t_train, 2),
+                   "encode_ms_per_shot": round(float(np.mean([r[4]["encode_ms"] for r in rows])), 1) if rows else None,
+                   "decode_ms_per_shot": round(float(np.mean([r[4]["decode_ms"] for r in rows])), 1) if rows else None},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +346,12 @@ def main():
     ap.add_argument("--seed", type=int, default=2025)
     ap.add_argument("--copy-calib", action="store_true",
                     help="time the plain device copy kernel (bandwidth / counter calibration)")
+    ap.add_argument("--workload", default="fd3d", choices=["fd3d", "batch-rtm", "fwi-grad", "rtm3d-learned"],
+                    help="fd3d: the headline 3D FD line (default); the others time the batched 2D RTM, "
+                         "the 2D FWI gradient and the 3D RTM with learned snapshot compression")
+    ap.add_argument("--shots", type=int, default=0, help="shots of the batch / FWI / learned-RTM workloads "
+                    "(0: the workload's own count)")
+    ap.add_argument("--nt", type=int, default=0, help="time steps of the non-default workloads (0: their own)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -152,6 +366,12 @@ def main():
     if args.copy_calib:
         copy_calibration(torch, vti)
         return
+    if args.workload == "batch-rtm":
+        return bench_batch_rtm(args, torch, vti)
+    if args.workload == "fwi-grad":
+        return bench_fwi_grad(args, torch, vti)
+    if args.workload == "rtm3d-learned":
+        return bench_rtm3d_learned(args, torch, vti)
     n, npml = args.size, args.npml
     nt = max(args.steps, 2000)
     log(f"[bench] building {n}^3 model on {torch.cuda.get_device_name(0)}")

@@ -1,0 +1,26 @@
+// vti_ae.h — internal entry points of the snapshot autoencoder used by the
+// learned-compression RTM (vti_rtm_learned.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/vti.h"
+
+namespace vti {
+
+// Encode the n^3 box at P[off + y*sy + x*sx + z] (fp32) into latent
+// ((n/8)^3 x 64 bf16); *scale_dev receives the box's largest |P|.
+int ae_encode(vti_ae *A, const float *P, long long off, long long sy, long long sx, int n, uint16_t *latent,
+              float *scale_dev, hipStream_t s);
+// Decode into a dense n^3 fp32 box (z fastest), rescaled by *scale_dev.
+int ae_decode(vti_ae *A, const uint16_t *latent, const float *scale_dev, int n, float *out, hipStream_t s);
+// nsteps Adam steps on random crops of the (ny, nx, nz) box of P.
+int ae_train_field(vti_ae *A, const float *P, long long off, long long sy, long long sx, int ny, int nx, int nz,
+                   int nsteps, float *loss);
+hipStream_t ae_stream(vti_ae *A);
+
+}  // namespace vti
+
+extern "C" int vti_ae_set_params_padded_(vti_ae *A, const float *h);

@@ -1,0 +1,577 @@
+// vti_ae.hip — the snapshot autoencoder: parameters, online training on FD
+// snapshots, and full-volume encode / decode of the source wavefield.
+//
+// Network (channels-last volumes, bf16 weights and activations, fp32
+// accumulation, fp32 master weights and Adam moments):
+//   E1 down 1->16, E2 down 16->32, E3 down 32->64 (leaky ReLU 1/8),
+//   E4 same 64->64 (linear)              -> latent: 64 ch at N/8, bf16
+//   D4 same 64->64, D3 up 64->32, D2 up 32->16 (leaky ReLU), D1 up 16->1.
+// The latent holds N^3/8 bf16 values for N^3 fp32 samples: 16x fewer
+// bytes.  Each snapshot is scaled by its largest |P| before encoding; the
+// scale is kept beside the latent.
+//
+// No counterpart in the reference: its RTM keeps no snapshots and re-runs
+// the forward operator (Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:1423-1437).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "vti_ae.h"
+#include "vti_internal.h"
+#include "vti_nn.h"
+
+using namespace vti;
+using namespace vti::nn;
+
+namespace {
+
+struct LayerSpec {
+    int kind, ci, co, act;
+};
+constexpr LayerSpec kNet[] = {
+    {DOWN, 1, 16, 1}, {DOWN, 16, 32, 1}, {DOWN, 32, 64, 1}, {SAME, 64, 64, 0},
+    {SAME, 64, 64, 1}, {UP, 64, 32, 1}, {UP, 32, 16, 1}, {UP, 16, 1, 0},
+};
+constexpr int kNL = 8, kNEnc = 4;
+
+struct Layer {
+    LayerSpec s;
+    int N, K, Npad, Kpad;        // forward matrix (rows, cols), padded
+    int Nt, Kt, Ntpad, Ktpad;    // backward-data matrix
+    size_t w_off, b_off;         // offsets into the parameter arrays (floats)
+    size_t wt_off;               // offset into the backward-data matrices
+};
+
+int nt_for(int npad) { return std::min(4, npad / 16); }
+
+}  // namespace
+
+struct vti_ae {
+    Layer L[kNL];
+    size_t nparam = 0, nwt = 0;
+    float *master = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+    uint16_t *wb = nullptr, *wt = nullptr;
+    int crop = 0, batch = 0;
+    // training buffers: act[i] = input of layer i (bf16); out fp32; g[i] = dL/dpre of layer i
+    uint16_t *act[kNL] = {}, *gbuf[kNL] = {};
+    float *out = nullptr, *target = nullptr, *dev_loss = nullptr, *scale = nullptr;
+    int *org = nullptr;
+    long long step = 0;
+    hipStream_t stream = nullptr;
+    std::mt19937_64 rng;
+    // full-volume buffers (grown on demand)
+    long long vol_cap = 0;
+    uint16_t *vact[kNL + 1] = {};
+    float lr = 1e-3f;
+
+    ~vti_ae()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto *p : {master, m, v, grad, out, target, dev_loss, scale}) (void)hipFree(p);
+        (void)hipFree(wb);
+        (void)hipFree(wt);
+        (void)hipFree(org);
+        for (int i = 0; i < kNL; ++i) { (void)hipFree(act[i]); (void)hipFree(gbuf[i]); }
+        for (auto *p : vact) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int aefail(int code, const std::string &m) { return fail(code, m); }
+#define AE_HIP(x)                                                                 \
+    do {                                                                          \
+        hipError_t e_ = (x);                                                      \
+        if (e_ != hipSuccess) return aefail(VTI_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// grid of the input of layer i for a cubic input of side n
+int side_in(int i, int n)
+{
+    int s = n;
+    for (int j = 0; j < i; ++j) {
+        if (kNet[j].kind == DOWN) s /= 2;
+        else if (kNet[j].kind == UP) s *= 2;
+    }
+    return s;
+}
+
+long long vol(int s) { return (long long)s * s * s; }
+
+// forward GEMM of layer i over nb cubes of input side s (source bf16 x or
+// fp32 xf with strides); output bf16 y or fp32 yf
+Gemm fwd_gemm(const vti_ae *A, int i, int nb, int s, const uint16_t *x, uint16_t *y, float *yf)
+{
+    const Layer &l = A->L[i];
+    Gemm g{};
+    g.kind = l.s.kind;
+    g.nb = nb;
+    const int gs = l.s.kind == DOWN ? s / 2 : s;
+    g.gy = g.gx = g.gz = gs;
+    g.iy = g.ix = g.iz = s;
+    g.ci = l.s.ci;
+    g.K = l.K; g.Kpad = l.Kpad; g.N = l.N; g.Npad = l.Npad; g.NT = nt_for(l.Npad);
+    g.x = x;
+    g.xscale = 1.0f;
+    g.w = A->wb + l.w_off;
+    g.bias = A->master + l.b_off;
+    g.co = l.s.co;
+    g.act = l.s.act;
+    g.y = y;
+    g.yf = yf;
+    g.yscale = 1.0f;
+    return g;
+}
+
+// backward-data GEMM of layer i: gradient at its output (dpre, bf16) ->
+// dL/dpre of layer i-1 (gated by layer i-1's activation) at its input
+Gemm bwd_gemm(const vti_ae *A, int i, int nb, int s, const uint16_t *dpre, uint16_t *dx, const uint16_t *gate)
+{
+    const Layer &l = A->L[i];
+    Gemm g{};
+    g.nb = nb;
+    const int so = l.s.kind == DOWN ? s / 2 : (l.s.kind == UP ? 2 * s : s);   // output side of the layer
+    if (l.s.kind == DOWN) {          // up-kind GEMM over the output grid, scattered to the input grid
+        g.kind = UP;
+        g.gy = g.gx = g.gz = so;
+        g.iy = g.ix = g.iz = so;
+        g.ci = l.s.co;
+        g.co = l.s.ci;
+    } else if (l.s.kind == UP) {     // down-kind gather over the output grid
+        g.kind = DOWN;
+        g.gy = g.gx = g.gz = s;
+        g.iy = g.ix = g.iz = so;
+        g.ci = l.s.co;
+        g.co = l.s.ci;
+    } else {
+        g.kind = SAME;
+        g.gy = g.gx = g.gz = s;
+        g.iy = g.ix = g.iz = s;
+        g.ci = l.s.co;
+        g.co = l.s.ci;
+    }
+    g.K = l.Kt; g.Kpad = l.Ktpad; g.N = l.Nt; g.Npad = l.Ntpad; g.NT = nt_for(l.Ntpad);
+    g.x = dpre;
+    g.xscale = 1.0f;
+    g.w = A->wt + l.wt_off;
+    g.bias = nullptr;
+    g.act = 0;
+    g.y = dx;
+    g.yscale = 1.0f;
+    g.gate = gate;
+    return g;
+}
+
+WGrad wgrad_for(const vti_ae *A, int i, int nb, int s, const uint16_t *x, const uint16_t *dpre)
+{
+    const Layer &l = A->L[i];
+    WGrad w{};
+    w.kind = l.s.kind;
+    w.nb = nb;
+    const int gs = l.s.kind == DOWN ? s / 2 : s;
+    w.gy = w.gx = w.gz = gs;
+    w.iy = w.ix = w.iz = s;
+    w.ci = l.s.ci;
+    w.K = l.K; w.Kpad = l.Kpad; w.N = l.N; w.Npad = l.Npad; w.co = l.s.co;
+    w.x = x;
+    w.xscale = 1.0f;
+    w.d = dpre;
+    w.dw = A->grad + l.w_off;
+    w.chunk = 512;
+    return w;
+}
+
+int check_shape(int n)
+{
+    if (n < 8 || n % 8) return aefail(VTI_ERR_ARG, "autoencoder: volume sides must be positive multiples of 8");
+    return VTI_OK;
+}
+
+// forward over the training buffers (x already in act[0]); out = fp32
+int train_forward(vti_ae *A)
+{
+    for (int i = 0; i < kNL; ++i) {
+        const int s = side_in(i, A->crop);
+        Gemm g = fwd_gemm(A, i, A->batch, s, A->act[i], i + 1 < kNL ? A->act[i + 1] : nullptr,
+                          i + 1 < kNL ? nullptr : A->out);
+        AE_HIP(launch_gemm(g, A->stream));
+    }
+    return VTI_OK;
+}
+
+int rebuild_wt(vti_ae *A)
+{
+    for (int i = 0; i < kNL; ++i) {
+        const Layer &l = A->L[i];
+        AE_HIP(launch_wtrans(l.s.kind, l.s.ci, l.s.co, A->wb + l.w_off, l.Kpad, A->wt + l.wt_off, l.Ktpad, l.Ntpad,
+                             A->stream));
+    }
+    return VTI_OK;
+}
+
+// one training step on the crops in A->target (fp32, batch x crop^3)
+int train_step(vti_ae *A, int apply, float *loss_out)
+{
+    const int n = A->crop, nb = A->batch;
+    const long long nin = (long long)nb * vol(n);
+    AE_HIP(launch_f32_to_bf16(A->target, A->act[0], nin, A->stream));
+    int st = train_forward(A);
+    if (st) return st;
+    AE_HIP(launch_fill_f32(A->dev_loss, 1, 0.0f, A->stream));
+    AE_HIP(launch_fill_f32(A->grad, (long long)A->nparam, 0.0f, A->stream));
+    AE_HIP(launch_mse(A->out, A->target, nin, A->dev_loss, A->gbuf[kNL - 1], A->stream));
+    for (int i = kNL - 1; i >= 0; --i) {
+        const Layer &l = A->L[i];
+        const int s = side_in(i, n);
+        AE_HIP(launch_wgrad(wgrad_for(A, i, nb, s, A->act[i], A->gbuf[i]), A->stream));
+        const int so = l.s.kind == DOWN ? s / 2 : (l.s.kind == UP ? 2 * s : s);
+        AE_HIP(launch_bgrad(A->gbuf[i], (long long)nb * vol(so), l.s.co, A->grad + l.b_off, A->stream));
+        if (i > 0) {
+            const uint16_t *gate = kNet[i - 1].act ? A->act[i] : nullptr;
+            AE_HIP(launch_gemm(bwd_gemm(A, i, nb, s, A->gbuf[i], A->gbuf[i - 1], gate), A->stream));
+        }
+    }
+    if (apply) {
+        ++A->step;
+        const double b1 = 0.9, b2 = 0.999;
+        const float c1 = (float)(1.0 / (1.0 - std::pow(b1, (double)A->step)));
+        const float c2 = (float)(1.0 / (1.0 - std::pow(b2, (double)A->step)));
+        AE_HIP(launch_adam(A->master, A->m, A->v, A->grad, A->wb, (long long)A->nparam, A->lr, (float)b1, (float)b2,
+                           1e-8f, c1, c2, A->stream));
+        int e = rebuild_wt(A);
+        if (e) return e;
+    }
+    if (loss_out) {
+        AE_HIP(hipMemcpyAsync(loss_out, A->dev_loss, sizeof(float), hipMemcpyDeviceToHost, A->stream));
+        AE_HIP(hipStreamSynchronize(A->stream));
+    }
+    return VTI_OK;
+}
+
+int grow_volume(vti_ae *A, int n)
+{
+    if (vol(n) <= A->vol_cap) return VTI_OK;
+    for (auto *&p : A->vact) { (void)hipFree(p); p = nullptr; }
+    A->vol_cap = 0;
+    for (int i = 1; i < kNL; ++i) {
+        const int s = side_in(i, n);
+        const size_t bytes = (size_t)vol(s) * kNet[i].ci * sizeof(uint16_t);
+        if (hipMalloc(&A->vact[i], bytes) != hipSuccess) return aefail(VTI_ERR_ALLOC, "autoencoder: volume buffers");
+    }
+    A->vol_cap = vol(n);
+    return VTI_OK;
+}
+
+}  // namespace
+
+namespace vti {
+
+int ae_encode(vti_ae *A, const float *P, long long off, long long sy, long long sx, int n, uint16_t *latent,
+              float *scale_dev, hipStream_t s)
+{
+    int st = check_shape(n);
+    if (st || (st = grow_volume(A, n))) return st;
+    AE_HIP(launch_fill_f32(scale_dev, 1, 0.0f, s));
+    AE_HIP(launch_maxabs(P, off, sy, sx, n, n, n, reinterpret_cast<unsigned *>(scale_dev), s));
+    // the scale is read on the host once per snapshot: the encoder's input
+    // scaling is a kernel argument
+    float sc = 0.0f;
+    AE_HIP(hipMemcpyAsync(&sc, scale_dev, sizeof(float), hipMemcpyDeviceToHost, s));
+    AE_HIP(hipStreamSynchronize(s));
+    if (!(sc > 0.0f)) {   // an all-zero snapshot: zero latent
+        AE_HIP(launch_fill_u16(latent, vol(n / 8) * 64, 0, s));
+        return VTI_OK;
+    }
+    for (int i = 0; i < kNEnc; ++i) {
+        const int si = side_in(i, n);
+        uint16_t *dst = i + 1 == kNEnc ? latent : A->vact[i + 1];
+        Gemm g = fwd_gemm(A, i, 1, si, A->vact[i], dst, nullptr);
+        if (i == 0) {
+            g.x = nullptr;
+            g.xf = P;
+            g.xoff = off;
+            g.xsb = 0;
+            g.xsy = sy;
+            g.xsx = sx;
+            g.xscale = 1.0f / sc;
+        }
+        AE_HIP(launch_gemm(g, s));
+    }
+    return VTI_OK;
+}
+
+int ae_decode(vti_ae *A, const uint16_t *latent, const float *scale_dev, int n, float *out, hipStream_t s)
+{
+    int st = check_shape(n);
+    if (st || (st = grow_volume(A, n))) return st;
+    float sc = 0.0f;
+    AE_HIP(hipMemcpyAsync(&sc, scale_dev, sizeof(float), hipMemcpyDeviceToHost, s));
+    AE_HIP(hipStreamSynchronize(s));
+    if (!(sc > 0.0f)) {
+        AE_HIP(launch_fill_f32(out, vol(n), 0.0f, s));
+        return VTI_OK;
+    }
+    for (int i = kNEnc; i < kNL; ++i) {
+        const int si = side_in(i, n);
+        const uint16_t *src = i == kNEnc ? latent : A->vact[i];
+        const bool last = i + 1 == kNL;
+        Gemm g = fwd_gemm(A, i, 1, si, src, last ? nullptr : A->vact[i + 1], last ? out : nullptr);
+        if (last) g.yscale = sc;
+        AE_HIP(launch_gemm(g, s));
+    }
+    return VTI_OK;
+}
+
+int ae_train_field(vti_ae *A, const float *P, long long off, long long sy, long long sx, int ny, int nx, int nz,
+                   int nsteps, float *loss)
+{
+    const int c = A->crop;
+    if (ny < c || nx < c || nz < c) return aefail(VTI_ERR_ARG, "autoencoder: snapshot smaller than a crop");
+    AE_HIP(launch_fill_f32(A->scale, 1, 0.0f, A->stream));
+    AE_HIP(launch_maxabs(P, off, sy, sx, ny, nx, nz, reinterpret_cast<unsigned *>(A->scale), A->stream));
+    float sc = 0.0f;
+    AE_HIP(hipMemcpyAsync(&sc, A->scale, sizeof(float), hipMemcpyDeviceToHost, A->stream));
+    AE_HIP(hipStreamSynchronize(A->stream));
+    if (loss) *loss = 0.0f;
+    if (!(sc > 0.0f)) return VTI_OK;   // nothing to learn from a zero field
+    std::vector<int> org(3 * (size_t)A->batch);
+    float acc = 0.0f;
+    for (int it = 0; it < nsteps; ++it) {
+        for (int b = 0; b < A->batch; ++b) {
+            org[3 * b + 0] = (int)(A->rng() % (unsigned long long)(ny - c + 1));
+            org[3 * b + 1] = (int)(A->rng() % (unsigned long long)(nx - c + 1));
+            org[3 * b + 2] = (int)(A->rng() % (unsigned long long)(nz - c + 1));
+        }
+        AE_HIP(hipMemcpyAsync(A->org, org.data(), org.size() * sizeof(int), hipMemcpyHostToDevice, A->stream));
+        AE_HIP(launch_crops(P, off, sy, sx, A->org, A->batch, c, A->scale, A->target, A->stream));
+        float l = 0.0f;
+        const int st = train_step(A, 1, &l);
+        if (st) return st;
+        acc += l;
+    }
+    if (loss) *loss = acc / (float)std::max(nsteps, 1);
+    return VTI_OK;
+}
+
+hipStream_t ae_stream(vti_ae *A) { return A->stream; }
+
+}  // namespace vti
+
+// ------------------------------------------------------------------ C ABI --
+extern "C" {
+
+int vti_ae_create(int crop, int batch, unsigned long long seed, vti_ae **out)
+{
+    if (!out || batch < 1 || crop < 8 || crop % 8) return aefail(VTI_ERR_ARG, "vti_ae_create: crop must be a multiple of 8, batch >= 1");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return aefail(VTI_ERR_NODEV, "vti_ae_create: no HIP device");
+    std::unique_ptr<vti_ae> A(new (std::nothrow) vti_ae());
+    if (!A) return aefail(VTI_ERR_ALLOC, "host");
+    A->crop = crop;
+    A->batch = batch;
+    A->rng.seed(seed);
+    size_t off = 0, woff = 0;
+    for (int i = 0; i < kNL; ++i) {
+        Layer &l = A->L[i];
+        l.s = kNet[i];
+        const int nt = ntap(l.s.kind);
+        if (l.s.kind == UP) { l.N = 8 * l.s.co; l.K = l.s.ci; }
+        else { l.N = l.s.co; l.K = nt * l.s.ci; }
+        l.Npad = pad16(l.N);
+        l.Kpad = pad32(l.K);
+        if (l.s.kind == DOWN) { l.Nt = 8 * l.s.ci; l.Kt = l.s.co; }
+        else if (l.s.kind == UP) { l.Nt = l.s.ci; l.Kt = 8 * l.s.co; }
+        else { l.Nt = l.s.ci; l.Kt = 27 * l.s.co; }
+        l.Ntpad = pad16(l.Nt);
+        l.Ktpad = pad32(l.Kt);
+        l.w_off = off;
+        off += (size_t)l.Npad * l.Kpad;
+        l.b_off = off;
+        off += (size_t)pad16(l.s.co);
+        l.wt_off = woff;
+        woff += (size_t)l.Ntpad * l.Ktpad;
+    }
+    A->nparam = off;
+    A->nwt = woff;
+    const int c = crop;
+    bool ok = hipStreamCreateWithFlags(&A->stream, hipStreamNonBlocking) == hipSuccess;
+    for (auto **p : {&A->master, &A->m, &A->v, &A->grad})
+        ok = ok && hipMalloc(p, off * sizeof(float)) == hipSuccess;
+    ok = ok && hipMalloc(&A->wb, off * sizeof(uint16_t)) == hipSuccess;
+    ok = ok && hipMalloc(&A->wt, woff * sizeof(uint16_t)) == hipSuccess;
+    for (int i = 0; i < kNL && ok; ++i) {
+        const int s = side_in(i, c);
+        ok = hipMalloc(&A->act[i], (size_t)batch * vol(s) * kNet[i].ci * sizeof(uint16_t)) == hipSuccess;
+        const int so = side_in(i + 1, c);
+        ok = ok && hipMalloc(&A->gbuf[i], (size_t)batch * vol(so) * kNet[i].co * sizeof(uint16_t)) == hipSuccess;
+    }
+    ok = ok && hipMalloc(&A->out, (size_t)batch * vol(c) * sizeof(float)) == hipSuccess;
+    ok = ok && hipMalloc(&A->target, (size_t)batch * vol(c) * sizeof(float)) == hipSuccess;
+    ok = ok && hipMalloc(&A->dev_loss, 64) == hipSuccess && hipMalloc(&A->scale, 64) == hipSuccess;
+    ok = ok && hipMalloc(&A->org, 3 * (size_t)batch * sizeof(int)) == hipSuccess;
+    if (!ok) return aefail(VTI_ERR_ALLOC, "vti_ae_create: device buffers");
+    // He initialisation of the master weights (fp32), zero biases and moments
+    std::vector<float> h(off, 0.0f);
+    std::normal_distribution<float> nd(0.0f, 1.0f);
+    for (int i = 0; i < kNL; ++i) {
+        const Layer &l = A->L[i];
+        const float sd = std::sqrt(2.0f / (float)(l.s.kind == UP ? l.s.ci : l.K));
+        for (int r = 0; r < l.N; ++r)
+            for (int k = 0; k < l.K; ++k) h[l.w_off + (size_t)r * l.Kpad + k] = sd * nd(A->rng);
+    }
+    int st = vti_ae_set_params_padded_(A.get(), h.data());
+    if (st) return st;
+    AE_HIP(launch_fill_f32(A->m, (long long)off, 0.0f, A->stream));
+    AE_HIP(launch_fill_f32(A->v, (long long)off, 0.0f, A->stream));
+    AE_HIP(hipStreamSynchronize(A->stream));
+    *out = A.release();
+    return VTI_OK;
+}
+
+// master weights from a padded host image; refreshes the bf16 copies
+int vti_ae_set_params_padded_(vti_ae *A, const float *h)
+{
+    AE_HIP(hipMemcpyAsync(A->master, h, A->nparam * sizeof(float), hipMemcpyHostToDevice, A->stream));
+    AE_HIP(launch_f32_to_bf16(A->master, A->wb, (long long)A->nparam, A->stream));
+    int st = rebuild_wt(A);
+    if (st) return st;
+    AE_HIP(hipStreamSynchronize(A->stream));
+    return VTI_OK;
+}
+
+int vti_ae_destroy(vti_ae *A)
+{
+    delete A;
+    return VTI_OK;
+}
+
+long long vti_ae_nparams(const vti_ae *A)
+{
+    if (!A) return 0;
+    long long n = 0;
+    for (const Layer &l : A->L) n += (long long)l.N * l.K + l.s.co;
+    return n;
+}
+
+// flat = per layer: W (rows x cols, row-major, unpadded) then b (co)
+static int params_io(vti_ae *A, float *flat, const float *in, bool grads)
+{
+    std::vector<float> h(A->nparam, 0.0f);
+    if (!in) {
+        AE_HIP(hipStreamSynchronize(A->stream));
+        AE_HIP(hipMemcpy(h.data(), grads ? A->grad : A->master, A->nparam * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    size_t f = 0;
+    for (const Layer &l : A->L) {
+        for (int r = 0; r < l.N; ++r)
+            for (int k = 0; k < l.K; ++k, ++f) {
+                float &d = h[l.w_off + (size_t)r * l.Kpad + k];
+                if (in) d = in[f];
+                else flat[f] = d;
+            }
+        for (int c = 0; c < l.s.co; ++c, ++f) {
+            float &d = h[l.b_off + c];
+            if (in) d = in[f];
+            else flat[f] = d;
+        }
+    }
+    if (in) return vti_ae_set_params_padded_(A, h.data());
+    return VTI_OK;
+}
+
+int vti_ae_set_params(vti_ae *A, const float *flat)
+{
+    if (!A || !flat) return aefail(VTI_ERR_ARG, "vti_ae_set_params: null");
+    return params_io(A, nullptr, flat, false);
+}
+
+int vti_ae_get_params(vti_ae *A, float *flat)
+{
+    if (!A || !flat) return aefail(VTI_ERR_ARG, "vti_ae_get_params: null");
+    return params_io(A, flat, nullptr, false);
+}
+
+int vti_ae_get_grads(vti_ae *A, float *flat)
+{
+    if (!A || !flat) return aefail(VTI_ERR_ARG, "vti_ae_get_grads: null");
+    return params_io(A, flat, nullptr, true);
+}
+
+int vti_ae_set_lr(vti_ae *A, float lr)
+{
+    if (!A || !(lr > 0.0f)) return aefail(VTI_ERR_ARG, "vti_ae_set_lr: bad arguments");
+    A->lr = lr;
+    return VTI_OK;
+}
+
+int vti_ae_train_batch(vti_ae *A, const float *crops, int apply, float *loss, float *recon)
+{
+    if (!A || !crops) return aefail(VTI_ERR_ARG, "vti_ae_train_batch: null");
+    const size_t n = (size_t)A->batch * vol(A->crop);
+    AE_HIP(hipMemcpyAsync(A->target, crops, n * sizeof(float), hipMemcpyHostToDevice, A->stream));
+    int st = train_step(A, apply, loss);
+    if (st) return st;
+    if (recon) {   // the reconstruction of this step's forward pass
+        AE_HIP(hipMemcpyAsync(recon, A->out, n * sizeof(float), hipMemcpyDeviceToHost, A->stream));
+        AE_HIP(hipStreamSynchronize(A->stream));
+    }
+    return VTI_OK;
+}
+
+int vti_ae_activation(vti_ae *A, int i, float *dst)
+{
+    if (!A || !dst || i < 0 || i > kNL) return aefail(VTI_ERR_ARG, "vti_ae_activation: bad arguments");
+    const int s = side_in(i, A->crop);
+    const long long n = (long long)A->batch * vol(s) * (i < kNL ? kNet[i].ci : 1);
+    AE_HIP(hipStreamSynchronize(A->stream));
+    if (i == kNL) {
+        AE_HIP(hipMemcpy(dst, A->out, n * sizeof(float), hipMemcpyDeviceToHost));
+        return VTI_OK;
+    }
+    std::vector<uint16_t> h(n);
+    AE_HIP(hipMemcpy(h.data(), A->act[i], n * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    for (long long j = 0; j < n; ++j) {
+        const uint32_t u = (uint32_t)h[j] << 16;
+        std::memcpy(dst + j, &u, 4);
+    }
+    return VTI_OK;
+}
+
+int vti_ae_train_field(vti_ae *A, const float *P, long long off, long long sy, long long sx, int ny, int nx, int nz,
+                       int nsteps, float *loss)
+{
+    if (!A || !P || nsteps < 0) return aefail(VTI_ERR_ARG, "vti_ae_train_field: bad arguments");
+    return ae_train_field(A, P, off, sy, sx, ny, nx, nz, nsteps, loss);
+}
+
+int vti_ae_encode(vti_ae *A, const float *P, long long off, long long sy, long long sx, int n, void *latent,
+                  float *scale_dev)
+{
+    if (!A || !P || !latent || !scale_dev) return aefail(VTI_ERR_ARG, "vti_ae_encode: null");
+    int st = ae_encode(A, P, off, sy, sx, n, static_cast<uint16_t *>(latent), scale_dev, A->stream);
+    if (st) return st;
+    AE_HIP(hipStreamSynchronize(A->stream));
+    return VTI_OK;
+}
+
+int vti_ae_decode(vti_ae *A, const void *latent, const float *scale_dev, int n, float *out)
+{
+    if (!A || !latent || !scale_dev || !out) return aefail(VTI_ERR_ARG, "vti_ae_decode: null");
+    int st = ae_decode(A, static_cast<const uint16_t *>(latent), scale_dev, n, out, A->stream);
+    if (st) return st;
+    AE_HIP(hipStreamSynchronize(A->stream));
+    return VTI_OK;
+}
+
+long long vti_ae_latent_elems(int n) { return n > 0 && n % 8 == 0 ? vol(n / 8) * 64 : 0; }
+
+}  // extern "C"

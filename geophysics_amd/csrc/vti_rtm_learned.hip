@@ -1,0 +1,274 @@
+// vti_rtm_learned.hip — 3D RTM with the source wavefield kept as
+// autoencoder latents instead of being recomputed from checkpoints.
+//
+// First sweep (the forward modelling of the shot, with record and source
+// illumination exactly as vti_rtm_shot): after every step it with
+// it % save_every == 0 the interior P box (n^3, n a multiple of 8) is
+// encoded into a latent slot (n^3/8 bf16 values + one scale).  Optionally
+// every check_every-th stored snapshot is decoded at once and its relative
+// L2 error against the true field measured; a snapshot above tol is marked.
+// A few full-state checkpoints (every ck_every steps) are kept as the
+// fallback: in the reverse sweep a marked snapshot is recomputed exactly
+// from the nearest checkpoint (plain checkpoint recomputation), every other
+// imaging step decodes its latent.  The receiver side is the same as in
+// vti_rtm_shot; imaging steps are it % save_every == 0 (image_every =
+// save_every there), so the two paths differ only by the compression error.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vti_ae.h"
+#include "vti_internal.h"
+#include "vti_nn.h"
+#include "vti_rtm.h"
+
+using namespace vti;
+
+namespace {
+
+// img[o] += S[o] * gP[c] over the interior (S dense, z fastest)
+__global__ void image_dense_kernel(Grid g, int np, int nx, int ny, int nz, const float *__restrict__ S,
+                                   const float *__restrict__ gP, float *img)
+{
+    const int iz = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ix = blockIdx.y, iy = blockIdx.z;
+    if (iz >= nz) return;
+    const long long c = (iz + np) + (long long)(ix + np) * g.ldz + (long long)(iy + np) * g.ldxy;
+    const long long o = iz + (long long)ix * nz + (long long)iy * nz * nx;
+    img[o] += S[o] * gP[c];
+}
+
+// sums of (S - P)^2 and P^2 over the interior (double atomics on two cells)
+__global__ void err_kernel(Grid g, int np, int n, const float *__restrict__ S, const float *__restrict__ P,
+                           double *acc)
+{
+    __shared__ double r0[256], r1[256];
+    double a = 0.0, b = 0.0;
+    const long long tot = (long long)n * n * n;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < tot; i += (long long)gridDim.x * blockDim.x) {
+        const int z = (int)(i % n);
+        const long long r = i / n;
+        const int x = (int)(r % n), y = (int)(r / n);
+        const float p = P[(z + np) + (long long)(x + np) * g.ldz + (long long)(y + np) * g.ldxy];
+        const double d = (double)S[i] - (double)p;
+        a += d * d;
+        b += (double)p * p;
+    }
+    r0[threadIdx.x] = a;
+    r1[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            r0[threadIdx.x] += r0[threadIdx.x + w];
+            r1[threadIdx.x] += r1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(acc, r0[0]);
+        atomicAdd(acc + 1, r1[0]);
+    }
+}
+
+struct Buf {
+    void *p = nullptr;
+    ~Buf() { (void)hipFree(p); }
+};
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+int vti_learned_defaults(vti_learned_opts *o)
+{
+    if (!o) return fail(VTI_ERR_ARG, "vti_learned_defaults: null");
+    std::memset(o, 0, sizeof(*o));
+    o->save_every = 4;
+    o->check_every = 8;
+    o->tol = 0.05f;
+    o->ck_every = 0;
+    return VTI_OK;
+}
+
+int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti_learned_opts *lo, const float *obs,
+                         double *stats)
+{
+    if (!c || !A || !o || !lo) return fail(VTI_ERR_ARG, "vti_rtm_shot_learned: null argument");
+    if (!c->have_model || !c->have_shot) return fail(VTI_ERR_STATE, "vti_rtm_shot_learned: model and shot first");
+    const vti_cfg &k = c->cfg;
+    const int n = k.nx;
+    if (k.ndim != 3 || k.ny != n || k.nz != n || n % 8)
+        return fail(VTI_ERR_ARG, "vti_rtm_shot_learned: 3D cubic interiors with a side divisible by 8 only");
+    if (c->nrec <= 0) return fail(VTI_ERR_STATE, "vti_rtm_shot_learned: the shot has no receivers");
+    if (lo->save_every < 1 || lo->check_every < 0 || lo->ck_every < 0)
+        return fail(VTI_ERR_ARG, "vti_rtm_shot_learned: bad options");
+    int st;
+    if (!c->img && (st = vti_rtm_reset(c))) return st;
+    const int nt = k.nt, nrec = c->nrec, np = k.npml, se = lo->save_every;
+    const int inj_z = o->inject_z >= 0 ? o->inject_z : c->shot.rec_z;
+    if (inj_z < 4 || inj_z >= c->g.nnz - 4) return fail(VTI_ERR_ARG, "vti_rtm_shot_learned: injection row outside the updated region");
+    const long long lat_n = vti_ae_latent_elems(n);
+    const int nsave = (nt + se - 1) / se;
+    const long long off = np + (long long)np * c->g.ldz + (long long)np * c->g.ldxy;
+    hipStream_t s = c->stream;
+    Buf lat, scl, dense, acc;
+    if (hipMalloc(&lat.p, (size_t)nsave * lat_n * sizeof(uint16_t)) != hipSuccess ||
+        hipMalloc(&scl.p, (size_t)nsave * 64) != hipSuccess ||
+        hipMalloc(&dense.p, (size_t)n * n * n * sizeof(float)) != hipSuccess ||
+        hipMalloc(&acc.p, 2 * sizeof(double)) != hipSuccess)
+        return fail(VTI_ERR_ALLOC, "vti_rtm_shot_learned: latent store");
+    uint16_t *L = static_cast<uint16_t *>(lat.p);
+    float *Sc = static_cast<float *>(scl.p);
+    float *D = static_cast<float *>(dense.p);
+    double *E = static_cast<double *>(acc.p);
+    // fallback checkpoints
+    const int nck = lo->ck_every > 0 ? (nt + lo->ck_every - 1) / lo->ck_every : 0;
+    std::vector<VtiState> ck(nck);
+    struct CkGuard {
+        std::vector<VtiState> &v;
+        ~CkGuard() { for (auto &x : v) state_free(x); }
+    } ckg{ck};
+    for (auto &x : ck)
+        if ((st = state_alloc(c, x))) return st;
+    VtiState G;
+    if ((st = state_alloc(c, G))) return st;
+    struct GGuard {
+        VtiState &g;
+        ~GGuard() { state_free(g); }
+    } gg{G};
+    const VtiState W = main_state(c);
+    const bool svf = o->sv_forward != 0 && k.sv != 0;
+    const bool svb = o->sv_backward != 0 && k.sv != 0;
+    const int clamp = o->illum_mode == 0;
+    auto hip = [](hipError_t e, const char *w) { return e == hipSuccess ? VTI_OK : fail(VTI_ERR_HIP, w); };
+    std::vector<char> bad(nsave, 0);
+    double t_enc = 0.0, t_dec = 0.0, err_sum = 0.0;
+    int nchecked = 0, nbad = 0;
+
+    // ---- first sweep
+    if ((st = hip(state_zero(c, W), "zero"))) return st;
+    for (int it = 0; it < nt; ++it) {
+        if (nck && it % lo->ck_every == 0 && (st = hip(state_copy(c, ck[it / lo->ck_every], W), "checkpoint")))
+            return st;
+        if (launch_vel(3, vel_args_for(c, W, it, true), s) != hipSuccess) return fail(VTI_ERR_HIP, "vel");
+        float *row = c->rec + (size_t)it * nrec;
+        if (launch_stress(3, stress_args_for(c, W, it, true, svf, row), s) != hipSuccess) return fail(VTI_ERR_HIP, "stress");
+        if ((st = hip(launch_illum(c, W.f[VTI_F_P], W.f[VTI_F_Q], c->illum, clamp), "illum"))) return st;
+        if (it % se) continue;
+        const int j = it / se;
+        const float *P = W.f[VTI_F_P];
+        const double t0 = now_ms();
+        if ((st = ae_encode(A, P, off, c->g.ldxy, c->g.ldz, n, L + (size_t)j * lat_n, Sc + (size_t)j * 16, s)))
+            return st;
+        t_enc += now_ms() - t0;
+        if (lo->check_every > 0 && j % lo->check_every == 0) {
+            if ((st = ae_decode(A, L + (size_t)j * lat_n, Sc + (size_t)j * 16, n, D, s))) return st;
+            if ((st = hip(nn::launch_fill_f32(reinterpret_cast<float *>(E), 4, 0.0f, s), "fill"))) return st;
+            hipLaunchKernelGGL(err_kernel, dim3(1024), dim3(256), 0, s, c->g, np, n, D, P, E);
+            double h[2];
+            if (hipMemcpyAsync(h, E, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+                return fail(VTI_ERR_HIP, "error check");
+            if (h[1] > 0.0) {
+                const double e = std::sqrt(h[0] / h[1]);
+                err_sum += e;
+                ++nchecked;
+                if (e > lo->tol && nck) {
+                    // this snapshot and its neighbours up to the next check are recomputed
+                    for (int q = j; q < std::min(nsave, j + lo->check_every); ++q) bad[q] = 1;
+                    ++nbad;
+                }
+            }
+        }
+    }
+    // ---- the record: observed or modelled + mute (as vti_rtm_shot)
+    if (obs) {
+        if (hipMemcpyAsync(c->rec_t, obs, (size_t)nrec * nt * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess)
+            return fail(VTI_ERR_HIP, "obs upload");
+        if ((st = hip(launch_transpose(c->rec_t, c->rec, nrec, nt, s), "transpose"))) return st;
+    } else if (o->mute_tt >= 0) {
+        const vti_shot &sh = c->shot;
+        if ((st = vti_mute3d_window(c, sh.sx - np + 1, 0, sh.sy - np + 1, 0, sh.sz - np + 1, 0, o->mute_tt))) return st;
+    }
+    // ---- reverse sweep
+    if ((st = hip(state_zero(c, G), "zero"))) return st;
+    for (int it = nt - 1; it >= 0; --it) {
+        if (launch_inject_rec(c, G, c->rec + (size_t)it * nrec, inj_z) != hipSuccess) return fail(VTI_ERR_HIP, "inject");
+        if (launch_vel(3, vel_args_for(c, G, it, false), s) != hipSuccess) return fail(VTI_ERR_HIP, "vel");
+        if (launch_stress(3, stress_args_for(c, G, it, false, svb, nullptr), s) != hipSuccess) return fail(VTI_ERR_HIP, "stress");
+        if (o->illum_mode == 1 && launch_illum(c, G.f[VTI_F_P], G.f[VTI_F_Q], c->illum, 0) != hipSuccess)
+            return fail(VTI_ERR_HIP, "illum");
+        if (it % se) continue;
+        const int j = it / se;
+        const dim3 blk(64), grd((k.nz + 63) / 64, k.nx, k.ny);
+        if (bad[j]) {   // exact source field from the nearest checkpoint at or before it+1
+            const int target = it + 1;
+            const int ci = std::min((target - 1) / lo->ck_every, nck - 1);
+            if ((st = hip(state_copy(c, W, ck[ci]), "restore"))) return st;
+            for (int work = ci * lo->ck_every; work < target; ++work) {
+                if (launch_vel(3, vel_args_for(c, W, work, true), s) != hipSuccess) return fail(VTI_ERR_HIP, "vel");
+                if (launch_stress(3, stress_args_for(c, W, work, true, svf, nullptr), s) != hipSuccess)
+                    return fail(VTI_ERR_HIP, "stress");
+            }
+            if ((st = hip(launch_image(c, W.f[VTI_F_P], G.f[VTI_F_P], c->img), "image"))) return st;
+        } else {
+            const double t0 = now_ms();
+            if ((st = ae_decode(A, L + (size_t)j * lat_n, Sc + (size_t)j * 16, n, D, s))) return st;
+            t_dec += now_ms() - t0;
+            hipLaunchKernelGGL(image_dense_kernel, grd, blk, 0, s, c->g, np, k.nx, k.ny, k.nz, D, G.f[VTI_F_P], c->img);
+        }
+        if ((st = hip(hipGetLastError(), "image"))) return st;
+    }
+    c->rtm_illum_mode = o->illum_mode;
+    if ((st = hip(hipStreamSynchronize(s), "sync"))) return st;
+    if (stats) {
+        stats[0] = t_enc;
+        stats[1] = t_dec;
+        stats[2] = nchecked ? err_sum / nchecked : -1.0;
+        stats[3] = nbad;
+        stats[4] = nsave;
+    }
+    return VTI_OK;
+}
+
+// Forward modelling of the current shot (no record kept) with nsteps Adam
+// steps of the autoencoder on every save_every-th snapshot.
+int vti_ae_train_shot(vti_ctx *c, vti_ae *A, int save_every, int nsteps, float *mean_loss)
+{
+    if (!c || !A || save_every < 1 || nsteps < 0) return fail(VTI_ERR_ARG, "vti_ae_train_shot: bad arguments");
+    if (!c->have_model || !c->have_shot) return fail(VTI_ERR_STATE, "vti_ae_train_shot: model and shot first");
+    const vti_cfg &k = c->cfg;
+    if (k.ndim != 3) return fail(VTI_ERR_ARG, "vti_ae_train_shot: 3D only");
+    const int np = k.npml;
+    const long long off = np + (long long)np * c->g.ldz + (long long)np * c->g.ldxy;
+    const VtiState W = main_state(c);
+    if (state_zero(c, W) != hipSuccess) return fail(VTI_ERR_HIP, "zero");
+    double acc = 0.0;
+    int cnt = 0;
+    for (int it = 0; it < k.nt; ++it) {
+        if (launch_vel(3, vel_args_for(c, W, it, true), c->stream) != hipSuccess ||
+            launch_stress(3, stress_args_for(c, W, it, true, k.sv != 0, nullptr), c->stream) != hipSuccess)
+            return fail(VTI_ERR_HIP, "step");
+        if (it % save_every) continue;
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(VTI_ERR_HIP, "sync");   // training runs on its own stream
+        float l = 0.0f;
+        const int st = ae_train_field(A, W.f[VTI_F_P], off, c->g.ldxy, c->g.ldz, k.ny, k.nx, k.nz, nsteps, &l);
+        if (st) return st;
+        acc += l;
+        ++cnt;
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(VTI_ERR_HIP, "sync");
+    if (mean_loss) *mean_loss = cnt ? (float)(acc / cnt) : 0.0f;
+    return VTI_OK;
+}
+
+}  // extern "C"

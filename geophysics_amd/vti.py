@@ -58,6 +58,10 @@ class RtmOpts(C.Structure):
 
 
 _lib = None
+class LearnedOpts(C.Structure):
+    _fields_ = [("save_every", C.c_int), ("check_every", C.c_int), ("tol", C.c_float), ("ck_every", C.c_int)]
+
+
 FP = C.POINTER(C.c_float)
 DP = C.POINTER(C.c_double)
 VP = C.c_void_p
@@ -114,6 +118,23 @@ def _declare(L):
         "vti_batch_rtm": (C.c_int, [VP, C.POINTER(RtmOpts), C.POINTER(FP), C.POINTER(C.c_longlong)]),
         "vti_batch_image": (C.c_int, [VP, C.c_int, C.c_int, FP, FP]),
         "vti_batch_record": (C.c_int, [VP, C.c_int, FP]),
+        "vti_ae_create": (C.c_int, [C.c_int, C.c_int, C.c_ulonglong, C.POINTER(VP)]),
+        "vti_ae_destroy": (C.c_int, [VP]),
+        "vti_ae_nparams": (C.c_longlong, [VP]),
+        "vti_ae_set_params": (C.c_int, [VP, FP]),
+        "vti_ae_get_params": (C.c_int, [VP, FP]),
+        "vti_ae_get_grads": (C.c_int, [VP, FP]),
+        "vti_ae_set_lr": (C.c_int, [VP, C.c_float]),
+        "vti_ae_train_batch": (C.c_int, [VP, FP, C.c_int, FP, FP]),
+        "vti_ae_activation": (C.c_int, [VP, C.c_int, FP]),
+        "vti_ae_train_field": (C.c_int, [VP, VP, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_int,
+                                         C.c_int, C.c_int, FP]),
+        "vti_ae_encode": (C.c_int, [VP, VP, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, VP, VP]),
+        "vti_ae_decode": (C.c_int, [VP, VP, VP, C.c_int, VP]),
+        "vti_ae_latent_elems": (C.c_longlong, [C.c_int]),
+        "vti_learned_defaults": (C.c_int, [C.POINTER(LearnedOpts)]),
+        "vti_rtm_shot_learned": (C.c_int, [VP, VP, C.POINTER(RtmOpts), C.POINTER(LearnedOpts), FP, DP]),
+        "vti_ae_train_shot": (C.c_int, [VP, VP, C.c_int, C.c_int, FP]),
         "vti_grad_create": (C.c_int, [C.POINTER(Cfg), C.c_int, C.POINTER(VP)]),
         "vti_grad_destroy": (C.c_int, [VP]),
         "vti_grad_set_model": (C.c_int, [VP, DP, DP, DP, C.c_int]),
@@ -177,6 +198,12 @@ def sweep_check(nt, nslot):
     steps, slots = C.c_longlong(), C.c_int()
     check(lib().vti_sweep_check(nt, nslot, C.byref(steps), C.byref(slots)), "sweep_check")
     return steps.value, slots.value
+
+
+def learned_defaults():
+    o = LearnedOpts()
+    check(lib().vti_learned_defaults(C.byref(o)), "learned_defaults")
+    return o
 
 
 def rtm_defaults(ndim=2):
@@ -346,6 +373,26 @@ class Propagator:
         check(lib().vti_rtm_image(self.h, 1 if finalize else 0, img.ctypes.data_as(FP),
                                   ill.ctypes.data_as(FP)), "rtm_image")
         return img, ill
+
+    def rtm_shot_learned(self, ae, opts=None, lopts=None, obs=None):
+        """RTM of the current shot with the source field stored as
+        autoencoder latents; returns the stats dict."""
+        o = opts if opts is not None else self.rtm_defaults()
+        lo = lopts if lopts is not None else learned_defaults()
+        st = np.zeros(5, np.float64)
+        p = None
+        if obs is not None:
+            a, p = _f32(obs)
+            self._obs_keep = a
+        check(lib().vti_rtm_shot_learned(self.h, ae.h, C.byref(o), C.byref(lo), p, st.ctypes.data_as(DP)),
+              "rtm_shot_learned")
+        return {"encode_ms": st[0], "decode_ms": st[1], "mean_checked_err": st[2], "recomputed": int(st[3]),
+                "snapshots": int(st[4])}
+
+    def ae_train_shot(self, ae, save_every=4, nsteps=2):
+        loss = np.zeros(1, np.float32)
+        check(lib().vti_ae_train_shot(self.h, ae.h, save_every, nsteps, loss.ctypes.data_as(FP)), "ae_train_shot")
+        return float(loss[0])
 
     def prof_enable(self, on=True):
         check(lib().vti_prof_enable(self.h, 1 if on else 0), "prof_enable")
@@ -557,6 +604,84 @@ class Batch:
     def close(self):
         if getattr(self, "h", None):
             lib().vti_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Autoencoder:
+    """Snapshot autoencoder (vti_ae_*): BF16-MFMA training and inference."""
+
+    def __init__(self, crop=32, batch=4, seed=0):
+        h = VP()
+        check(lib().vti_ae_create(crop, batch, seed, C.byref(h)), "vti_ae_create")
+        self.h, self.crop, self.batch = h, crop, batch
+
+    @property
+    def nparams(self):
+        return int(lib().vti_ae_nparams(self.h))
+
+    def set_params(self, flat):
+        a, p = _f32(flat)
+        assert a.size == self.nparams
+        check(lib().vti_ae_set_params(self.h, p), "ae_set_params")
+
+    def params(self):
+        out = np.zeros(self.nparams, np.float32)
+        check(lib().vti_ae_get_params(self.h, out.ctypes.data_as(FP)), "ae_get_params")
+        return out
+
+    def grads(self):
+        out = np.zeros(self.nparams, np.float32)
+        check(lib().vti_ae_get_grads(self.h, out.ctypes.data_as(FP)), "ae_get_grads")
+        return out
+
+    def set_lr(self, lr):
+        check(lib().vti_ae_set_lr(self.h, lr), "ae_set_lr")
+
+    def train_batch(self, crops, apply=True, want_recon=False):
+        a, p = _f32(crops)
+        assert a.size == self.batch * self.crop ** 3
+        loss = np.zeros(1, np.float32)
+        rec = np.zeros(a.shape, np.float32) if want_recon else None
+        check(lib().vti_ae_train_batch(self.h, p, 1 if apply else 0, loss.ctypes.data_as(FP),
+                                       rec.ctypes.data_as(FP) if want_recon else None), "ae_train_batch")
+        return (float(loss[0]), rec) if want_recon else float(loss[0])
+
+    def activation(self, i, shape):
+        out = np.zeros(shape, np.float32)
+        check(lib().vti_ae_activation(self.h, i, out.ctypes.data_as(FP)), "ae_activation")
+        return out
+
+    def train_field(self, ptr, off, sy, sx, ny, nx, nz, nsteps):
+        loss = np.zeros(1, np.float32)
+        check(lib().vti_ae_train_field(self.h, ptr, off, sy, sx, ny, nx, nz, nsteps, loss.ctypes.data_as(FP)),
+              "ae_train_field")
+        return float(loss[0])
+
+    def encode(self, ptr, off, sy, sx, n, latent_ptr, scale_ptr):
+        check(lib().vti_ae_encode(self.h, ptr, off, sy, sx, n, latent_ptr, scale_ptr), "ae_encode")
+
+    def decode(self, latent_ptr, scale_ptr, n, out_ptr):
+        check(lib().vti_ae_decode(self.h, latent_ptr, scale_ptr, n, out_ptr), "ae_decode")
+
+    @staticmethod
+    def latent_elems(n):
+        return int(lib().vti_ae_latent_elems(n))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vti_ae_destroy(self.h)
             self.h = None
 
     def __enter__(self):

@@ -251,6 +251,72 @@ int vti_batch_rtm(vti_batch *b, const vti_rtm_opts *opts, const float *const *ob
 int vti_batch_image(vti_batch *b, int shot, int finalize, float *img, float *illum);
 int vti_batch_record(vti_batch *b, int shot, float *dst);
 
+/* ---- learned snapshot compression ---------------------------------------- */
+/* A 4-level 3D convolutional autoencoder (BF16 weights and activations,
+ * FP32 accumulation on MFMA, FP32 master weights and Adam moments) that
+ * stores source-wavefield snapshots in 16x fewer bytes than FP32.  It has no
+ * counterpart in the reference, whose RTM keeps no snapshots and re-runs the
+ * forward operator from the final state
+ * (Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:1423-1437).  Layers: down-sampling
+ * 2x2x2 convolutions 1->16->32->64, a 3x3x3 convolution 64->64 (latent),
+ * a 3x3x3 convolution 64->64 and transposed 2x2x2 convolutions
+ * 64->32->16->1; leaky ReLU (slope 1/8) between layers. */
+typedef struct vti_ae vti_ae;
+/* crop: side of the training crops (multiple of 8); batch: crops per step. */
+int vti_ae_create(int crop, int batch, unsigned long long seed, vti_ae **out);
+int vti_ae_destroy(vti_ae *ae);
+/* Parameter count of the flat form: per layer its weight matrix (rows x
+ * cols, row-major; down/same: rows = out channels, cols = tap*in + c;
+ * up: rows = tap*out + c, cols = in channels), then its biases. */
+long long vti_ae_nparams(const vti_ae *ae);
+int vti_ae_set_params(vti_ae *ae, const float *flat);
+int vti_ae_get_params(vti_ae *ae, float *flat);
+/* Gradients of the last training step, flat form. */
+int vti_ae_get_grads(vti_ae *ae, float *flat);
+int vti_ae_set_lr(vti_ae *ae, float lr);
+/* One step on host crops (batch x crop^3 fp32): MSE loss, gradients and,
+ * when apply != 0, an Adam update.  recon (optional, host) receives the
+ * step's reconstruction. */
+int vti_ae_train_batch(vti_ae *ae, const float *crops, int apply, float *loss, float *recon);
+/* Stored output of layer i-1 (the input of layer i) of the last step, as
+ * fp32 host values; i = 8: the reconstruction. */
+int vti_ae_activation(vti_ae *ae, int i, float *dst);
+/* nsteps Adam steps on random crops of the (ny, nx, nz) box of a device
+ * field P[off + y*sy + x*sx + z], each crop scaled by the box's max |P|. */
+int vti_ae_train_field(vti_ae *ae, const float *P, long long off, long long sy, long long sx,
+                       int ny, int nx, int nz, int nsteps, float *loss);
+/* Device-side encode of an n^3 box (n a multiple of 8) into latent
+ * (vti_ae_latent_elems(n) bf16 values) and its scale (one float). */
+int vti_ae_encode(vti_ae *ae, const float *P, long long off, long long sy, long long sx, int n,
+                  void *latent, float *scale_dev);
+int vti_ae_decode(vti_ae *ae, const void *latent, const float *scale_dev, int n, float *out);
+long long vti_ae_latent_elems(int n);
+
+/* 3D RTM with the source wavefield stored as autoencoder latents (cubic
+ * interiors, side a multiple of 8).  The first sweep is vti_rtm_shot's
+ * (record, illumination); after every step it with it % save_every == 0 the
+ * interior P is encoded.  The reverse sweep images those steps with the
+ * decoded field (vti_rtm_shot with image_every = save_every images the same
+ * steps with the exact field).  Fallback: every check_every-th snapshot is
+ * decoded at once and compared with the true field; when its relative L2
+ * error exceeds tol and ck_every > 0, that snapshot and the next
+ * check_every - 1 are recomputed exactly from full-state checkpoints kept
+ * every ck_every steps (plain checkpoint recomputation). */
+typedef struct {
+    int save_every;    /* 4 */
+    int check_every;   /* 8; 0: no checks */
+    float tol;         /* 0.05 */
+    int ck_every;      /* 0: no fallback checkpoints */
+} vti_learned_opts;
+int vti_learned_defaults(vti_learned_opts *o);
+/* stats (optional, 5 doubles): encode ms, decode ms (host-timed), mean
+ * checked relative error, snapshots marked for recomputation, snapshots. */
+int vti_rtm_shot_learned(vti_ctx *ctx, vti_ae *ae, const vti_rtm_opts *opts,
+                         const vti_learned_opts *lo, const float *obs, double *stats);
+/* Forward modelling of the current shot with nsteps Adam steps of the
+ * autoencoder on random crops of every save_every-th snapshot. */
+int vti_ae_train_shot(vti_ctx *ctx, vti_ae *ae, int save_every, int nsteps, float *mean_loss);
+
 /* ---- adjoint and FWI gradient -------------------------------------------- */
 /* No counterpart in the reference for VTI; the reference's FWI gradient is
  * the ISO one of 3D-iso-fd-rtm-fwi-cuda/fwi3d/Toa_gpu_3dFWI.cu:295-311,

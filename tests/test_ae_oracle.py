@@ -1,0 +1,127 @@
+"""The autoencoder oracle against torch.nn.functional (CPU, float64) for the
+three layer kinds, forward and backward, and against finite differences of
+the full network loss.  No GPU needed."""
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from oracle import ae_oracle as ae
+
+
+def _to_torch_w(kind, w, ci, co):
+    if kind == "down":
+        return torch.tensor(w.reshape(co, 2, 2, 2, ci).transpose(0, 4, 1, 2, 3).copy())
+    if kind == "same":
+        return torch.tensor(w.reshape(co, 3, 3, 3, ci).transpose(0, 4, 1, 2, 3).copy())
+    # up: W[(tap, co)][ci] -> torch (ci, co, 2, 2, 2)
+    return torch.tensor(w.reshape(2, 2, 2, co, ci).transpose(4, 3, 0, 1, 2).copy())
+
+
+def _torch_layer(kind, x, w, b):
+    if kind == "down":
+        return F.conv3d(x, w, b, stride=2)
+    if kind == "same":
+        return F.conv3d(x, w, b, padding=1)
+    return F.conv_transpose3d(x, w, b, stride=2)
+
+
+@pytest.mark.parametrize("kind,ci,co", [("down", 1, 16), ("down", 16, 32), ("same", 8, 8),
+                                        ("up", 16, 8), ("up", 8, 1)])
+def test_layer_kinds_equal_torch(kind, ci, co):
+    rng = np.random.default_rng(hash((kind, ci, co)) % 2**32)
+    shape = (2, 4, 6, 8, ci)
+    x = rng.standard_normal(shape)
+    rows, cols = ae.kdim(kind, ci, co)
+    w = rng.standard_normal((rows, cols))
+    b = rng.standard_normal(co)
+    y = ae.conv_fwd(kind, x, w, b, 0)
+    xt = torch.tensor(x.transpose(0, 4, 1, 2, 3).copy(), requires_grad=True)
+    wt = _to_torch_w(kind, w, ci, co).requires_grad_(True)
+    bt = torch.tensor(b, requires_grad=True)
+    yt = _torch_layer(kind, xt, wt, bt)
+    assert np.allclose(y, yt.detach().numpy().transpose(0, 2, 3, 4, 1), rtol=1e-12, atol=1e-12)
+    assert y.shape == ae.out_shape(kind, shape, co)
+    # backward with a random upstream gradient
+    g = rng.standard_normal(y.shape)
+    yt.backward(torch.tensor(g.transpose(0, 4, 1, 2, 3).copy()))
+    dx = ae.conv_bwd_data(kind, g, w, shape)
+    dw, db = ae.conv_bwd_weight(kind, x, g)
+    assert np.allclose(dx, xt.grad.numpy().transpose(0, 2, 3, 4, 1), rtol=1e-12, atol=1e-12)
+    wg = wt.grad.numpy()
+    if kind == "up":
+        wg = wg.transpose(2, 3, 4, 1, 0).reshape(rows, cols)
+    else:
+        k = 3 if kind == "same" else 2
+        wg = wg.transpose(0, 2, 3, 4, 1).reshape(co, k ** 3 * ci)
+    assert np.allclose(dw, wg, rtol=1e-12, atol=1e-12)
+    assert np.allclose(db, bt.grad.numpy(), rtol=1e-12, atol=1e-12)
+
+
+def test_bf16_rounding_is_nearest_even():
+    vals = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -9, -2.5, 3.0e38, 1e-40], np.float32)
+    got = ae.bf16(vals)
+    # 1+2^-8 is a tie between 1 and 1+2^-7: even mantissa (1.0) wins;
+    # 1+3*2^-8 ties between 1+2^-7 (odd) and 1+2^-6 (even): rounds up
+    assert got[0] == 1.0 and got[1] == 1.0 and got[2] == np.float32(1.0 + 2 ** -6)
+    assert got[3] == 1.0 and got[4] == -2.5
+    ref = torch.tensor(vals).to(torch.bfloat16).to(torch.float32).numpy()
+    assert np.array_equal(got, ref)
+
+
+def test_network_shapes_and_compression():
+    p = ae.init_params(0)
+    x = np.random.default_rng(1).standard_normal((1, 16, 16, 16, 1)).astype(np.float32)
+    outs = ae.forward(p, x)
+    lat = outs[ae.N_ENC]
+    assert lat.shape == (1, 2, 2, 2, 64)
+    assert outs[-1].shape == x.shape
+    # latent in bf16 (2 B) against the fp32 input (4 B): 16x fewer bytes
+    assert x.size * 4 == 16 * lat.size * 2
+
+
+def test_network_gradient_matches_finite_differences():
+    """Loss gradient of the oracle network in float64 (no bf16 rounding)
+    against central differences along random directions."""
+    rng = np.random.default_rng(3)
+    p = [(w.astype(np.float64), b.astype(np.float64) + 0.01 * rng.standard_normal(b.shape))
+         for w, b in ae.init_params(5)]
+    x = rng.standard_normal((1, 8, 8, 8, 1))
+
+    def loss64(params):
+        h = x
+        for li, (kind, ci, co, a) in enumerate(ae.NET):
+            h = ae.conv_fwd(kind, h, params[li][0], params[li][1], a)
+        return float(((h - x) ** 2).mean())
+
+    # analytic gradient: the oracle's backward without rounding
+    outs = [x]
+    h = x
+    for li, (kind, ci, co, a) in enumerate(ae.NET):
+        h = ae.conv_fwd(kind, h, p[li][0], p[li][1], a)
+        outs.append(h)
+    g = 2.0 * (outs[-1] - x) / x.size
+    grads = [None] * len(ae.NET)
+    for li in range(len(ae.NET) - 1, -1, -1):
+        kind = ae.NET[li][0]
+        grads[li] = ae.conv_bwd_weight(kind, outs[li], g)
+        if li > 0:
+            g = ae.act_grad(outs[li], ae.conv_bwd_data(kind, g, p[li][0], outs[li].shape), ae.NET[li - 1][3])
+    for trial in range(4):
+        dirs = [(rng.standard_normal(w.shape), rng.standard_normal(b.shape)) for w, b in p]
+        ana = sum(float((gw * dw).sum() + (gb * db).sum()) for (gw, gb), (dw, db) in zip(grads, dirs))
+        h = 1e-6
+        pp = [(w + h * dw, b + h * db) for (w, b), (dw, db) in zip(p, dirs)]
+        pm = [(w - h * dw, b - h * db) for (w, b), (dw, db) in zip(p, dirs)]
+        num = (loss64(pp) - loss64(pm)) / (2 * h)
+        assert abs(num - ana) <= 1e-6 * max(1.0, abs(ana)), (trial, num, ana)
+
+
+def test_adam_step_closed_form():
+    """First Adam step moves every weight by lr * sign(g) (bias-corrected
+    moments are g and g^2)."""
+    w = np.array([1.0, -2.0, 0.5], np.float32)
+    g = np.array([0.3, -1e-3, 2.0], np.float32)
+    w1, m1, v1 = ae.adam(w, np.zeros(3), np.zeros(3), g, 1, 1e-2, eps=0.0)
+    assert np.allclose(w1, w - 1e-2 * np.sign(g), rtol=0, atol=1e-7)
+    assert np.allclose(m1, 0.1 * g) and np.allclose(v1, 0.001 * g * g)
