@@ -3,9 +3,12 @@
 //
 // Network (channels-last volumes, bf16 weights and activations, fp32
 // accumulation, fp32 master weights and Adam moments):
-//   E1 down 1->16, E2 down 16->32, E3 down 32->64 (leaky ReLU 1/8),
+//   E1 down 1->8, E2 down 8->64, E3 down 64->64 (leaky ReLU 1/8),
 //   E4 same 64->64 (linear)              -> latent: 64 ch at N/8, bf16
-//   D4 same 64->64, D3 up 64->32, D2 up 32->16 (leaky ReLU), D1 up 16->1.
+//   D4 same 64->64, D3 up 64->64, D2 up 64->8 (leaky ReLU), D1 up 8->1.
+// No biases: the network is positively homogeneous (f(a x) = a f(x) for
+// a > 0), so a silent region decodes to exactly zero and the amplitude
+// decay of a wavefield costs the network nothing.
 // The latent holds N^3/8 bf16 values for N^3 fp32 samples: 16x fewer
 // bytes.  Each snapshot is scaled by its largest |P| before encoding; the
 // scale is kept beside the latent.
@@ -35,9 +38,13 @@ namespace {
 struct LayerSpec {
     int kind, ci, co, act;
 };
+// E1-E3 keep every degree of freedom until E3 (8 values per 2^3 cell, then
+// 64 per 4^3 cell; leaky ReLU is invertible), so the encoder can represent
+// any linear transform of an 8^3 block, principal components included; E3
+// is the one reduction (512 -> 64 per 8^3 block).
 constexpr LayerSpec kNet[] = {
-    {DOWN, 1, 16, 1}, {DOWN, 16, 32, 1}, {DOWN, 32, 64, 1}, {SAME, 64, 64, 0},
-    {SAME, 64, 64, 1}, {UP, 64, 32, 1}, {UP, 32, 16, 1}, {UP, 16, 1, 0},
+    {DOWN, 1, 8, 1}, {DOWN, 8, 64, 1}, {DOWN, 64, 64, 1}, {SAME, 64, 64, 0},
+    {SAME, 64, 64, 1}, {UP, 64, 64, 1}, {UP, 64, 8, 1}, {UP, 8, 1, 0},
 };
 constexpr int kNL = 8, kNEnc = 4;
 
@@ -45,7 +52,7 @@ struct Layer {
     LayerSpec s;
     int N, K, Npad, Kpad;        // forward matrix (rows, cols), padded
     int Nt, Kt, Ntpad, Ktpad;    // backward-data matrix
-    size_t w_off, b_off;         // offsets into the parameter arrays (floats)
+    size_t w_off;                // offset into the parameter arrays (floats)
     size_t wt_off;               // offset into the backward-data matrices
 };
 
@@ -61,7 +68,7 @@ struct vti_ae {
     int crop = 0, batch = 0;
     // training buffers: act[i] = input of layer i (bf16); out fp32; g[i] = dL/dpre of layer i
     uint16_t *act[kNL] = {}, *gbuf[kNL] = {};
-    float *out = nullptr, *target = nullptr, *dev_loss = nullptr, *scale = nullptr;
+    float *out = nullptr, *target = nullptr, *dev_loss = nullptr, *scale = nullptr, *crop_scale = nullptr;
     int *org = nullptr;
     long long step = 0;
     hipStream_t stream = nullptr;
@@ -74,7 +81,7 @@ struct vti_ae {
     ~vti_ae()
     {
         if (stream) (void)hipStreamSynchronize(stream);
-        for (auto *p : {master, m, v, grad, out, target, dev_loss, scale}) (void)hipFree(p);
+        for (auto *p : {master, m, v, grad, out, target, dev_loss, scale, crop_scale}) (void)hipFree(p);
         (void)hipFree(wb);
         (void)hipFree(wt);
         (void)hipFree(org);
@@ -122,7 +129,7 @@ Gemm fwd_gemm(const vti_ae *A, int i, int nb, int s, const uint16_t *x, uint16_t
     g.x = x;
     g.xscale = 1.0f;
     g.w = A->wb + l.w_off;
-    g.bias = A->master + l.b_off;
+    g.bias = nullptr;
     g.co = l.s.co;
     g.act = l.s.act;
     g.y = y;
@@ -232,8 +239,6 @@ int train_step(vti_ae *A, int apply, float *loss_out)
         const Layer &l = A->L[i];
         const int s = side_in(i, n);
         AE_HIP(launch_wgrad(wgrad_for(A, i, nb, s, A->act[i], A->gbuf[i]), A->stream));
-        const int so = l.s.kind == DOWN ? s / 2 : (l.s.kind == UP ? 2 * s : s);
-        AE_HIP(launch_bgrad(A->gbuf[i], (long long)nb * vol(so), l.s.co, A->grad + l.b_off, A->stream));
         if (i > 0) {
             const uint16_t *gate = kNet[i - 1].act ? A->act[i] : nullptr;
             AE_HIP(launch_gemm(bwd_gemm(A, i, nb, s, A->gbuf[i], A->gbuf[i - 1], gate), A->stream));
@@ -335,13 +340,7 @@ int ae_train_field(vti_ae *A, const float *P, long long off, long long sy, long 
 {
     const int c = A->crop;
     if (ny < c || nx < c || nz < c) return aefail(VTI_ERR_ARG, "autoencoder: snapshot smaller than a crop");
-    AE_HIP(launch_fill_f32(A->scale, 1, 0.0f, A->stream));
-    AE_HIP(launch_maxabs(P, off, sy, sx, ny, nx, nz, reinterpret_cast<unsigned *>(A->scale), A->stream));
-    float sc = 0.0f;
-    AE_HIP(hipMemcpyAsync(&sc, A->scale, sizeof(float), hipMemcpyDeviceToHost, A->stream));
-    AE_HIP(hipStreamSynchronize(A->stream));
     if (loss) *loss = 0.0f;
-    if (!(sc > 0.0f)) return VTI_OK;   // nothing to learn from a zero field
     std::vector<int> org(3 * (size_t)A->batch);
     float acc = 0.0f;
     for (int it = 0; it < nsteps; ++it) {
@@ -351,7 +350,7 @@ int ae_train_field(vti_ae *A, const float *P, long long off, long long sy, long 
             org[3 * b + 2] = (int)(A->rng() % (unsigned long long)(nz - c + 1));
         }
         AE_HIP(hipMemcpyAsync(A->org, org.data(), org.size() * sizeof(int), hipMemcpyHostToDevice, A->stream));
-        AE_HIP(launch_crops(P, off, sy, sx, A->org, A->batch, c, A->scale, A->target, A->stream));
+        AE_HIP(launch_crops(P, off, sy, sx, A->org, A->batch, c, A->crop_scale, A->target, A->stream));
         float l = 0.0f;
         const int st = train_step(A, 1, &l);
         if (st) return st;
@@ -395,8 +394,6 @@ int vti_ae_create(int crop, int batch, unsigned long long seed, vti_ae **out)
         l.Ktpad = pad32(l.Kt);
         l.w_off = off;
         off += (size_t)l.Npad * l.Kpad;
-        l.b_off = off;
-        off += (size_t)pad16(l.s.co);
         l.wt_off = woff;
         woff += (size_t)l.Ntpad * l.Ktpad;
     }
@@ -417,9 +414,10 @@ int vti_ae_create(int crop, int batch, unsigned long long seed, vti_ae **out)
     ok = ok && hipMalloc(&A->out, (size_t)batch * vol(c) * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&A->target, (size_t)batch * vol(c) * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&A->dev_loss, 64) == hipSuccess && hipMalloc(&A->scale, 64) == hipSuccess;
+    ok = ok && hipMalloc(&A->crop_scale, (size_t)batch * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&A->org, 3 * (size_t)batch * sizeof(int)) == hipSuccess;
     if (!ok) return aefail(VTI_ERR_ALLOC, "vti_ae_create: device buffers");
-    // He initialisation of the master weights (fp32), zero biases and moments
+    // He initialisation of the master weights (fp32), zero moments
     std::vector<float> h(off, 0.0f);
     std::normal_distribution<float> nd(0.0f, 1.0f);
     for (int i = 0; i < kNL; ++i) {
@@ -458,11 +456,11 @@ long long vti_ae_nparams(const vti_ae *A)
 {
     if (!A) return 0;
     long long n = 0;
-    for (const Layer &l : A->L) n += (long long)l.N * l.K + l.s.co;
+    for (const Layer &l : A->L) n += (long long)l.N * l.K;
     return n;
 }
 
-// flat = per layer: W (rows x cols, row-major, unpadded) then b (co)
+// flat = per layer: W (rows x cols, row-major, unpadded)
 static int params_io(vti_ae *A, float *flat, const float *in, bool grads)
 {
     std::vector<float> h(A->nparam, 0.0f);
@@ -478,11 +476,6 @@ static int params_io(vti_ae *A, float *flat, const float *in, bool grads)
                 if (in) d = in[f];
                 else flat[f] = d;
             }
-        for (int c = 0; c < l.s.co; ++c, ++f) {
-            float &d = h[l.b_off + c];
-            if (in) d = in[f];
-            else flat[f] = d;
-        }
     }
     if (in) return vti_ae_set_params_padded_(A, h.data());
     return VTI_OK;

@@ -67,8 +67,6 @@ struct WGrad {
 
 hipError_t launch_gemm(const Gemm &g, hipStream_t s);
 hipError_t launch_wgrad(const WGrad &g, hipStream_t s);
-// db[c] += sum over positions of d[pos][c], npos positions of co channels
-hipError_t launch_bgrad(const uint16_t *d, long long npos, int co, float *db, hipStream_t s);
 // loss += sum (out - t)^2 / n ; dout = bf16(2 (out - t) / n)
 hipError_t launch_mse(const float *out, const float *t, long long n, float *loss, uint16_t *dout, hipStream_t s);
 // Adam on n fp32 parameters; wb receives the bf16 copy
@@ -84,9 +82,10 @@ hipError_t launch_bf16_to_f32(const uint16_t *in, float *out, long long n, hipSt
 // max |v| over an (ny, nx, nz) box of a strided fp32 volume, as float bits
 hipError_t launch_maxabs(const float *v, long long off, long long sy, long long sx, int ny, int nx, int nz,
                          unsigned *out_bits, hipStream_t s);
-// crops: dst[c][y][x][z] = src[off + (oy_c + y) sy + (ox_c + x) sx + oz_c + z] / scale
+// crops: dst[c][y][x][z] = src[off + (oy_c + y) sy + (ox_c + x) sx + oz_c + z] / crop_scale[c],
+// crop_scale[c] = max |crop c| (an all-zero crop stays zero)
 hipError_t launch_crops(const float *src, long long off, long long sy, long long sx, const int *org, int ncrop,
-                        int n, const float *scale_ptr, float *dst, hipStream_t s);
+                        int n, float *crop_scale, float *dst, hipStream_t s);
 
 }  // namespace nn
 }  // namespace vti

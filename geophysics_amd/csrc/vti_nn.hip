@@ -195,16 +195,6 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WGrad g)
     }
 }
 
-__global__ void bgrad_kernel(const uint16_t *__restrict__ d, long long npos, int co, float *db)
-{
-    const int c = threadIdx.x % co, per = blockDim.x / co;
-    if (threadIdx.x >= per * co) return;
-    float s = 0.0f;
-    for (long long p = (long long)blockIdx.x * per + threadIdx.x / co; p < npos; p += (long long)gridDim.x * per)
-        s += bf2f(d[p * co + c]);
-    atomicAdd(db + c, s);
-}
-
 __global__ void mse_kernel(const float *__restrict__ out, const float *__restrict__ t, long long n, float *loss,
                            uint16_t *dout)
 {
@@ -307,11 +297,9 @@ __global__ void maxabs_kernel(const float *__restrict__ v, long long off, long l
 }
 
 __global__ void crops_kernel(const float *__restrict__ src, long long off, long long sy, long long sx,
-                             const int *__restrict__ org, int ncrop, int n, const float *__restrict__ scale_ptr,
-                             float *dst)
+                             const int *__restrict__ org, int ncrop, int n, float *dst)
 {
     const long long per = (long long)n * n * n, total = per * ncrop;
-    const float s = *scale_ptr;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x) {
         const int cidx = (int)(i / per);
         long long r = i % per;
@@ -319,8 +307,28 @@ __global__ void crops_kernel(const float *__restrict__ src, long long off, long 
         r /= n;
         const int x = (int)(r % n), y = (int)(r / n);
         const int *o = org + 3 * cidx;
-        dst[i] = src[off + (o[0] + y) * sy + (o[1] + x) * sx + o[2] + z] / s;
+        dst[i] = src[off + (o[0] + y) * sy + (o[1] + x) * sx + o[2] + z];
     }
+}
+
+// one workgroup per crop: scale[c] = max |crop c|, then crop c /= scale[c]
+// (an all-zero crop stays zero)
+__global__ void crop_norm_kernel(float *dst, long long per, float *scale)
+{
+    __shared__ float red[256];
+    float *p = dst + (long long)blockIdx.x * per;
+    float m = 0.0f;
+    for (long long i = threadIdx.x; i < per; i += blockDim.x) m = fmaxf(m, fabsf(p[i]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    const float sc = red[0];
+    if (threadIdx.x == 0) scale[blockIdx.x] = sc;
+    if (!(sc > 0.0f)) return;
+    for (long long i = threadIdx.x; i < per; i += blockDim.x) p[i] = p[i] / sc;
 }
 
 unsigned blocks_for(long long n, int per = 256, unsigned cap = 8192)
@@ -351,12 +359,6 @@ hipError_t launch_wgrad(const WGrad &g, hipStream_t s)
     const long long per_block = 4LL * g.chunk;
     const dim3 grd((unsigned)((P + per_block - 1) / per_block), (unsigned)((g.Npad / 16) * (g.Kpad / 16)));
     hipLaunchKernelGGL(wgrad_kernel, grd, dim3(256), 0, s, g);
-    return hipGetLastError();
-}
-
-hipError_t launch_bgrad(const uint16_t *d, long long npos, int co, float *db, hipStream_t s)
-{
-    hipLaunchKernelGGL(bgrad_kernel, dim3(256), dim3(256), 0, s, d, npos, co, db);
     return hipGetLastError();
 }
 
@@ -414,10 +416,11 @@ hipError_t launch_maxabs(const float *v, long long off, long long sy, long long 
 }
 
 hipError_t launch_crops(const float *src, long long off, long long sy, long long sx, const int *org, int ncrop, int n,
-                        const float *scale_ptr, float *dst, hipStream_t s)
+                        float *crop_scale, float *dst, hipStream_t s)
 {
-    hipLaunchKernelGGL(crops_kernel, dim3(blocks_for((long long)ncrop * n * n * n)), dim3(256), 0, s, src, off, sy, sx,
-                       org, ncrop, n, scale_ptr, dst);
+    const long long per = (long long)n * n * n;
+    hipLaunchKernelGGL(crops_kernel, dim3(blocks_for(ncrop * per)), dim3(256), 0, s, src, off, sy, sx, org, ncrop, n, dst);
+    hipLaunchKernelGGL(crop_norm_kernel, dim3(ncrop), dim3(256), 0, s, dst, per, crop_scale);
     return hipGetLastError();
 }
 

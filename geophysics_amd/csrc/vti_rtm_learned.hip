@@ -6,11 +6,11 @@
 // it % save_every == 0 the interior P box (n^3, n a multiple of 8) is
 // encoded into a latent slot (n^3/8 bf16 values + one scale).  Optionally
 // every check_every-th stored snapshot is decoded at once and its relative
-// L2 error against the true field measured; a snapshot above tol is marked.
-// A few full-state checkpoints (every ck_every steps) are kept as the
-// fallback: in the reverse sweep a marked snapshot is recomputed exactly
-// from the nearest checkpoint (plain checkpoint recomputation), every other
-// imaging step decodes its latent.  The receiver side is the same as in
+// L2 error against the true field measured.  A snapshot above tol takes the
+// fallback: it is kept exactly (uncompressed interior, up to max_exact of
+// them), or, once those slots are used, recomputed in the reverse sweep
+// from full-state checkpoints kept every ck_every steps (plain checkpoint
+// recomputation).  Every other imaging step decodes its latent.  The receiver side is the same as in
 // vti_rtm_shot; imaging steps are it % save_every == 0 (image_every =
 // save_every there), so the two paths differ only by the compression error.
 #include <hip/hip_runtime.h>
@@ -41,6 +41,18 @@ __global__ void image_dense_kernel(Grid g, int np, int nx, int ny, int nz, const
     const long long c = (iz + np) + (long long)(ix + np) * g.ldz + (long long)(iy + np) * g.ldxy;
     const long long o = iz + (long long)ix * nz + (long long)iy * nz * nx;
     img[o] += S[o] * gP[c];
+}
+
+// out = the interior n^3 box of P (dense, z fastest)
+__global__ void interior_kernel(Grid g, int np, int n, const float *__restrict__ P, float *out)
+{
+    const long long tot = (long long)n * n * n;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < tot; i += (long long)gridDim.x * blockDim.x) {
+        const int z = (int)(i % n);
+        const long long r = i / n;
+        const int x = (int)(r % n), y = (int)(r / n);
+        out[i] = P[(z + np) + (long long)(x + np) * g.ldz + (long long)(y + np) * g.ldxy];
+    }
 }
 
 // sums of (S - P)^2 and P^2 over the interior (double atomics on two cells)
@@ -97,6 +109,7 @@ int vti_learned_defaults(vti_learned_opts *o)
     o->check_every = 8;
     o->tol = 0.05f;
     o->ck_every = 0;
+    o->max_exact = 0;
     return VTI_OK;
 }
 
@@ -110,7 +123,7 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
     if (k.ndim != 3 || k.ny != n || k.nz != n || n % 8)
         return fail(VTI_ERR_ARG, "vti_rtm_shot_learned: 3D cubic interiors with a side divisible by 8 only");
     if (c->nrec <= 0) return fail(VTI_ERR_STATE, "vti_rtm_shot_learned: the shot has no receivers");
-    if (lo->save_every < 1 || lo->check_every < 0 || lo->ck_every < 0)
+    if (lo->save_every < 1 || lo->check_every < 0 || lo->ck_every < 0 || lo->max_exact < 0)
         return fail(VTI_ERR_ARG, "vti_rtm_shot_learned: bad options");
     int st;
     if (!c->img && (st = vti_rtm_reset(c))) return st;
@@ -121,7 +134,7 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
     const int nsave = (nt + se - 1) / se;
     const long long off = np + (long long)np * c->g.ldz + (long long)np * c->g.ldxy;
     hipStream_t s = c->stream;
-    Buf lat, scl, dense, acc;
+    Buf lat, scl, dense, acc, exa;
     if (hipMalloc(&lat.p, (size_t)nsave * lat_n * sizeof(uint16_t)) != hipSuccess ||
         hipMalloc(&scl.p, (size_t)nsave * 64) != hipSuccess ||
         hipMalloc(&dense.p, (size_t)n * n * n * sizeof(float)) != hipSuccess ||
@@ -131,6 +144,12 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
     float *Sc = static_cast<float *>(scl.p);
     float *D = static_cast<float *>(dense.p);
     double *E = static_cast<double *>(acc.p);
+    const size_t nn3 = (size_t)n * n * n;
+    if (lo->max_exact > 0 && hipMalloc(&exa.p, (size_t)lo->max_exact * nn3 * sizeof(float)) != hipSuccess)
+        return fail(VTI_ERR_ALLOC, "vti_rtm_shot_learned: exact fallback store");
+    float *X = static_cast<float *>(exa.p);
+    std::vector<int> exact(nsave, -1);
+    int nexact = 0;
     // fallback checkpoints
     const int nck = lo->ck_every > 0 ? (nt + lo->ck_every - 1) / lo->ck_every : 0;
     std::vector<VtiState> ck(nck);
@@ -182,10 +201,14 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
                 const double e = std::sqrt(h[0] / h[1]);
                 err_sum += e;
                 ++nchecked;
-                if (e > lo->tol && nck) {
-                    // this snapshot and its neighbours up to the next check are recomputed
-                    for (int q = j; q < std::min(nsave, j + lo->check_every); ++q) bad[q] = 1;
+                if (e > lo->tol) {
                     ++nbad;
+                    if (nexact < lo->max_exact && lo->check_every == 1) {   // keep this snapshot exactly
+                        hipLaunchKernelGGL(interior_kernel, dim3(2048), dim3(256), 0, s, c->g, np, n, P, X + (size_t)nexact * nn3);
+                        exact[j] = nexact++;
+                    } else if (nck) {   // this snapshot and the ones up to the next check are recomputed
+                        for (int q = j; q < std::min(nsave, j + lo->check_every); ++q) bad[q] = 1;
+                    }
                 }
             }
         }
@@ -210,7 +233,10 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         if (it % se) continue;
         const int j = it / se;
         const dim3 blk(64), grd((k.nz + 63) / 64, k.nx, k.ny);
-        if (bad[j]) {   // exact source field from the nearest checkpoint at or before it+1
+        if (exact[j] >= 0) {
+            hipLaunchKernelGGL(image_dense_kernel, grd, blk, 0, s, c->g, np, k.nx, k.ny, k.nz, X + (size_t)exact[j] * nn3,
+                               G.f[VTI_F_P], c->img);
+        } else if (bad[j]) {   // exact source field from the nearest checkpoint at or before it+1
             const int target = it + 1;
             const int ci = std::min((target - 1) / lo->ck_every, nck - 1);
             if ((st = hip(state_copy(c, W, ck[ci]), "restore"))) return st;
@@ -236,6 +262,7 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         stats[2] = nchecked ? err_sum / nchecked : -1.0;
         stats[3] = nbad;
         stats[4] = nsave;
+        stats[5] = nexact;
     }
     return VTI_OK;
 }

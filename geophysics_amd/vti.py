@@ -59,7 +59,8 @@ class RtmOpts(C.Structure):
 
 _lib = None
 class LearnedOpts(C.Structure):
-    _fields_ = [("save_every", C.c_int), ("check_every", C.c_int), ("tol", C.c_float), ("ck_every", C.c_int)]
+    _fields_ = [("save_every", C.c_int), ("check_every", C.c_int), ("tol", C.c_float), ("ck_every", C.c_int),
+                ("max_exact", C.c_int)]
 
 
 FP = C.POINTER(C.c_float)
@@ -379,15 +380,15 @@ class Propagator:
         autoencoder latents; returns the stats dict."""
         o = opts if opts is not None else self.rtm_defaults()
         lo = lopts if lopts is not None else learned_defaults()
-        st = np.zeros(5, np.float64)
+        st = np.zeros(6, np.float64)
         p = None
         if obs is not None:
             a, p = _f32(obs)
             self._obs_keep = a
         check(lib().vti_rtm_shot_learned(self.h, ae.h, C.byref(o), C.byref(lo), p, st.ctypes.data_as(DP)),
               "rtm_shot_learned")
-        return {"encode_ms": st[0], "decode_ms": st[1], "mean_checked_err": st[2], "recomputed": int(st[3]),
-                "snapshots": int(st[4])}
+        return {"encode_ms": st[0], "decode_ms": st[1], "mean_checked_err": st[2], "over_tol": int(st[3]),
+                "snapshots": int(st[4]), "exact": int(st[5])}
 
     def ae_train_shot(self, ae, save_every=4, nsteps=2):
         loss = np.zeros(1, np.float32)

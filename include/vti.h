@@ -258,16 +258,17 @@ int vti_batch_record(vti_batch *b, int shot, float *dst);
  * counterpart in the reference, whose RTM keeps no snapshots and re-runs the
  * forward operator from the final state
  * (Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:1423-1437).  Layers: down-sampling
- * 2x2x2 convolutions 1->16->32->64, a 3x3x3 convolution 64->64 (latent),
+ * 2x2x2 convolutions 1->8->64->64, a 3x3x3 convolution 64->64 (latent),
  * a 3x3x3 convolution 64->64 and transposed 2x2x2 convolutions
- * 64->32->16->1; leaky ReLU (slope 1/8) between layers. */
+ * 64->64->8->1; leaky ReLU (slope 1/8) between layers; no biases (the
+ * network is positively homogeneous: zero in, zero out). */
 typedef struct vti_ae vti_ae;
 /* crop: side of the training crops (multiple of 8); batch: crops per step. */
 int vti_ae_create(int crop, int batch, unsigned long long seed, vti_ae **out);
 int vti_ae_destroy(vti_ae *ae);
 /* Parameter count of the flat form: per layer its weight matrix (rows x
  * cols, row-major; down/same: rows = out channels, cols = tap*in + c;
- * up: rows = tap*out + c, cols = in channels), then its biases. */
+ * up: rows = tap*out + c, cols = in channels). */
 long long vti_ae_nparams(const vti_ae *ae);
 int vti_ae_set_params(vti_ae *ae, const float *flat);
 int vti_ae_get_params(vti_ae *ae, float *flat);
@@ -282,7 +283,9 @@ int vti_ae_train_batch(vti_ae *ae, const float *crops, int apply, float *loss, f
  * fp32 host values; i = 8: the reconstruction. */
 int vti_ae_activation(vti_ae *ae, int i, float *dst);
 /* nsteps Adam steps on random crops of the (ny, nx, nz) box of a device
- * field P[off + y*sy + x*sx + z], each crop scaled by the box's max |P|. */
+ * field P[off + y*sy + x*sx + z], each crop scaled by its own max |P| (the
+ * network is positively homogeneous, so the scale does not change what it
+ * learns, only how much every crop weighs). */
 int vti_ae_train_field(vti_ae *ae, const float *P, long long off, long long sy, long long sx,
                        int ny, int nx, int nz, int nsteps, float *loss);
 /* Device-side encode of an n^3 box (n a multiple of 8) into latent
@@ -299,18 +302,22 @@ long long vti_ae_latent_elems(int n);
  * decoded field (vti_rtm_shot with image_every = save_every images the same
  * steps with the exact field).  Fallback: every check_every-th snapshot is
  * decoded at once and compared with the true field; when its relative L2
- * error exceeds tol and ck_every > 0, that snapshot and the next
- * check_every - 1 are recomputed exactly from full-state checkpoints kept
- * every ck_every steps (plain checkpoint recomputation). */
+ * error exceeds tol it is kept exactly (uncompressed interior; with
+ * check_every == 1, up to max_exact snapshots) or, past that, it and the
+ * next check_every - 1 are recomputed exactly from full-state checkpoints
+ * kept every ck_every steps (plain checkpoint recomputation; ck_every = 0:
+ * no checkpoints, the latent is used). */
 typedef struct {
     int save_every;    /* 4 */
     int check_every;   /* 8; 0: no checks */
     float tol;         /* 0.05 */
     int ck_every;      /* 0: no fallback checkpoints */
+    int max_exact;     /* 0: no exactly kept snapshots */
 } vti_learned_opts;
 int vti_learned_defaults(vti_learned_opts *o);
-/* stats (optional, 5 doubles): encode ms, decode ms (host-timed), mean
- * checked relative error, snapshots marked for recomputation, snapshots. */
+/* stats (optional, 6 doubles): encode ms, decode ms (host-timed), mean
+ * checked relative error, snapshots above tol, snapshots, snapshots kept
+ * exactly. */
 int vti_rtm_shot_learned(vti_ctx *ctx, vti_ae *ae, const vti_rtm_opts *opts,
                          const vti_learned_opts *lo, const float *obs, double *stats);
 /* Forward modelling of the current shot with nsteps Adam steps of the

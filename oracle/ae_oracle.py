@@ -19,6 +19,7 @@ network of geophysics_amd/csrc/vti_nn.hip term by term:
     (T = 2 or 3, same-taps offset by -1);
   - up:          W[n][ci], n = tap * co + c, tap = (dy*2 + dx)*2 + dz;
 * activation: leaky ReLU with slope 1/8 (exact in bf16 and fp32) or none;
+  the network has no biases (positively homogeneous: zero in, zero out);
 * bf16 (round to nearest even) at the points the GPU stores bf16: weights,
   activations, the latent, activation gradients; fp32 master weights and
   Adam moments; the reconstruction and the loss in fp32.
@@ -32,14 +33,14 @@ LEAK = 0.125
 
 # (kind, ci, co, act) of the 4-level encoder / decoder; latent after "E4"
 NET = [
-    ("down", 1, 16, 1),    # E1  N   -> N/2
-    ("down", 16, 32, 1),   # E2  N/2 -> N/4
-    ("down", 32, 64, 1),   # E3  N/4 -> N/8
+    ("down", 1, 8, 1),     # E1  N   -> N/2
+    ("down", 8, 64, 1),    # E2  N/2 -> N/4
+    ("down", 64, 64, 1),   # E3  N/4 -> N/8
     ("same", 64, 64, 0),   # E4  latent (64 ch at N/8: 16x fewer bytes in bf16 than fp32 input)
     ("same", 64, 64, 1),   # D4
-    ("up", 64, 32, 1),     # D3  N/8 -> N/4
-    ("up", 32, 16, 1),     # D2  N/4 -> N/2
-    ("up", 16, 1, 0),      # D1  N/2 -> N
+    ("up", 64, 64, 1),     # D3  N/8 -> N/4
+    ("up", 64, 8, 1),      # D2  N/4 -> N/2
+    ("up", 8, 1, 0),       # D1  N/2 -> N
 ]
 N_ENC = 4
 
@@ -157,14 +158,13 @@ def conv_bwd_weight(kind, x, gpre):
 
 
 def init_params(seed=0):
-    """He-style initial fp32 master weights and zero biases, per layer."""
+    """He-style initial fp32 master weights, one matrix per layer."""
     rng = np.random.default_rng(seed)
     params = []
     for kind, ci, co, a in NET:
         rows, cols = kdim(kind, ci, co)
         fan_in = cols if kind != "up" else ci
-        w = (rng.standard_normal((rows, cols)) * np.sqrt(2.0 / fan_in)).astype(np.float32)
-        params.append((w, np.zeros(co, np.float32)))
+        params.append((rng.standard_normal((rows, cols)) * np.sqrt(2.0 / fan_in)).astype(np.float32))
     return params
 
 
@@ -174,15 +174,14 @@ def forward(params, x):
     h = bf16(x)
     outs = [h]
     for li, (kind, ci, co, a) in enumerate(NET):
-        w, b = params[li]
-        y = conv_fwd(kind, h, bf16(w), b, a).astype(np.float32)
+        y = conv_fwd(kind, h, bf16(params[li]), np.zeros(co), a).astype(np.float32)
         h = y if li == len(NET) - 1 else bf16(y)
         outs.append(h)
     return outs
 
 
 def loss_and_grads(params, x):
-    """MSE loss (fp32 target x) and fp32 gradients of every (w, b),
+    """MSE loss (fp32 target x) and fp32 gradients of every weight matrix,
     following the GPU's rounding points: dL/dout, and every activation
     gradient handed to the next layer down, are rounded to bf16."""
     outs = forward(params, x)
@@ -194,12 +193,11 @@ def loss_and_grads(params, x):
     grads = [None] * len(NET)
     for li in range(len(NET) - 1, -1, -1):
         kind, ci, co, a = NET[li]
-        w, b = params[li]
-        gpre = g if li == len(NET) - 1 else g      # act' already folded into g below
-        dW, db = conv_bwd_weight(kind, outs[li], gpre)
-        grads[li] = (dW.astype(np.float32), db.astype(np.float32))
+        gpre = g                                    # act' already folded into g below
+        dW, _ = conv_bwd_weight(kind, outs[li], gpre)
+        grads[li] = dW.astype(np.float32)
         if li > 0:
-            dx = conv_bwd_data(kind, gpre, bf16(w), outs[li].shape)
+            dx = conv_bwd_data(kind, gpre, bf16(params[li]), outs[li].shape)
             pa = NET[li - 1][3]
             g = bf16(act_grad(outs[li], dx, pa).astype(np.float32))
     return loss, grads

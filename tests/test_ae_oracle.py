@@ -80,40 +80,48 @@ def test_network_shapes_and_compression():
     assert x.size * 4 == 16 * lat.size * 2
 
 
+def test_network_is_positively_homogeneous():
+    """No biases: zero in, zero out, and f(a x) = a f(x) for a > 0 (a a
+    power of two, so bf16 rounding commutes with the scaling)."""
+    p = ae.init_params(2)
+    x = np.random.default_rng(4).standard_normal((1, 16, 16, 16, 1)).astype(np.float32)
+    assert not ae.forward(p, np.zeros_like(x))[-1].any()
+    y1 = ae.forward(p, x)[-1]
+    y4 = ae.forward(p, 4.0 * x)[-1]
+    assert np.allclose(y4, 4.0 * y1, rtol=1e-6, atol=1e-12)
+
+
 def test_network_gradient_matches_finite_differences():
     """Loss gradient of the oracle network in float64 (no bf16 rounding)
     against central differences along random directions."""
     rng = np.random.default_rng(3)
-    p = [(w.astype(np.float64), b.astype(np.float64) + 0.01 * rng.standard_normal(b.shape))
-         for w, b in ae.init_params(5)]
+    p = [w.astype(np.float64) for w in ae.init_params(5)]
     x = rng.standard_normal((1, 8, 8, 8, 1))
 
     def loss64(params):
         h = x
         for li, (kind, ci, co, a) in enumerate(ae.NET):
-            h = ae.conv_fwd(kind, h, params[li][0], params[li][1], a)
+            h = ae.conv_fwd(kind, h, params[li], np.zeros(co), a)
         return float(((h - x) ** 2).mean())
 
     # analytic gradient: the oracle's backward without rounding
     outs = [x]
     h = x
     for li, (kind, ci, co, a) in enumerate(ae.NET):
-        h = ae.conv_fwd(kind, h, p[li][0], p[li][1], a)
+        h = ae.conv_fwd(kind, h, p[li], np.zeros(co), a)
         outs.append(h)
     g = 2.0 * (outs[-1] - x) / x.size
     grads = [None] * len(ae.NET)
     for li in range(len(ae.NET) - 1, -1, -1):
         kind = ae.NET[li][0]
-        grads[li] = ae.conv_bwd_weight(kind, outs[li], g)
+        grads[li] = ae.conv_bwd_weight(kind, outs[li], g)[0]
         if li > 0:
-            g = ae.act_grad(outs[li], ae.conv_bwd_data(kind, g, p[li][0], outs[li].shape), ae.NET[li - 1][3])
+            g = ae.act_grad(outs[li], ae.conv_bwd_data(kind, g, p[li], outs[li].shape), ae.NET[li - 1][3])
     for trial in range(4):
-        dirs = [(rng.standard_normal(w.shape), rng.standard_normal(b.shape)) for w, b in p]
-        ana = sum(float((gw * dw).sum() + (gb * db).sum()) for (gw, gb), (dw, db) in zip(grads, dirs))
+        dirs = [rng.standard_normal(w.shape) for w in p]
+        ana = sum(float((gw * dw).sum()) for gw, dw in zip(grads, dirs))
         h = 1e-6
-        pp = [(w + h * dw, b + h * db) for (w, b), (dw, db) in zip(p, dirs)]
-        pm = [(w - h * dw, b - h * db) for (w, b), (dw, db) in zip(p, dirs)]
-        num = (loss64(pp) - loss64(pm)) / (2 * h)
+        num = (loss64([w + h * d for w, d in zip(p, dirs)]) - loss64([w - h * d for w, d in zip(p, dirs)])) / (2 * h)
         assert abs(num - ana) <= 1e-6 * max(1.0, abs(ana)), (trial, num, ana)
 
 

@@ -34,10 +34,11 @@ def _plain(prop, obs, nsnap=6):
     return prop.rtm_image(finalize=False)
 
 
-def test_fallback_recomputation_equals_plain_rtm_bitwise():
-    """Every checked snapshot above the tolerance is recomputed from the
-    fallback checkpoints: with tol < 0 and a check on every snapshot the
-    learned path images exactly the plain path's field."""
+@pytest.mark.parametrize("exact", [0, 1])
+def test_fallback_equals_plain_rtm_bitwise(exact):
+    """With tol < 0 and a check on every snapshot, every snapshot takes the
+    fallback (exact storage, or recomputation from checkpoints): the learned
+    path then images exactly the plain path's field."""
     cfg, vp, e, d = _setup()
     rng = np.random.default_rng(4)
     with vti.Propagator(cfg, vp, e, d) as prop, vti.Autoencoder(crop=16, batch=1, seed=1) as A:
@@ -46,39 +47,47 @@ def test_fallback_recomputation_equals_plain_rtm_bitwise():
         img_p, ill_p = _plain(prop, obs)
         o = prop.rtm_defaults()
         lo = vti.learned_defaults()
-        lo.check_every, lo.tol, lo.ck_every = 1, -1.0, 16
+        lo.check_every, lo.tol = 1, -1.0
+        if exact:
+            lo.max_exact = NT // 4
+        else:
+            lo.ck_every = 16
         prop.rtm_reset()
         st = prop.rtm_shot_learned(A, o, lo, obs)
         img_l, ill_l = prop.rtm_image(finalize=False)
     _say("stats:", st)
     assert st["snapshots"] == NT // 4
-    assert st["recomputed"] > 0
+    assert st["over_tol"] > 0
+    assert st["exact"] == (st["over_tol"] if exact else 0)
     assert np.array_equal(ill_l, ill_p)
     assert np.array_equal(img_l, img_p), rel_l2(img_l, img_p)
 
 
 def test_trained_autoencoder_image_error():
     """Online training on two shots' snapshots, then a third shot imaged
-    from latents only (no fallback) against the plain RTM image."""
+    from latents, without and with the exact-storage fallback, against the
+    plain RTM image of the same steps."""
     cfg, vp, e, d = _setup()
-    with vti.Propagator(cfg, vp, e, d) as prop, vti.Autoencoder(crop=16, batch=8, seed=2) as A:
-        A.set_lr(2e-3)
+    with vti.Propagator(cfg, vp, e, d) as prop, vti.Autoencoder(crop=32, batch=2, seed=2) as A:
+        A.set_lr(1e-3)
         losses = []
-        for rep in range(3):
+        for rep in range(4):
             for cx, cy in ((10, 12), (22, 20)):
                 prop.set_shot(_shot(cfg, cx, cy))
-                losses.append(prop.ae_train_shot(A, save_every=4, nsteps=4))
+                losses.append(prop.ae_train_shot(A, save_every=4, nsteps=6))
         _say("mean training loss per pass:", losses)
         prop.set_shot(_shot(cfg, 16, 16))
         img_p, ill_p = _plain(prop, None)
-        o = prop.rtm_defaults()
-        lo = vti.learned_defaults()
-        lo.check_every, lo.ck_every = 4, 0
-        prop.rtm_reset()
-        st = prop.rtm_shot_learned(A, o, lo, None)
-        img_l, ill_l = prop.rtm_image(finalize=False)
-    err = rel_l2(img_l, img_p)
-    _say(f"learned vs plain image rel-L2 {err:.3e}; mean snapshot error {st['mean_checked_err']:.3e}; stats {st}")
-    assert np.array_equal(ill_l, ill_p)
+        res = {}
+        for name, max_exact in (("latent only", 0), ("with exact fallback", NT // 8)):
+            lo = vti.learned_defaults()
+            lo.check_every, lo.tol, lo.ck_every, lo.max_exact = 1, 0.05, 0, max_exact
+            prop.rtm_reset()
+            st = prop.rtm_shot_learned(A, prop.rtm_defaults(), lo, None)
+            img_l, ill_l = prop.rtm_image(finalize=False)
+            assert np.array_equal(ill_l, ill_p)
+            res[name] = rel_l2(img_l, img_p)
+            _say(f"{name}: image rel-L2 {res[name]:.3e}; mean snapshot error {st['mean_checked_err']:.3e}; {st}")
     assert losses[-1] < losses[0]
-    assert err < 0.3
+    assert res["with exact fallback"] <= res["latent only"]
+    assert res["with exact fallback"] < 0.1
