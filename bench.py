@@ -159,7 +159,7 @@ def bench_batch_rtm(args, torch, vti):
     cfgs, shots, models = [], [], []
     for k in range(nshot):
         nx, nz = int(rng.integers(300, 1201)), int(rng.integers(200, 801))
-        favg = float(rng.uniform(10.0, 25.0))
+        favg = int(rng.integers(10, 26))          # the reference drivers take an integer frequency
         cfg = vti.cfg_ref2d(vti.MODE_GPU_REF, nx, nz, 10, 10, 20, 1, favg, nt, 1000)
         cfgs.append(cfg)
         shots.append(vti.shot_ref2d(cfg, 1, nx // 2, 0, 2, rec_depth=2))
@@ -209,7 +209,8 @@ def bench_fwi_grad(args, torch, vti):
     for sgn, (cx, cz) in ((1.0, (0.35 * nx, 0.5 * nz)), (-1.0, (0.65 * nx, 0.6 * nz))):
         vp = vp + (sgn * 300.0 * np.exp(-((X - cx) ** 2 + (Z - cz) ** 2) / (2 * 20.0 ** 2))).astype(np.float32)
     v0 = gaussian_filter(vp.astype(np.float64), 15.0, mode="nearest")
-    cfg = vti.cfg_ref2d(vti.MODE_GPU_REF, nx, nz, dx, dx, 20, 0, 10, nt, 1000)
+    cfg = vti.cfg_ref2d(vti.MODE_GPU_REF, nx, nz, 12, 12, 20, 0, 10, nt, 1000)
+    cfg.dx = cfg.dz = dx                           # the reference helpers take integer spacings
     log(f"[bench] FWI gradient: {nshot} shots, nt {nt}, grid {nx}x{nz}")
     G = vti.Gradient(cfg, fp64=False)
     total = np.zeros((nx, nz))
@@ -244,9 +245,11 @@ def bench_fwi_grad(args, torch, vti):
         # GPU FP32 gradient of the same case for the error statement
         from oracle import pyoracle as po
         nt_c = min(nt, 300)
-        cfg_c = vti.cfg_ref2d(vti.MODE_GPU_REF, nx, nz, dx, dx, 20, 0, 10, nt_c, 1000)
+        cfg_c = vti.cfg_ref2d(vti.MODE_GPU_REF, nx, nz, 12, 12, 20, 0, 10, nt_c, 1000)
+        cfg_c.dx = cfg_c.dz = dx
         shot_c = vti.shot_ref2d(cfg_c, 1, 20, 0, 2, rec_depth=2)
-        p = po.params_gpu2d(nx, nz, dx, dx, 20, 0, 10, 1, 20, 0, 2, nt_c, 1000, 2)
+        p = po.params_gpu2d(nx, nz, 12, 12, 20, 0, 10, 1, 20, 0, 2, nt_c, 1000, 2)
+        p.dx = p.dz = dx
         dobs_c = po.Sim(p, vp, ep, dl, "f64").run()
         t0 = time.perf_counter()
         J64, g64, _, _ = po.fwi_grad(po.Sim(p, v0, ep, dl, "f64"), dobs=dobs_c)
@@ -298,21 +301,26 @@ def bench_rtm3d_learned(args, torch, vti):
     rng = np.random.default_rng(args.seed)
     pos = [(int(rng.integers(n // 4, 3 * n // 4)), int(rng.integers(n // 4, 3 * n // 4))) for _ in range(nshot)]
     A = vti.Autoencoder(crop=64, batch=4, seed=args.seed)
-    A.set_lr(1e-3)
-    # plain path: checkpoint slots in the same memory as the latent store
-    lat_bytes = (nt // 4) * vti.Autoencoder.latent_elems(n) * 2
+    nsave = (nt + 3) // 4
+    max_exact = max(1, nsave // 10)               # exact-storage fallback budget: 10 % of the snapshots
+    lat_bytes = nsave * vti.Autoencoder.latent_elems(n) * 2
+    exact_bytes = max_exact * n ** 3 * 4
+    # plain path: checkpoint slots in the same memory as latents + fallback store
     state_bytes = 11 * 4 * (n + 2 * npml) ** 2 * ((n + 2 * npml + 31) // 32 * 32)
-    nsnap_equal = max(1, int(lat_bytes // state_bytes))
+    nsnap_equal = max(1, int((lat_bytes + exact_bytes) // state_bytes))
     rows = []
-    t_train = 0.0
+    t_fit = 0.0
     ntrain = min(3, nshot)
+    explained = None
     for k, (cx, cy) in enumerate(pos):
         prop.set_shot(vti.shot_ref3d(cfg, 0, cx, 0, cy, 0, 1, 0, rec_depth=3, rec_step=2))
         if k < ntrain:
             t0 = time.perf_counter()
-            loss = prop.ae_train_shot(A, save_every=4, nsteps=2)
-            t_train += time.perf_counter() - t0
-            log(f"[bench] training shot {k}: mean loss {loss:.4e}")
+            prop.ae_fit_shot(A, save_every=4, nblocks=2048)
+            if k == ntrain - 1:
+                explained = A.fit_end()
+            t_fit += time.perf_counter() - t0
+            log(f"[bench] fit shot {k} done ({time.perf_counter() - t0:.1f} s)")
             continue
         o = prop.rtm_defaults()
         o.image_every = 4
@@ -324,17 +332,22 @@ def bench_rtm3d_learned(args, torch, vti):
         prop.sync()
         t_plain = time.perf_counter() - t0
         img_p, _ = prop.rtm_image(finalize=False)
+        fin_p, _ = prop.rtm_image(finalize=True)
         lo = vti.learned_defaults()
-        lo.check_every, lo.ck_every = 16, 0
+        lo.check_every, lo.tol, lo.ck_every, lo.max_exact = 1, 0.02, 0, max_exact
         prop.rtm_reset()
         t0 = time.perf_counter()
         st = prop.rtm_shot_learned(A, prop.rtm_defaults(), lo, None)
         t_learn = time.perf_counter() - t0
         img_l, _ = prop.rtm_image(finalize=False)
-        err = float(np.linalg.norm(img_l - img_p) / max(np.linalg.norm(img_p), 1e-30))
-        rows.append((t_plain, t_learn, err, fsteps, st))
+        fin_l, _ = prop.rtm_image(finalize=True)
+        err_raw = float(np.linalg.norm(img_l - img_p) / max(np.linalg.norm(img_p), 1e-30))
+        err = float(np.linalg.norm(fin_l - fin_p) / max(np.linalg.norm(fin_p), 1e-30))
+        rows.append((t_plain, t_learn, err, fsteps, st, err_raw))
         log(f"[bench] shot {k}: plain {t_plain:.2f} s ({fsteps} fwd steps, {nsnap_equal} slots), "
-            f"learned {t_learn:.2f} s, image rel-L2 {err:.3e}, snapshot err {st['mean_checked_err']:.3e}")
+            f"learned {t_learn:.2f} s, final image rel-L2 {err:.3e} (raw {err_raw:.3e}), "
+            f"snapshot err {st['mean_checked_err']:.3e}, "
+            f"exact {st['exact']}/{st['snapshots']}")
     prop.close()
     A.close()
     tp = float(np.mean([r[0] for r in rows])) if rows else 0.0
@@ -344,13 +357,17 @@ def bench_rtm3d_learned(args, torch, vti):
         "value": round(tp / tl, 3) if tl else None, "unit": "speedup (plain s / learned s)", "n_gpus": 1,
         "steps": nt, "warmup": 0, "ms_per_step": round(tl / nt * 1e3, 4) if rows else None,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16", "data": "synthetic",
-        "config": {"workload": f"3D VTI RTM {n}^3, {nt} steps, dt 0.8 ms, {ntrain} training shots then "
+        "config": {"workload": f"3D VTI RTM {n}^3, {nt} steps, dt 0.8 ms, 15 Hz, {ntrain} fitting shots then "
                                f"{len(rows)} shots imaged both ways",
                    "plain_s_per_shot": round(tp, 3), "learned_s_per_shot": round(tl, 3),
-                   "image_rel_l2_max": max((r[2] for r in rows), default=None),
-                   "image_rel_l2_mean": float(np.mean([r[2] for r in rows])) if rows else None,
+                   "final_image_rel_l2_max": max((r[2] for r in rows), default=None),
+                   "final_image_rel_l2_mean": float(np.mean([r[2] for r in rows])) if rows else None,
+                   "raw_image_rel_l2_mean": float(np.mean([r[5] for r in rows])) if rows else None,
+                   "snapshot_rel_err_mean": float(np.mean([r[4]["mean_checked_err"] for r in rows])) if rows else None,
                    "plain_checkpoint_slots": nsnap_equal, "plain_forward_steps": rows[0][3] if rows else None,
-                   "latent_store_GB": round(lat_bytes / 1e9, 3), "training_s": round(t_train, 2),
+                   "latent_store_GB": round(lat_bytes / 1e9, 3), "exact_store_GB": round(exact_bytes / 1e9, 3),
+                   "fit_s": round(t_fit, 2), "fit_energy_captured": explained,
+                   "exact_snapshots_mean": float(np.mean([r[4]["exact"] for r in rows])) if rows else None,
                    "encode_ms_per_shot": round(float(np.mean([r[4]["encode_ms"] for r in rows])), 1) if rows else None,
                    "decode_ms_per_shot": round(float(np.mean([r[4]["decode_ms"] for r in rows])), 1) if rows else None},
         "roofline": None, "cpu_baseline": None,

@@ -20,6 +20,10 @@ int ae_decode(vti_ae *A, const uint16_t *latent, const float *scale_dev, int n, 
 int ae_train_field(vti_ae *A, const float *P, long long off, long long sy, long long sx, int ny, int nx, int nz,
                    int nsteps, float *loss);
 hipStream_t ae_stream(vti_ae *A);
+// Principal-component fit: sample nblk random 8^3 blocks of the n^3 box
+// (scaled by its max |P|) into the Gram matrix; ae_fit_end sets the weights.
+int ae_fit_field(vti_ae *A, const float *P, long long off, long long sy, long long sx, int n, int nblk);
+int ae_fit_end(vti_ae *A, double *explained);
 
 }  // namespace vti
 

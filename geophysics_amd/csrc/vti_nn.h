@@ -75,6 +75,12 @@ hipError_t launch_adam(float *w, float *m, float *v, const float *g, uint16_t *w
 // Rebuild the backward-data matrices of one layer from its forward matrix.
 hipError_t launch_wtrans(int kind, int ci, int co, const uint16_t *w, int Kpad, uint16_t *wt, int Ktpad,
                          int Ntpad, hipStream_t s);
+// 8^3 blocks of an n^3 box as rows of 512 bf16 values (scaled by 1/scale),
+// in the order the encoder's third layer sees them
+hipError_t launch_gather_blocks(const float *P, long long off, long long sy, long long sx, int n, const int *blk,
+                                int nblk, const float *scale_ptr, uint16_t *B, hipStream_t s);
+// G (K x K fp32) += B^T B, B = P rows of K bf16 values (MFMA)
+hipError_t launch_gram(const uint16_t *B, long long P, int K, float *G, hipStream_t s);
 hipError_t launch_fill_u16(uint16_t *p, long long n, uint16_t v, hipStream_t s);
 hipError_t launch_fill_f32(float *p, long long n, float v, hipStream_t s);
 hipError_t launch_f32_to_bf16(const float *in, uint16_t *out, long long n, hipStream_t s);

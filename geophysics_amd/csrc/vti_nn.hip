@@ -331,6 +331,57 @@ __global__ void crop_norm_kernel(float *dst, long long per, float *scale)
     for (long long i = threadIdx.x; i < per; i += blockDim.x) p[i] = p[i] / sc;
 }
 
+// B[b][k] = bf16(P[block b voxel k] / scale), k = (t2*8 + t1)*8 + t0 with
+// the voxel offset (4 dy2 + 2 dy1 + dy0, ...) of an 8^3 block, tap bits
+// (dy, dx, dz) = (t >> 2, (t >> 1) & 1, t & 1): the order in which the
+// encoder's third layer sees a block.  blk[b] = index of the block in the
+// (n/8)^3 grid of the box.
+__global__ void gather_blocks_kernel(const float *__restrict__ P, long long off, long long sy, long long sx, int n,
+                                     const int *__restrict__ blk, int nblk, const float *__restrict__ scale_ptr,
+                                     uint16_t *B)
+{
+    const int nb8 = n / 8;
+    const float inv = 1.0f / *scale_ptr;
+    const long long total = (long long)nblk * 512;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x) {
+        const int b = (int)(i / 512), k = (int)(i % 512);
+        const int q = blk[b];
+        const int bz = q % nb8, bx = (q / nb8) % nb8, by = q / (nb8 * nb8);
+        const int t2 = k >> 6, t1 = (k >> 3) & 7, t0 = k & 7;
+        const int y = 4 * (t2 >> 2) + 2 * (t1 >> 2) + (t0 >> 2);
+        const int x = 4 * ((t2 >> 1) & 1) + 2 * ((t1 >> 1) & 1) + ((t0 >> 1) & 1);
+        const int z = 4 * (t2 & 1) + 2 * (t1 & 1) + (t0 & 1);
+        B[i] = f2bf(P[off + (8 * by + y) * sy + (8 * bx + x) * sx + 8 * bz + z] * inv);
+    }
+}
+
+// G[a][b] += sum_p B[p][a] B[p][b] (K x K, K a multiple of 16) on MFMA:
+// one 16x16 tile of G per wavefront and chunk of positions, fp32 atomics.
+__global__ __launch_bounds__(256) void gram_kernel(const uint16_t *__restrict__ B, long long P, int K, float *G,
+                                                   int chunk)
+{
+    const int lane = threadIdx.x & 63;
+    const int nt = K / 16;
+    const int a0 = (blockIdx.y / nt) * 16, b0 = (blockIdx.y % nt) * 16;
+    const long long ps = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * chunk;
+    if (ps >= P) return;   // wavefront-uniform
+    const long long pe = ps + chunk < P ? ps + chunk : P;
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (long long p = ps; p < pe; p += 32) {
+        us8 av = {0, 0, 0, 0, 0, 0, 0, 0}, bv = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const long long pos = p + 8 * (lane >> 4) + e;
+            if (pos >= pe) continue;
+            av[e] = B[pos * K + a0 + (lane & 15)];
+            bv[e] = B[pos * K + b0 + (lane & 15)];
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(av), as_bf(bv), acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) atomicAdd(G + (long long)(a0 + 4 * (lane >> 4) + e) * K + b0 + (lane & 15), acc[e]);
+}
+
 unsigned blocks_for(long long n, int per = 256, unsigned cap = 8192)
 {
     const long long b = (n + per - 1) / per;
@@ -338,6 +389,22 @@ unsigned blocks_for(long long n, int per = 256, unsigned cap = 8192)
 }
 
 }  // namespace
+
+hipError_t launch_gather_blocks(const float *P, long long off, long long sy, long long sx, int n, const int *blk,
+                                int nblk, const float *scale_ptr, uint16_t *B, hipStream_t s)
+{
+    hipLaunchKernelGGL(gather_blocks_kernel, dim3(blocks_for((long long)nblk * 512)), dim3(256), 0, s, P, off, sy, sx, n,
+                       blk, nblk, scale_ptr, B);
+    return hipGetLastError();
+}
+
+hipError_t launch_gram(const uint16_t *B, long long P, int K, float *G, hipStream_t s)
+{
+    const int chunk = 256;
+    const dim3 grd((unsigned)((P + 4LL * chunk - 1) / (4LL * chunk)), (unsigned)((K / 16) * (K / 16)));
+    hipLaunchKernelGGL(gram_kernel, grd, dim3(256), 0, s, B, P, K, G, chunk);
+    return hipGetLastError();
+}
 
 hipError_t launch_gemm(const Gemm &g, hipStream_t s)
 {

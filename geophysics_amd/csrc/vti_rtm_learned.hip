@@ -267,6 +267,32 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
     return VTI_OK;
 }
 
+// Forward modelling of the current shot (no record kept); every
+// save_every-th snapshot contributes nblocks sampled 8^3 blocks to the
+// autoencoder's principal-component fit (finished by vti_ae_fit_end).
+int vti_ae_fit_shot(vti_ctx *c, vti_ae *A, int save_every, int nblocks)
+{
+    if (!c || !A || save_every < 1 || nblocks < 1) return fail(VTI_ERR_ARG, "vti_ae_fit_shot: bad arguments");
+    if (!c->have_model || !c->have_shot) return fail(VTI_ERR_STATE, "vti_ae_fit_shot: model and shot first");
+    const vti_cfg &k = c->cfg;
+    if (k.ndim != 3 || k.ny != k.nx || k.nz != k.nx || k.nx % 8)
+        return fail(VTI_ERR_ARG, "vti_ae_fit_shot: 3D cubic interiors with a side divisible by 8 only");
+    const int np = k.npml;
+    const long long off = np + (long long)np * c->g.ldz + (long long)np * c->g.ldxy;
+    const VtiState W = main_state(c);
+    if (state_zero(c, W) != hipSuccess) return fail(VTI_ERR_HIP, "zero");
+    for (int it = 0; it < k.nt; ++it) {
+        if (launch_vel(3, vel_args_for(c, W, it, true), c->stream) != hipSuccess ||
+            launch_stress(3, stress_args_for(c, W, it, true, k.sv != 0, nullptr), c->stream) != hipSuccess)
+            return fail(VTI_ERR_HIP, "step");
+        if (it % save_every) continue;
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(VTI_ERR_HIP, "sync");   // the fit runs on its own stream
+        const int st = ae_fit_field(A, W.f[VTI_F_P], off, c->g.ldxy, c->g.ldz, k.nx, nblocks);
+        if (st) return st;
+    }
+    return hipStreamSynchronize(c->stream) == hipSuccess ? VTI_OK : fail(VTI_ERR_HIP, "sync");
+}
+
 // Forward modelling of the current shot (no record kept) with nsteps Adam
 // steps of the autoencoder on every save_every-th snapshot.
 int vti_ae_train_shot(vti_ctx *c, vti_ae *A, int save_every, int nsteps, float *mean_loss)

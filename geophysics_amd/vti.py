@@ -133,6 +133,10 @@ def _declare(L):
         "vti_ae_encode": (C.c_int, [VP, VP, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, VP, VP]),
         "vti_ae_decode": (C.c_int, [VP, VP, VP, C.c_int, VP]),
         "vti_ae_latent_elems": (C.c_longlong, [C.c_int]),
+        "vti_ae_fit_field": (C.c_int, [VP, VP, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_int]),
+        "vti_ae_fit_end": (C.c_int, [VP, DP]),
+        "vti_sym_eigen": (C.c_int, [C.c_int, DP, DP, DP]),
+        "vti_ae_fit_shot": (C.c_int, [VP, VP, C.c_int, C.c_int]),
         "vti_learned_defaults": (C.c_int, [C.POINTER(LearnedOpts)]),
         "vti_rtm_shot_learned": (C.c_int, [VP, VP, C.POINTER(RtmOpts), C.POINTER(LearnedOpts), FP, DP]),
         "vti_ae_train_shot": (C.c_int, [VP, VP, C.c_int, C.c_int, FP]),
@@ -389,6 +393,9 @@ class Propagator:
               "rtm_shot_learned")
         return {"encode_ms": st[0], "decode_ms": st[1], "mean_checked_err": st[2], "over_tol": int(st[3]),
                 "snapshots": int(st[4]), "exact": int(st[5])}
+
+    def ae_fit_shot(self, ae, save_every=4, nblocks=4096):
+        check(lib().vti_ae_fit_shot(self.h, ae.h, save_every, nblocks), "ae_fit_shot")
 
     def ae_train_shot(self, ae, save_every=4, nsteps=2):
         loss = np.zeros(1, np.float32)
@@ -676,6 +683,14 @@ class Autoencoder:
     def decode(self, latent_ptr, scale_ptr, n, out_ptr):
         check(lib().vti_ae_decode(self.h, latent_ptr, scale_ptr, n, out_ptr), "ae_decode")
 
+    def fit_field(self, ptr, off, sy, sx, n, nblocks=4096):
+        check(lib().vti_ae_fit_field(self.h, ptr, off, sy, sx, n, nblocks), "ae_fit_field")
+
+    def fit_end(self):
+        e = C.c_double()
+        check(lib().vti_ae_fit_end(self.h, C.byref(e)), "ae_fit_end")
+        return e.value
+
     @staticmethod
     def latent_elems(n):
         return int(lib().vti_ae_latent_elems(n))
@@ -696,3 +711,14 @@ class Autoencoder:
             self.close()
         except Exception:
             pass
+
+
+def sym_eigen(a):
+    """Eigenvalues and eigenvectors (columns) of a symmetric matrix with the
+    library's host Jacobi routine (unsorted)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n = a.shape[0]
+    w = np.zeros(n)
+    v = np.zeros((n, n))
+    check(lib().vti_sym_eigen(n, a.ctypes.data_as(DP), w.ctypes.data_as(DP), v.ctypes.data_as(DP)), "sym_eigen")
+    return w, v

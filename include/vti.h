@@ -294,6 +294,18 @@ int vti_ae_encode(vti_ae *ae, const float *P, long long off, long long sy, long 
                   void *latent, float *scale_dev);
 int vti_ae_decode(vti_ae *ae, const void *latent, const float *scale_dev, int n, float *out);
 long long vti_ae_latent_elems(int n);
+/* Principal-component fit (the closed-form optimum of the linear
+ * autoencoder): nblocks random 8^3 blocks of an n^3 box of a device field,
+ * scaled by the box's max |P|, go into a Gram matrix (BF16 MFMA, FP32
+ * sums); vti_ae_fit_end sets the encoder to the projection on the 64
+ * leading components, the decoder to the reconstruction, the latent
+ * convolutions to identity; *explained = captured energy fraction. */
+int vti_ae_fit_field(vti_ae *ae, const float *P, long long off, long long sy, long long sx, int n,
+                     int nblocks);
+int vti_ae_fit_end(vti_ae *ae, double *explained);
+/* Symmetric eigen-decomposition used by the fit (cyclic Jacobi, host):
+ * a n x n row-major, w eigenvalues, v eigenvectors as columns, unsorted. */
+int vti_sym_eigen(int n, const double *a, double *w, double *v);
 
 /* 3D RTM with the source wavefield stored as autoencoder latents (cubic
  * interiors, side a multiple of 8).  The first sweep is vti_rtm_shot's
@@ -320,6 +332,9 @@ int vti_learned_defaults(vti_learned_opts *o);
  * exactly. */
 int vti_rtm_shot_learned(vti_ctx *ctx, vti_ae *ae, const vti_rtm_opts *opts,
                          const vti_learned_opts *lo, const float *obs, double *stats);
+/* Forward modelling of the current shot; every save_every-th snapshot adds
+ * nblocks sampled 8^3 blocks to the autoencoder's principal-component fit. */
+int vti_ae_fit_shot(vti_ctx *ctx, vti_ae *ae, int save_every, int nblocks);
 /* Forward modelling of the current shot with nsteps Adam steps of the
  * autoencoder on random crops of every save_every-th snapshot. */
 int vti_ae_train_shot(vti_ctx *ctx, vti_ae *ae, int save_every, int nsteps, float *mean_loss);

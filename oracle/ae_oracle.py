@@ -18,8 +18,9 @@ network of geophysics_amd/csrc/vti_nn.hip term by term:
   - down / same: W[co][k], k = tap * ci + c, tap = (dy*T + dx)*T + dz
     (T = 2 or 3, same-taps offset by -1);
   - up:          W[n][ci], n = tap * co + c, tap = (dy*2 + dx)*2 + dz;
-* activation: leaky ReLU with slope 1/8 (exact in bf16 and fp32) or none;
-  the network has no biases (positively homogeneous: zero in, zero out);
+* activation: leaky ReLU with slope 1/8 (exact in bf16 and fp32) or none
+  (the network's layers are linear; the kernels implement both); no biases
+  (homogeneous: zero in, zero out);
 * bf16 (round to nearest even) at the points the GPU stores bf16: weights,
   activations, the latent, activation gradients; fp32 master weights and
   Adam moments; the reconstruction and the loss in fp32.
@@ -33,13 +34,13 @@ LEAK = 0.125
 
 # (kind, ci, co, act) of the 4-level encoder / decoder; latent after "E4"
 NET = [
-    ("down", 1, 8, 1),     # E1  N   -> N/2
-    ("down", 8, 64, 1),    # E2  N/2 -> N/4
-    ("down", 64, 64, 1),   # E3  N/4 -> N/8
+    ("down", 1, 8, 0),     # E1  N   -> N/2
+    ("down", 8, 64, 0),    # E2  N/2 -> N/4
+    ("down", 64, 64, 0),   # E3  N/4 -> N/8
     ("same", 64, 64, 0),   # E4  latent (64 ch at N/8: 16x fewer bytes in bf16 than fp32 input)
-    ("same", 64, 64, 1),   # D4
-    ("up", 64, 64, 1),     # D3  N/8 -> N/4
-    ("up", 64, 8, 1),      # D2  N/4 -> N/2
+    ("same", 64, 64, 0),   # D4
+    ("up", 64, 64, 0),     # D3  N/8 -> N/4
+    ("up", 64, 8, 0),      # D2  N/4 -> N/2
     ("up", 8, 1, 0),       # D1  N/2 -> N
 ]
 N_ENC = 4
@@ -155,6 +156,36 @@ def conv_bwd_weight(kind, x, gpre):
         return gt.T @ x.reshape(-1, ci), db
     A = _gather(x, kind).reshape(-1, len(taps(kind)) * x.shape[-1])
     return g.reshape(-1, co).T @ A, db
+
+
+def block_order_offsets():
+    """Voxel offset (y, x, z) inside an 8^3 block of each of the 512 inputs of
+    the third encoder layer, k = (t2*8 + t1)*8 + t0, t = (dy*2 + dx)*2 + dz."""
+    out = []
+    for k in range(512):
+        t2, t1, t0 = k >> 6, (k >> 3) & 7, k & 7
+        b = lambda t, s: (t >> s) & 1
+        out.append((4 * b(t2, 2) + 2 * b(t1, 2) + b(t0, 2), 4 * b(t2, 1) + 2 * b(t1, 1) + b(t0, 1),
+                    4 * b(t2, 0) + 2 * b(t1, 0) + b(t0, 0)))
+    return out
+
+
+def pca_params(V):
+    """Weights of the network whose encoder projects every 8^3 block on the
+    rows of V (64 x 512, block order of block_order_offsets) and whose
+    decoder reconstructs from them; latent convolutions identity."""
+    p = []
+    for kind, ci, co, a in NET:
+        p.append(np.zeros(kdim(kind, ci, co), np.float32))
+    p[0][:] = np.eye(8)
+    p[1][:] = np.eye(64)
+    p[2][:] = V
+    for i in (3, 4):
+        p[i][np.arange(64), 13 * 64 + np.arange(64)] = 1.0
+    p[5][:] = V.T
+    p[6][:] = np.eye(64)
+    p[7][:] = np.eye(8)
+    return p
 
 
 def init_params(seed=0):

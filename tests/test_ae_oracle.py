@@ -133,3 +133,54 @@ def test_adam_step_closed_form():
     w1, m1, v1 = ae.adam(w, np.zeros(3), np.zeros(3), g, 1, 1e-2, eps=0.0)
     assert np.allclose(w1, w - 1e-2 * np.sign(g), rtol=0, atol=1e-7)
     assert np.allclose(m1, 0.1 * g) and np.allclose(v1, 0.001 * g * g)
+
+
+def test_pca_weights_make_the_network_a_block_projection():
+    """With pca_params(V) the oracle network maps every 8^3 block b of the
+    input to V^T V b (block order of block_order_offsets): the weight
+    layout, tap order and block ordering agree with a plain numpy
+    projection."""
+    rng = np.random.default_rng(8)
+    Q, _ = np.linalg.qr(rng.standard_normal((512, 64)))
+    V = Q.T.astype(np.float64)
+    p = pca_params(V)
+    x = rng.standard_normal((1, 16, 16, 16, 1))
+    h = x
+    for li, (kind, ci, co, a) in enumerate(ae.NET):
+        h = ae.conv_fwd(kind, h, p[li], np.zeros(co), a)
+    offs = ae.block_order_offsets()
+    ref = np.zeros_like(x)
+    for by in range(2):
+        for bx in range(2):
+            for bz in range(2):
+                blk = np.array([x[0, 8 * by + y, 8 * bx + xx, 8 * bz + z, 0] for y, xx, z in offs])
+                rec = V.T @ (V @ blk)
+                for (y, xx, z), val in zip(offs, rec):
+                    ref[0, 8 * by + y, 8 * bx + xx, 8 * bz + z, 0] = val
+    assert np.allclose(h, ref, rtol=0, atol=1e-6)      # weights are held in fp32
+    assert sorted(offs) == [(y, xx, z) for y in range(8) for xx in range(8) for z in range(8)]
+
+
+def pca_params(V):
+    return ae.pca_params(V)
+
+
+def test_library_eigen_solver_matches_numpy():
+    """The host Jacobi routine used by the fit against numpy.linalg.eigh on
+    a Gram matrix of smooth 8^3 blocks (fast-decaying spectrum) and on a
+    random symmetric matrix."""
+    from geophysics_amd import vti
+    rng = np.random.default_rng(9)
+    g = np.arange(8.0)
+    blocks = []
+    for _ in range(600):
+        k = rng.normal(size=3) * 0.6
+        Y, X, Z = np.meshgrid(g, g, g, indexing="ij")
+        blocks.append(np.cos(k[0] * Y + k[1] * X + k[2] * Z + rng.uniform(0, 6.28)).ravel())
+    B = np.array(blocks)
+    for A in (B.T @ B, (lambda M: M + M.T)(rng.standard_normal((40, 40)))):
+        w, v = vti.sym_eigen(A)
+        wr = np.linalg.eigvalsh(A)
+        assert np.allclose(np.sort(w), wr, rtol=1e-9, atol=1e-9 * np.abs(wr).max())
+        assert np.allclose(A @ v, v * w, atol=1e-8 * np.abs(wr).max())
+        assert np.allclose(v.T @ v, np.eye(len(w)), atol=1e-10)

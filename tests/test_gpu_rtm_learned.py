@@ -63,31 +63,65 @@ def test_fallback_equals_plain_rtm_bitwise(exact):
     assert np.array_equal(img_l, img_p), rel_l2(img_l, img_p)
 
 
-def test_trained_autoencoder_image_error():
-    """Online training on two shots' snapshots, then a third shot imaged
-    from latents, without and with the exact-storage fallback, against the
-    plain RTM image of the same steps."""
+def test_fitted_autoencoder_image_error():
+    """Principal-component fit on two shots' snapshots, then a third shot
+    imaged from latents, without and with the exact-storage fallback,
+    against the plain RTM image of the same steps: the raw image and the
+    final image (divided by the illumination, Laplacian-filtered).  This
+    32^3 / 25 Hz case is far harder for 16x compression than the 15 Hz
+    benchmark (more energy near the grid Nyquist)."""
     cfg, vp, e, d = _setup()
     with vti.Propagator(cfg, vp, e, d) as prop, vti.Autoencoder(crop=32, batch=2, seed=2) as A:
-        A.set_lr(1e-3)
-        losses = []
-        for rep in range(4):
-            for cx, cy in ((10, 12), (22, 20)):
-                prop.set_shot(_shot(cfg, cx, cy))
-                losses.append(prop.ae_train_shot(A, save_every=4, nsteps=6))
-        _say("mean training loss per pass:", losses)
+        for cx, cy in ((10, 12), (22, 20)):
+            prop.set_shot(_shot(cfg, cx, cy))
+            prop.ae_fit_shot(A, save_every=4, nblocks=64)
+        explained = A.fit_end()
+        _say(f"energy captured by 64 components: {explained:.6f}")
         prop.set_shot(_shot(cfg, 16, 16))
         img_p, ill_p = _plain(prop, None)
+        fin_p, _ = prop.rtm_image(finalize=True)
         res = {}
         for name, max_exact in (("latent only", 0), ("with exact fallback", NT // 8)):
             lo = vti.learned_defaults()
-            lo.check_every, lo.tol, lo.ck_every, lo.max_exact = 1, 0.05, 0, max_exact
+            lo.check_every, lo.tol, lo.ck_every, lo.max_exact = 1, 0.02, 0, max_exact
             prop.rtm_reset()
             st = prop.rtm_shot_learned(A, prop.rtm_defaults(), lo, None)
             img_l, ill_l = prop.rtm_image(finalize=False)
+            fin_l, _ = prop.rtm_image(finalize=True)
             assert np.array_equal(ill_l, ill_p)
-            res[name] = rel_l2(img_l, img_p)
-            _say(f"{name}: image rel-L2 {res[name]:.3e}; mean snapshot error {st['mean_checked_err']:.3e}; {st}")
-    assert losses[-1] < losses[0]
-    assert res["with exact fallback"] <= res["latent only"]
-    assert res["with exact fallback"] < 0.1
+            res[name] = (rel_l2(img_l, img_p), rel_l2(fin_l, fin_p), st)
+            _say(f"{name}: raw image rel-L2 {res[name][0]:.3e}, final image rel-L2 {res[name][1]:.3e}; "
+                 f"mean snapshot error {st['mean_checked_err']:.3e}; {st}")
+    assert explained > 0.99
+    assert res["latent only"][2]["mean_checked_err"] < 0.5
+    assert res["with exact fallback"][2]["exact"] == NT // 8
+    assert res["with exact fallback"][1] < 0.5
+
+def test_fit_gives_an_orthonormal_projection_network():
+    """After the fit the encoder's third layer holds orthonormal rows V and
+    the network decodes every 8^3 block to its projection V^T V b (checked
+    against the oracle network built from the same V)."""
+    import torch
+    from oracle import ae_oracle as ae
+    cfg, vp, e, d = _setup()
+    with vti.Propagator(cfg, vp, e, d) as prop, vti.Autoencoder(crop=32, batch=1, seed=3) as A:
+        prop.set_shot(_shot(cfg, 16, 16))
+        prop.ae_fit_shot(A, save_every=8, nblocks=64)
+        A.fit_end()
+        flat = A.params()
+        prop.set_shot(_shot(cfg, 12, 20))
+        prop.reset()
+        prop.step(0, 120)
+        P = prop.field("P")                      # padded (nny, nnx, nnz)
+        box = np.ascontiguousarray(P[NPML:NPML + N, NPML:NPML + N, NPML:NPML + N])
+    sizes = [int(np.prod(ae.kdim(k, ci, co))) for k, ci, co, a in ae.NET]
+    offs = np.cumsum([0] + sizes)
+    V = flat[offs[2]:offs[3]].reshape(64, 512).astype(np.float64)
+    assert np.allclose(V @ V.T, np.eye(64), atol=1e-5)
+    with vti.Autoencoder(crop=N, batch=1, seed=4) as B:
+        B.set_params(flat)
+        s = np.abs(box).max()
+        _, rec = B.train_batch((box / s)[None, ..., None], apply=False, want_recon=True)
+    ref = ae.forward([flat[offs[i]:offs[i + 1]].reshape(ae.kdim(*ae.NET[i][:3])) for i in range(len(ae.NET))],
+                     (box / s)[None, ..., None].astype(np.float32))[-1]
+    assert rel_l2(rec, ref) < 1e-2
