@@ -6,8 +6,11 @@
 // it % save_every == 0 the interior P box (n^3, n a multiple of 8) is
 // encoded into a latent slot (n^3/8 bf16 values + one scale).  Optionally
 // every check_every-th stored snapshot is decoded at once and its relative
-// L2 error against the true field measured.  A snapshot above tol takes the
-// fallback: it is kept exactly (uncompressed interior, up to max_exact of
+// L2 error against the true field measured, relative to the largest snapshot
+// energy so far.  A box of half-width src_box around the source (the
+// grid-scale near field of the point source and the pseudo-acoustic shear
+// artefact live there) is kept exactly for every snapshot and masked out of
+// what the autoencoder sees.  A snapshot above tol takes the fallback: it is kept exactly (uncompressed interior, up to max_exact of
 // them), or, once those slots are used, recomputed in the reverse sweep
 // from full-state checkpoints kept every ck_every steps (plain checkpoint
 // recomputation).  Every other imaging step decodes its latent.  The receiver side is the same as in
@@ -52,6 +55,52 @@ __global__ void interior_kernel(Grid g, int np, int n, const float *__restrict__
         const long long r = i / n;
         const int x = (int)(r % n), y = (int)(r / n);
         out[i] = P[(z + np) + (long long)(x + np) * g.ldz + (long long)(y + np) * g.ldxy];
+    }
+}
+
+// Box of the interior around the source, [y0, y0+by) x [x0, x0+bx) x [z0, z0+bz).
+struct Box {
+    int y0, x0, z0, by, bx, bz;
+    __host__ __device__ bool inside(int y, int x, int z) const
+    {
+        return y >= y0 && y < y0 + by && x >= x0 && x < x0 + bx && z >= z0 && z < z0 + bz;
+    }
+    __host__ __device__ long long size() const { return (long long)by * bx * bz; }
+};
+
+// D = the interior n^3 box of P with the source box zeroed (dense, z fastest)
+__global__ void masked_interior_kernel(Grid g, int np, int n, const float *__restrict__ P, Box b, float *D)
+{
+    const long long tot = (long long)n * n * n;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < tot; i += (long long)gridDim.x * blockDim.x) {
+        const int z = (int)(i % n);
+        const long long r = i / n;
+        const int x = (int)(r % n), y = (int)(r / n);
+        D[i] = b.inside(y, x, z) ? 0.0f : P[(z + np) + (long long)(x + np) * g.ldz + (long long)(y + np) * g.ldxy];
+    }
+}
+
+// out = P over the source box (exact copy kept beside the latent)
+__global__ void box_save_kernel(Grid g, int np, const float *__restrict__ P, Box b, float *out)
+{
+    const long long tot = b.size();
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < tot; i += (long long)gridDim.x * blockDim.x) {
+        const int z = (int)(i % b.bz);
+        const long long r = i / b.bz;
+        const int x = (int)(r % b.bx), y = (int)(r / b.bx);
+        out[i] = P[(b.z0 + z + np) + (long long)(b.x0 + x + np) * g.ldz + (long long)(b.y0 + y + np) * g.ldxy];
+    }
+}
+
+// D (dense n^3) over the source box = the kept exact values
+__global__ void box_restore_kernel(int n, Box b, const float *__restrict__ in, float *D)
+{
+    const long long tot = b.size();
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < tot; i += (long long)gridDim.x * blockDim.x) {
+        const int z = (int)(i % b.bz);
+        const long long r = i / b.bz;
+        const int x = (int)(r % b.bx), y = (int)(r / b.bx);
+        D[(b.z0 + z) + (long long)(b.x0 + x) * n + (long long)(b.y0 + y) * n * n] = in[i];
     }
 }
 
@@ -110,6 +159,7 @@ int vti_learned_defaults(vti_learned_opts *o)
     o->tol = 0.05f;
     o->ck_every = 0;
     o->max_exact = 0;
+    o->src_box = 8;
     return VTI_OK;
 }
 
@@ -123,7 +173,7 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
     if (k.ndim != 3 || k.ny != n || k.nz != n || n % 8)
         return fail(VTI_ERR_ARG, "vti_rtm_shot_learned: 3D cubic interiors with a side divisible by 8 only");
     if (c->nrec <= 0) return fail(VTI_ERR_STATE, "vti_rtm_shot_learned: the shot has no receivers");
-    if (lo->save_every < 1 || lo->check_every < 0 || lo->ck_every < 0 || lo->max_exact < 0)
+    if (lo->save_every < 1 || lo->check_every < 0 || lo->ck_every < 0 || lo->max_exact < 0 || lo->src_box < 0)
         return fail(VTI_ERR_ARG, "vti_rtm_shot_learned: bad options");
     int st;
     if (!c->img && (st = vti_rtm_reset(c))) return st;
@@ -134,7 +184,7 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
     const int nsave = (nt + se - 1) / se;
     const long long off = np + (long long)np * c->g.ldz + (long long)np * c->g.ldxy;
     hipStream_t s = c->stream;
-    Buf lat, scl, dense, acc, exa;
+    Buf lat, scl, dense, dense2, acc, exa, boxes;
     if (hipMalloc(&lat.p, (size_t)nsave * lat_n * sizeof(uint16_t)) != hipSuccess ||
         hipMalloc(&scl.p, (size_t)nsave * 64) != hipSuccess ||
         hipMalloc(&dense.p, (size_t)n * n * n * sizeof(float)) != hipSuccess ||
@@ -148,6 +198,26 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
     if (lo->max_exact > 0 && hipMalloc(&exa.p, (size_t)lo->max_exact * nn3 * sizeof(float)) != hipSuccess)
         return fail(VTI_ERR_ALLOC, "vti_rtm_shot_learned: exact fallback store");
     float *X = static_cast<float *>(exa.p);
+    // exact box around the source (interior coordinates), kept for every snapshot
+    Box bx{0, 0, 0, 0, 0, 0};
+    if (lo->src_box > 0) {
+        const vti_shot &sh = c->shot;
+        auto span = [&](int centre, int &lo0, int &len) {
+            lo0 = std::max(0, centre - lo->src_box);
+            len = std::min(n, centre + lo->src_box) - lo0;
+        };
+        span(sh.sy - np, bx.y0, bx.by);
+        span(sh.sx - np, bx.x0, bx.bx);
+        span(sh.sz - np, bx.z0, bx.bz);
+        if (bx.by <= 0 || bx.bx <= 0 || bx.bz <= 0) bx = Box{0, 0, 0, 0, 0, 0};
+    }
+    const long long box_n = bx.size();
+    if ((box_n && hipMalloc(&boxes.p, (size_t)nsave * box_n * sizeof(float)) != hipSuccess) ||
+        hipMalloc(&dense2.p, nn3 * sizeof(float)) != hipSuccess)
+        return fail(VTI_ERR_ALLOC, "vti_rtm_shot_learned: source boxes");
+    float *BX = static_cast<float *>(boxes.p);
+    float *D2 = static_cast<float *>(dense2.p);
+    double emax = 0.0;   // largest snapshot energy so far: the fallback's reference
     std::vector<int> exact(nsave, -1);
     int nexact = 0;
     // fallback checkpoints
@@ -187,21 +257,32 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         const int j = it / se;
         const float *P = W.f[VTI_F_P];
         const double t0 = now_ms();
-        if ((st = ae_encode(A, P, off, c->g.ldxy, c->g.ldz, n, L + (size_t)j * lat_n, Sc + (size_t)j * 16, s)))
-            return st;
+        if (box_n) {   // the source box is kept exactly; the autoencoder sees the rest
+            hipLaunchKernelGGL(box_save_kernel, dim3(64), dim3(256), 0, s, c->g, np, P, bx, BX + (size_t)j * box_n);
+            hipLaunchKernelGGL(masked_interior_kernel, dim3(2048), dim3(256), 0, s, c->g, np, n, P, bx, D);
+            st = ae_encode(A, D, 0, (long long)n * n, n, n, L + (size_t)j * lat_n, Sc + (size_t)j * 16, s);
+        } else {
+            st = ae_encode(A, P, off, c->g.ldxy, c->g.ldz, n, L + (size_t)j * lat_n, Sc + (size_t)j * 16, s);
+        }
+        if (st) return st;
         t_enc += now_ms() - t0;
         if (lo->check_every > 0 && j % lo->check_every == 0) {
-            if ((st = ae_decode(A, L + (size_t)j * lat_n, Sc + (size_t)j * 16, n, D, s))) return st;
+            if ((st = ae_decode(A, L + (size_t)j * lat_n, Sc + (size_t)j * 16, n, D2, s))) return st;
+            if (box_n)
+                hipLaunchKernelGGL(box_restore_kernel, dim3(64), dim3(256), 0, s, n, bx, BX + (size_t)j * box_n, D2);
             if ((st = hip(nn::launch_fill_f32(reinterpret_cast<float *>(E), 4, 0.0f, s), "fill"))) return st;
-            hipLaunchKernelGGL(err_kernel, dim3(1024), dim3(256), 0, s, c->g, np, n, D, P, E);
+            hipLaunchKernelGGL(err_kernel, dim3(1024), dim3(256), 0, s, c->g, np, n, D2, P, E);
             double h[2];
             if (hipMemcpyAsync(h, E, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
                 return fail(VTI_ERR_HIP, "error check");
+            emax = std::max(emax, h[1]);
             if (h[1] > 0.0) {
                 const double e = std::sqrt(h[0] / h[1]);
                 err_sum += e;
                 ++nchecked;
-                if (e > lo->tol) {
+                // the error against the largest snapshot energy so far: weak
+                // snapshots weigh little in the image whatever their relative error
+                if (std::sqrt(h[0] / emax) > lo->tol) {
                     ++nbad;
                     if (nexact < lo->max_exact && lo->check_every == 1) {   // keep this snapshot exactly
                         hipLaunchKernelGGL(interior_kernel, dim3(2048), dim3(256), 0, s, c->g, np, n, P, X + (size_t)nexact * nn3);
@@ -249,6 +330,8 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         } else {
             const double t0 = now_ms();
             if ((st = ae_decode(A, L + (size_t)j * lat_n, Sc + (size_t)j * 16, n, D, s))) return st;
+            if (box_n)
+                hipLaunchKernelGGL(box_restore_kernel, dim3(64), dim3(256), 0, s, n, bx, BX + (size_t)j * box_n, D);
             t_dec += now_ms() - t0;
             hipLaunchKernelGGL(image_dense_kernel, grd, blk, 0, s, c->g, np, k.nx, k.ny, k.nz, D, G.f[VTI_F_P], c->img);
         }
@@ -268,17 +351,36 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
 }
 
 // Forward modelling of the current shot (no record kept); every
-// save_every-th snapshot contributes nblocks sampled 8^3 blocks to the
-// autoencoder's principal-component fit (finished by vti_ae_fit_end).
-int vti_ae_fit_shot(vti_ctx *c, vti_ae *A, int save_every, int nblocks)
+// save_every-th snapshot, the source box of half-width src_box masked out,
+// contributes nblocks sampled 8^3 blocks to the autoencoder's
+// principal-component fit (finished by vti_ae_fit_end).
+int vti_ae_fit_shot(vti_ctx *c, vti_ae *A, int save_every, int nblocks, int src_box)
 {
     if (!c || !A || save_every < 1 || nblocks < 1) return fail(VTI_ERR_ARG, "vti_ae_fit_shot: bad arguments");
     if (!c->have_model || !c->have_shot) return fail(VTI_ERR_STATE, "vti_ae_fit_shot: model and shot first");
     const vti_cfg &k = c->cfg;
     if (k.ndim != 3 || k.ny != k.nx || k.nz != k.nx || k.nx % 8)
         return fail(VTI_ERR_ARG, "vti_ae_fit_shot: 3D cubic interiors with a side divisible by 8 only");
-    const int np = k.npml;
+    if (src_box < 0) return fail(VTI_ERR_ARG, "vti_ae_fit_shot: src_box >= 0");
+    const int np = k.npml, n = k.nx;
     const long long off = np + (long long)np * c->g.ldz + (long long)np * c->g.ldxy;
+    // the source box is masked out, as in vti_rtm_shot_learned
+    Box bx{0, 0, 0, 0, 0, 0};
+    if (src_box > 0) {
+        const vti_shot &sh = c->shot;
+        auto span = [&](int centre, int &lo0, int &len) {
+            lo0 = std::max(0, centre - src_box);
+            len = std::min(n, centre + src_box) - lo0;
+        };
+        span(sh.sy - np, bx.y0, bx.by);
+        span(sh.sx - np, bx.x0, bx.bx);
+        span(sh.sz - np, bx.z0, bx.bz);
+        if (bx.by <= 0 || bx.bx <= 0 || bx.bz <= 0) bx = Box{0, 0, 0, 0, 0, 0};
+    }
+    Buf dense;
+    if (bx.size() && hipMalloc(&dense.p, (size_t)n * n * n * sizeof(float)) != hipSuccess)
+        return fail(VTI_ERR_ALLOC, "vti_ae_fit_shot: masked field");
+    float *D = static_cast<float *>(dense.p);
     const VtiState W = main_state(c);
     if (state_zero(c, W) != hipSuccess) return fail(VTI_ERR_HIP, "zero");
     for (int it = 0; it < k.nt; ++it) {
@@ -286,8 +388,11 @@ int vti_ae_fit_shot(vti_ctx *c, vti_ae *A, int save_every, int nblocks)
             launch_stress(3, stress_args_for(c, W, it, true, k.sv != 0, nullptr), c->stream) != hipSuccess)
             return fail(VTI_ERR_HIP, "step");
         if (it % save_every) continue;
+        if (bx.size())
+            hipLaunchKernelGGL(masked_interior_kernel, dim3(2048), dim3(256), 0, c->stream, c->g, np, n, W.f[VTI_F_P], bx, D);
         if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(VTI_ERR_HIP, "sync");   // the fit runs on its own stream
-        const int st = ae_fit_field(A, W.f[VTI_F_P], off, c->g.ldxy, c->g.ldz, k.nx, nblocks);
+        const int st = bx.size() ? ae_fit_field(A, D, 0, (long long)n * n, n, n, nblocks)
+                                 : ae_fit_field(A, W.f[VTI_F_P], off, c->g.ldxy, c->g.ldz, n, nblocks);
         if (st) return st;
     }
     return hipStreamSynchronize(c->stream) == hipSuccess ? VTI_OK : fail(VTI_ERR_HIP, "sync");

@@ -60,7 +60,7 @@ class RtmOpts(C.Structure):
 _lib = None
 class LearnedOpts(C.Structure):
     _fields_ = [("save_every", C.c_int), ("check_every", C.c_int), ("tol", C.c_float), ("ck_every", C.c_int),
-                ("max_exact", C.c_int)]
+                ("max_exact", C.c_int), ("src_box", C.c_int)]
 
 
 FP = C.POINTER(C.c_float)
@@ -136,7 +136,7 @@ def _declare(L):
         "vti_ae_fit_field": (C.c_int, [VP, VP, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_int]),
         "vti_ae_fit_end": (C.c_int, [VP, DP]),
         "vti_sym_eigen": (C.c_int, [C.c_int, DP, DP, DP]),
-        "vti_ae_fit_shot": (C.c_int, [VP, VP, C.c_int, C.c_int]),
+        "vti_ae_fit_shot": (C.c_int, [VP, VP, C.c_int, C.c_int, C.c_int]),
         "vti_learned_defaults": (C.c_int, [C.POINTER(LearnedOpts)]),
         "vti_rtm_shot_learned": (C.c_int, [VP, VP, C.POINTER(RtmOpts), C.POINTER(LearnedOpts), FP, DP]),
         "vti_ae_train_shot": (C.c_int, [VP, VP, C.c_int, C.c_int, FP]),
@@ -394,8 +394,8 @@ class Propagator:
         return {"encode_ms": st[0], "decode_ms": st[1], "mean_checked_err": st[2], "over_tol": int(st[3]),
                 "snapshots": int(st[4]), "exact": int(st[5])}
 
-    def ae_fit_shot(self, ae, save_every=4, nblocks=4096):
-        check(lib().vti_ae_fit_shot(self.h, ae.h, save_every, nblocks), "ae_fit_shot")
+    def ae_fit_shot(self, ae, save_every=4, nblocks=4096, src_box=8):
+        check(lib().vti_ae_fit_shot(self.h, ae.h, save_every, nblocks, src_box), "ae_fit_shot")
 
     def ae_train_shot(self, ae, save_every=4, nsteps=2):
         loss = np.zeros(1, np.float32)

@@ -313,18 +313,22 @@ int vti_sym_eigen(int n, const double *a, double *w, double *v);
  * interior P is encoded.  The reverse sweep images those steps with the
  * decoded field (vti_rtm_shot with image_every = save_every images the same
  * steps with the exact field).  Fallback: every check_every-th snapshot is
- * decoded at once and compared with the true field; when its relative L2
- * error exceeds tol it is kept exactly (uncompressed interior; with
+ * decoded at once and compared with the true field; when its L2 error
+ * relative to the largest snapshot energy so far exceeds tol it is kept
+ * exactly (uncompressed interior; with
  * check_every == 1, up to max_exact snapshots) or, past that, it and the
  * next check_every - 1 are recomputed exactly from full-state checkpoints
  * kept every ck_every steps (plain checkpoint recomputation; ck_every = 0:
- * no checkpoints, the latent is used). */
+ * no checkpoints, the latent is used).  A box of half-width src_box cells
+ * around the source is kept exactly in every snapshot and masked out of the
+ * encoder's input. */
 typedef struct {
     int save_every;    /* 4 */
     int check_every;   /* 8; 0: no checks */
     float tol;         /* 0.05 */
     int ck_every;      /* 0: no fallback checkpoints */
     int max_exact;     /* 0: no exactly kept snapshots */
+    int src_box;       /* 8: half-width of the exactly kept box around the source (0: none) */
 } vti_learned_opts;
 int vti_learned_defaults(vti_learned_opts *o);
 /* stats (optional, 6 doubles): encode ms, decode ms (host-timed), mean
@@ -332,9 +336,11 @@ int vti_learned_defaults(vti_learned_opts *o);
  * exactly. */
 int vti_rtm_shot_learned(vti_ctx *ctx, vti_ae *ae, const vti_rtm_opts *opts,
                          const vti_learned_opts *lo, const float *obs, double *stats);
-/* Forward modelling of the current shot; every save_every-th snapshot adds
- * nblocks sampled 8^3 blocks to the autoencoder's principal-component fit. */
-int vti_ae_fit_shot(vti_ctx *ctx, vti_ae *ae, int save_every, int nblocks);
+/* Forward modelling of the current shot; every save_every-th snapshot, the
+ * box of half-width src_box around the source masked out (as
+ * vti_rtm_shot_learned does), adds nblocks sampled 8^3 blocks to the
+ * autoencoder's principal-component fit. */
+int vti_ae_fit_shot(vti_ctx *ctx, vti_ae *ae, int save_every, int nblocks, int src_box);
 /* Forward modelling of the current shot with nsteps Adam steps of the
  * autoencoder on random crops of every save_every-th snapshot. */
 int vti_ae_train_shot(vti_ctx *ctx, vti_ae *ae, int save_every, int nsteps, float *mean_loss);

@@ -92,10 +92,12 @@ def test_fitted_autoencoder_image_error():
             res[name] = (rel_l2(img_l, img_p), rel_l2(fin_l, fin_p), st)
             _say(f"{name}: raw image rel-L2 {res[name][0]:.3e}, final image rel-L2 {res[name][1]:.3e}; "
                  f"mean snapshot error {st['mean_checked_err']:.3e}; {st}")
-    assert explained > 0.99
-    assert res["latent only"][2]["mean_checked_err"] < 0.5
-    assert res["with exact fallback"][2]["exact"] == NT // 8
-    assert res["with exact fallback"][1] < 0.5
+    assert explained > 0.98
+    assert res["latent only"][2]["mean_checked_err"] < 0.2
+    assert 0 < res["with exact fallback"][2]["exact"] <= NT // 8
+    assert res["with exact fallback"][0] < res["latent only"][0]
+    assert res["with exact fallback"][0] < 5e-3          # raw image (measured 8.1e-4)
+    assert res["with exact fallback"][1] < 2e-2          # final image (measured 7.8e-3)
 
 def test_fit_gives_an_orthonormal_projection_network():
     """After the fit the encoder's third layer holds orthonormal rows V and
