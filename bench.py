@@ -304,7 +304,7 @@ def bench_rtm3d_learned(args, torch, vti):
     pos = [(int(rng.integers(n // 4, 3 * n // 4)), int(rng.integers(n // 4, 3 * n // 4))) for _ in range(nshot)]
     A = vti.Autoencoder(crop=64, batch=4, seed=args.seed)
     nsave = (nt + 3) // 4
-    max_exact = max(1, nsave // 10)               # exact-storage fallback budget: 10 % of the snapshots
+    max_exact = max(1, int(nsave * args.exact_frac))   # exact-storage fallback budget
     lat_bytes = nsave * vti.Autoencoder.latent_elems(n) * 2
     exact_bytes = max_exact * n ** 3 * 4
     # plain path: checkpoint slots in the same memory as latents + fallback store
@@ -370,6 +370,8 @@ def bench_rtm3d_learned(args, torch, vti):
                    "latent_store_GB": round(lat_bytes / 1e9, 3), "exact_store_GB": round(exact_bytes / 1e9, 3),
                    "fit_s": round(t_fit, 2), "fit_energy_captured": explained,
                    "exact_snapshots_mean": float(np.mean([r[4]["exact"] for r in rows])) if rows else None,
+                   "exact_budget": max_exact, "snapshots": nsave,
+                   "compression_of_stored_snapshots": round(nsave * n ** 3 * 4 / (lat_bytes + exact_bytes), 2),
                    "encode_ms_per_shot": round(float(np.mean([r[4]["encode_ms"] for r in rows])), 1) if rows else None,
                    "decode_ms_per_shot": round(float(np.mean([r[4]["decode_ms"] for r in rows])), 1) if rows else None},
         "roofline": None, "cpu_baseline": None,
@@ -394,6 +396,10 @@ def main():
     ap.add_argument("--shots", type=int, default=0, help="shots of the batch / FWI / learned-RTM workloads "
                     "(0: the workload's own count)")
     ap.add_argument("--nt", type=int, default=0, help="time steps of the non-default workloads (0: their own)")
+    ap.add_argument("--fused", action="store_true",
+                    help="fd3d: one fused kernel per step instead of the velocity and stress kernels")
+    ap.add_argument("--exact-frac", type=float, default=0.25,
+                    help="learned RTM: fraction of snapshots the exact-storage fallback may keep")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -422,6 +428,7 @@ def main():
 
     cfg = vti.cfg_ref3d(n, n, n, 10, 10, 10, npml, 1, 15, nt, 800)
     prop = vti.Propagator(cfg)
+    prop.set_fused(args.fused)
     prop.set_model(vp.data_ptr(), eps.data_ptr(), dl.data_ptr(), on_device=True)
     c = n // 2
     prop.set_shot(vti.shot_ref3d(cfg, 0, c, 0, c, 0, 1, 0, rec_depth=3, rec_step=2))
@@ -456,9 +463,17 @@ def main():
     gcells = cells * args.steps / secs / 1e9
     str_avg = str_ms / max(nl, 1) / 1e3
     vel_avg = vel_ms / max(nl, 1) / 1e3
-    achieved = BYTES_STRESS * cells / str_avg / 1e9
+    fused = args.fused
+    if fused:
+        # one kernel per step: the step's algorithmic bytes over its duration
+        kernel = "vti::f3d::fused3d<32, true>"
+        achieved = BYTES_STEP * cells / str_avg / 1e9
+        tfile = os.path.join(ROOT, "profiles", "fused3d_traffic.json")
+    else:
+        kernel = "vti::s3d::stress3d<16, true>"
+        achieved = BYTES_STRESS * cells / str_avg / 1e9
+        tfile = os.path.join(ROOT, "profiles", "stress3d_traffic.json")
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "stress3d_traffic.json")
     if os.path.exists(tfile) and n == 512:
         try:
             traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
@@ -485,10 +500,13 @@ def main():
             "grid": [n, n, n], "padded": [n + 2 * npml] * 3, "steps_per_shot": 2000,
             "cells_updated_per_step": cells, "bytes_per_cell_step_algorithmic": BYTES_STEP,
             "pct_hbm_peak_step": round(100.0 * BYTES_STEP * gcells / HBM_PEAK_GBS, 2),
-            "vel_kernel_ms": round(vel_avg * 1e3, 4), "stress_kernel_ms": round(str_avg * 1e3, 4),
+            "fused_step": fused,
+            "vel_kernel_ms": None if fused else round(vel_avg * 1e3, 4),
+            "stress_kernel_ms": None if fused else round(str_avg * 1e3, 4),
+            "fused_kernel_ms": round(str_avg * 1e3, 4) if fused else None,
         },
         "roofline": {
-            "bound": "hbm", "kernel": "vti::s3d::stress3d<16, true>",
+            "bound": "hbm", "kernel": kernel,
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
         },

@@ -249,7 +249,6 @@ int train_step(vti_ae *A, int apply, float *loss_out)
     AE_HIP(launch_fill_f32(A->grad, (long long)A->nparam, 0.0f, A->stream));
     AE_HIP(launch_mse(A->out, A->target, nin, A->dev_loss, A->gbuf[kNL - 1], A->stream));
     for (int i = kNL - 1; i >= 0; --i) {
-        const Layer &l = A->L[i];
         const int s = side_in(i, n);
         AE_HIP(launch_wgrad(wgrad_for(A, i, nb, s, A->act[i], A->gbuf[i]), A->stream));
         if (i > 0) {

@@ -107,6 +107,7 @@ hipError_t state_copy(vti_ctx *c, const VtiState &dst, const VtiState &src)
 
 VtiState main_state(vti_ctx *c)
 {
+    c->alt_valid = false;   // callers may change the fields outside vti_step
     VtiState s;
     for (int k = 0; k < 11; ++k) s.f[k] = c->f[k];
     return s;
@@ -348,6 +349,9 @@ int vti_destroy(vti_ctx *c)
     hipFree(c->dtap);
     hipFree(c->rec);
     hipFree(c->rec_t);
+    hipFree(c->alt);
+    hipFree(c->img);
+    hipFree(c->illum);
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
     delete c;
     return VTI_OK;
@@ -549,6 +553,7 @@ int vti_reset(vti_ctx *c)
     if (!c) return fail(VTI_ERR_ARG, "vti_reset: null ctx");
     for (int k = 0; k < 11; ++k)
         if (c->f[k]) HIP_TRY(launch_zero(c->f[k], c->nvol, c->stream));
+    c->alt_valid = false;
     if (c->rec) HIP_TRY(launch_zero(c->rec, (size_t)c->nrec * c->cfg.nt, c->stream));
     return VTI_OK;
 }
@@ -582,6 +587,42 @@ int vti_step(vti_ctx *c, int it0, int nsteps)
     int st = check_step(c, it0);
     if (st) return st;
     if (nsteps < 0 || it0 + nsteps > c->cfg.nt) return fail(VTI_ERR_ARG, "vti_step: range past nt");
+    if (c->cfg.ndim == 3 && c->fused && nsteps > 0) {
+        // fused steps: u, v, w, P, Q alternate between c->f and the second
+        // buffers; the cells outside the updated region are read by the
+        // stencils but never written, so the second buffers are refreshed
+        // from c->f whenever something other than a fused step may have
+        // changed the fields
+        static const int kAlt[5] = {VTI_F_U, VTI_F_V, VTI_F_W, VTI_F_P, VTI_F_Q};
+        if (!c->alt) {
+            if (hipMalloc(&c->alt, 5 * c->nvol * sizeof(float)) != hipSuccess)
+                return fail(VTI_ERR_ALLOC, "vti_step: second buffers of the fused step");
+            for (int k = 0; k < 5; ++k) c->altf[kAlt[k]] = c->alt + (size_t)k * c->nvol;
+        }
+        if (!c->alt_valid)
+            for (int k : kAlt)
+                HIP_TRY(hipMemcpyAsync(c->altf[k], c->f[k], c->nvol * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        for (int it = it0; it < it0 + nsteps; ++it) {
+            hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+            if (c->prof) {
+                e0 = next_event(c); e1 = next_event(c); e2 = next_event(c);
+                if (!e0 || !e1 || !e2) return fail(VTI_ERR_HIP, "vti_step: event creation");
+                HIP_TRY(hipEventRecord(e0, c->stream));
+                HIP_TRY(hipEventRecord(e1, c->stream));   // no separate velocity kernel
+            }
+            FusedArgs fa{};
+            fa.v = vel_args(c, it);
+            fa.u0 = c->f[VTI_F_U]; fa.v0 = c->f[VTI_F_V]; fa.w0 = c->f[VTI_F_W];
+            fa.v.u = c->altf[VTI_F_U]; fa.v.v = c->altf[VTI_F_V]; fa.v.w = c->altf[VTI_F_W];
+            fa.s = stress_args(c, it);
+            fa.s.P = c->altf[VTI_F_P]; fa.s.Q = c->altf[VTI_F_Q];
+            HIP_TRY(launch_fused(fa, c->stream));
+            if (c->prof) HIP_TRY(hipEventRecord(e2, c->stream));
+            for (int k : kAlt) std::swap(c->f[k], c->altf[k]);
+        }
+        c->alt_valid = true;   // both sets hold the same cells outside the updated region
+        return VTI_OK;
+    }
     for (int it = it0; it < it0 + nsteps; ++it) {
         hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
         if (c->prof) {
@@ -594,6 +635,13 @@ int vti_step(vti_ctx *c, int it0, int nsteps)
         HIP_TRY(launch_stress(c->cfg.ndim, stress_args(c, it), c->stream));
         if (c->prof) HIP_TRY(hipEventRecord(e2, c->stream));
     }
+    return VTI_OK;
+}
+
+int vti_set_fused(vti_ctx *c, int on)
+{
+    if (!c) return fail(VTI_ERR_ARG, "vti_set_fused: null ctx");
+    c->fused = on != 0;
     return VTI_OK;
 }
 
@@ -693,6 +741,7 @@ int vti_set_field(vti_ctx *c, int which, const float *src)
     if (!c || !src) return fail(VTI_ERR_ARG, "vti_set_field: null argument");
     float *p = field_ptr(c, which);
     if (!p) return fail(VTI_ERR_ARG, "vti_set_field: no such field");
+    c->alt_valid = false;
     const Grid &g = c->g;
     HIP_TRY(hipMemcpy2DAsync(p, g.ldz * sizeof(float), src, g.nnz * sizeof(float),
                              g.nnz * sizeof(float), (size_t)g.nnx * g.nny, hipMemcpyHostToDevice, c->stream));

@@ -57,7 +57,17 @@ struct StressArgs {
     int rec_kind, rz, rx0, rdx, rnx, ry0, rdy, rny;
 };
 
+// One fused 3D step (vti_fused3d.h): velocity update into new u, v, w from
+// the previous P, Q (v.P, v.Q) and velocities (u0, v0, w0), then the stress
+// update into new P, Q (s.P, s.Q), split fields in place.
+struct FusedArgs {
+    VelArgs v;
+    const float *u0, *v0, *w0;
+    StressArgs s;
+};
+
 // ---- kernel launchers (vti_kernels.hip) ---------------------------------
+hipError_t launch_fused(const FusedArgs &a, hipStream_t s);
 hipError_t launch_vel(int ndim, const VelArgs &a, hipStream_t s);
 hipError_t launch_stress(int ndim, const StressArgs &a, hipStream_t s);
 hipError_t launch_pad_model(const Grid &g, int ndim, int nx, int ny, int nz,
@@ -127,6 +137,13 @@ struct vti_ctx {
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
+
+    // second buffers of u, v, w, P, Q for the fused 3D step (vti_step), which
+    // reads one set and writes the other; the pointers in f swap each step
+    float *alt = nullptr;
+    float *altf[11] = {};
+    bool fused = false;
+    bool alt_valid = false;   // altf equals f outside the updated region
 
     // imaging (RTM): interior-sized image and illumination, accumulated over shots
     float *img = nullptr, *illum = nullptr;

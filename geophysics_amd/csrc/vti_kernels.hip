@@ -21,6 +21,7 @@
 
 #include "vti_internal.h"
 #include "vti_stencil3d.h"
+#include "vti_fused3d.h"
 
 namespace vti {
 namespace {
@@ -239,6 +240,18 @@ hipError_t launch_vel(int ndim, const VelArgs &a, hipStream_t s)
         const int grd = s3d::grid3d(a.g, kYChunk, nzt, nt, ntp);
         hipLaunchKernelGGL((s3d::vel3d<kYChunk, true>), dim3(grd), dim3(s3d::TZ, s3d::TX), 0, s, a, nzt, nt, ntp);
     }
+    return hipGetLastError();
+}
+
+// Fused 3D step: y chunks of kFusedChunk planes (the prologue recomputes 7
+// planes of v per chunk).
+constexpr int kFusedChunk = 32;
+
+hipError_t launch_fused(const FusedArgs &a, hipStream_t s)
+{
+    int nzt, nt, ntp;
+    const int grd = f3d::grid(a.v.g, kFusedChunk, nzt, nt, ntp);
+    hipLaunchKernelGGL((f3d::fused3d<kFusedChunk, true>), dim3(grd), dim3(f3d::TZ, f3d::TX), 0, s, a, nzt, nt, ntp);
     return hipGetLastError();
 }
 

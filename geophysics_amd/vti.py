@@ -98,6 +98,7 @@ def _declare(L):
         "vti_get_table": (C.c_int, [VP, C.c_int, FP]),
         "vti_padded_dims": (C.c_int, [VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "vti_prof_enable": (C.c_int, [VP, C.c_int]),
+        "vti_set_fused": (C.c_int, [VP, C.c_int]),
         "vti_prof_read": (C.c_int, [VP, DP, DP, C.POINTER(C.c_int)]),
         "vti_calib_copy": (C.c_int, [VP, VP, C.c_size_t, VP]),
         "vti_sync": (C.c_int, [VP]),
@@ -401,6 +402,10 @@ class Propagator:
         loss = np.zeros(1, np.float32)
         check(lib().vti_ae_train_shot(self.h, ae.h, save_every, nsteps, loss.ctypes.data_as(FP)), "ae_train_shot")
         return float(loss[0])
+
+    def set_fused(self, on=True):
+        """3D: one fused kernel per step, or the separate velocity and stress kernels (default)."""
+        check(lib().vti_set_fused(self.h, 1 if on else 0), "set_fused")
 
     def prof_enable(self, on=True):
         check(lib().vti_prof_enable(self.h, 1 if on else 0), "prof_enable")

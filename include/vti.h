@@ -373,9 +373,19 @@ int vti_grad_run(vti_grad *g, const double *dobs, const double *resid, int nsnap
                  double *misfit, double *grad_vp, double *grad_src, double *d_out,
                  long long *fwd_steps);
 
+/* 3D only: vti_step runs each step as one fused kernel (on = 1) or as the
+ * velocity and stress kernels of vti_update_vel / vti_update_stress (on =
+ * 0, the default and the faster one on MI355X: 4.47 against 7.3 ms per
+ * step at 572^3); the results are bit-identical.  The fused step keeps
+ * second copies of u, v, w, P, Q (5 more fields) and swaps them with the
+ * context's fields every step.  No counterpart in the reference
+ * (GPU_vti3dfd_kernels.cu:515-518 launches the two kernels). */
+int vti_set_fused(vti_ctx *ctx, int on);
+
 /* ---- measurement ------------------------------------------------------- */
 /* When on, vti_step brackets every velocity and stress launch with HIP
- * events on the context stream; vti_prof_read sums them (ms). */
+ * events on the context stream; vti_prof_read sums them (ms).  A fused 3D
+ * step counts entirely as stress time. */
 int vti_prof_enable(vti_ctx *ctx, int on);
 int vti_prof_read(vti_ctx *ctx, double *vel_ms, double *stress_ms, int *nlaunch);
 /* Plain device copy of nbytes (bandwidth calibration kernel). */
