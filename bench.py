@@ -285,14 +285,19 @@ def bench_rtm3d_learned(args, torch, vti):
     """3D RTM with learned snapshot compression against plain checkpoint
     recomputation: 384^3 grid, dx 10 m, 4000 steps, dt 0.8 ms, smoothed
     random vp / eps / delta as the headline workload; snapshots every 4
-    steps through the autoencoder, trained online on the first 3 shots'
-    snapshots, applied to the following shots; per-shot wall time and image
-    error against the plain path imaging the same steps."""
+    steps, compressed with the block basis fitted online on the first 3
+    shots' snapshots, applied to the following shots; per-shot wall time and
+    image error against the plain path imaging the same steps, the plain
+    path given as many checkpoint slots as the compressed store's bytes.
+    codec "sparse": error-bounded sparse coefficients (every snapshot within
+    relative L2 error --tol); codec "latent": the autoencoder's 64-channel
+    latents with the exact-storage fallback (--exact-frac)."""
     n = args.size if args.size != 512 else 384
     nt = args.nt or 4000
     nshot = args.shots or 19
     npml = args.npml
-    log(f"[bench] learned RTM: {n}^3, nt {nt}, {nshot} shots")
+    sparse = args.codec == "sparse"
+    log(f"[bench] learned RTM ({args.codec} codec): {n}^3, nt {nt}, {nshot} shots")
     vp, eps, dl = make_model(n, args.seed, torch.device("cuda:0"))
     torch.cuda.synchronize()
     cfg = vti.cfg_ref3d(n, n, n, 10, 10, 10, npml, 1, 15, nt, 800)
@@ -304,12 +309,11 @@ def bench_rtm3d_learned(args, torch, vti):
     pos = [(int(rng.integers(n // 4, 3 * n // 4)), int(rng.integers(n // 4, 3 * n // 4))) for _ in range(nshot)]
     A = vti.Autoencoder(crop=64, batch=4, seed=args.seed)
     nsave = (nt + 3) // 4
-    max_exact = max(1, int(nsave * args.exact_frac))   # exact-storage fallback budget
+    max_exact = max(1, int(nsave * args.exact_frac))   # exact-storage fallback budget (latent codec)
     lat_bytes = nsave * vti.Autoencoder.latent_elems(n) * 2
     exact_bytes = max_exact * n ** 3 * 4
-    # plain path: checkpoint slots in the same memory as latents + fallback store
+    # one checkpoint slot of the plain path: the 11 state fields of the padded grid
     state_bytes = 11 * 4 * (n + 2 * npml) ** 2 * ((n + 2 * npml + 31) // 32 * 32)
-    nsnap_equal = max(1, int((lat_bytes + exact_bytes) // state_bytes))
     rows = []
     t_fit = 0.0
     ntrain = min(3, nshot)
@@ -324,6 +328,21 @@ def bench_rtm3d_learned(args, torch, vti):
             t_fit += time.perf_counter() - t0
             log(f"[bench] fit shot {k} done ({time.perf_counter() - t0:.1f} s)")
             continue
+        lo = vti.learned_defaults()
+        if sparse:
+            # store sized for 2x compression; only the bytes used count
+            lo.codec, lo.tol, lo.floor, lo.budget = 1, args.tol, args.floor, nsave * n ** 3 * 2
+        else:
+            lo.check_every, lo.tol, lo.ck_every, lo.max_exact = 1, 0.02, 0, max_exact
+        prop.rtm_reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = prop.rtm_shot_learned(A, prop.rtm_defaults(), lo, None)
+        t_learn = time.perf_counter() - t0
+        img_l, _ = prop.rtm_image(finalize=False)
+        fin_l, _ = prop.rtm_image(finalize=True)
+        store = st["store_bytes"]
+        nsnap_equal = max(1, int(store // state_bytes))
         o = prop.rtm_defaults()
         o.image_every = 4
         o.nsnap = nsnap_equal
@@ -335,45 +354,44 @@ def bench_rtm3d_learned(args, torch, vti):
         t_plain = time.perf_counter() - t0
         img_p, _ = prop.rtm_image(finalize=False)
         fin_p, _ = prop.rtm_image(finalize=True)
-        lo = vti.learned_defaults()
-        lo.check_every, lo.tol, lo.ck_every, lo.max_exact = 1, 0.02, 0, max_exact
-        prop.rtm_reset()
-        t0 = time.perf_counter()
-        st = prop.rtm_shot_learned(A, prop.rtm_defaults(), lo, None)
-        t_learn = time.perf_counter() - t0
-        img_l, _ = prop.rtm_image(finalize=False)
-        fin_l, _ = prop.rtm_image(finalize=True)
         err_raw = float(np.linalg.norm(img_l - img_p) / max(np.linalg.norm(img_p), 1e-30))
         err = float(np.linalg.norm(fin_l - fin_p) / max(np.linalg.norm(fin_p), 1e-30))
-        rows.append((t_plain, t_learn, err, fsteps, st, err_raw))
-        log(f"[bench] shot {k}: plain {t_plain:.2f} s ({fsteps} fwd steps, {nsnap_equal} slots), "
-            f"learned {t_learn:.2f} s, final image rel-L2 {err:.3e} (raw {err_raw:.3e}), "
-            f"snapshot err {st['mean_checked_err']:.3e}, "
-            f"exact {st['exact']}/{st['snapshots']}")
+        rows.append(dict(t_plain=t_plain, t_learn=t_learn, err=err, err_raw=err_raw, fsteps=fsteps, st=st,
+                         slots=nsnap_equal, store=store))
+        log(f"[bench] shot {k}: learned {t_learn:.2f} s (store {store / 1e9:.2f} GB = "
+            f"{nsave * n ** 3 * 4 / store:.2f}x below fp32, snapshot err {st['mean_checked_err']:.3e}), "
+            f"plain {t_plain:.2f} s ({fsteps} fwd steps, {nsnap_equal} slots), "
+            f"final image rel-L2 {err:.3e} (raw {err_raw:.3e})")
     prop.close()
     A.close()
-    tp = float(np.mean([r[0] for r in rows])) if rows else 0.0
-    tl = float(np.mean([r[1] for r in rows])) if rows else 0.0
+
+    def mean(key, sub=None):
+        v = [(r[key][sub] if sub else r[key]) for r in rows]
+        return float(np.mean(v)) if v else None
+
+    tp, tl = mean("t_plain"), mean("t_learn")
     out = {
         "metric": "3D RTM shot time, learned snapshot compression vs plain checkpoint recomputation at equal memory",
-        "value": round(tp / tl, 3) if tl else None, "unit": "speedup (plain s / learned s)", "n_gpus": 1,
+        "value": round(tp / tl, 3) if rows else None, "unit": "speedup (plain s / learned s)", "n_gpus": 1,
         "steps": nt, "warmup": 0, "ms_per_step": round(tl / nt * 1e3, 4) if rows else None,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16", "data": "synthetic",
         "config": {"workload": f"3D VTI RTM {n}^3, {nt} steps, dt 0.8 ms, 15 Hz, {ntrain} fitting shots then "
                                f"{len(rows)} shots imaged both ways",
-                   "plain_s_per_shot": round(tp, 3), "learned_s_per_shot": round(tl, 3),
-                   "final_image_rel_l2_max": max((r[2] for r in rows), default=None),
-                   "final_image_rel_l2_mean": float(np.mean([r[2] for r in rows])) if rows else None,
-                   "raw_image_rel_l2_mean": float(np.mean([r[5] for r in rows])) if rows else None,
-                   "snapshot_rel_err_mean": float(np.mean([r[4]["mean_checked_err"] for r in rows])) if rows else None,
-                   "plain_checkpoint_slots": nsnap_equal, "plain_forward_steps": rows[0][3] if rows else None,
-                   "latent_store_GB": round(lat_bytes / 1e9, 3), "exact_store_GB": round(exact_bytes / 1e9, 3),
-                   "fit_s": round(t_fit, 2), "fit_energy_captured": explained,
-                   "exact_snapshots_mean": float(np.mean([r[4]["exact"] for r in rows])) if rows else None,
-                   "exact_budget": max_exact, "snapshots": nsave,
-                   "compression_of_stored_snapshots": round(nsave * n ** 3 * 4 / (lat_bytes + exact_bytes), 2),
-                   "encode_ms_per_shot": round(float(np.mean([r[4]["encode_ms"] for r in rows])), 1) if rows else None,
-                   "decode_ms_per_shot": round(float(np.mean([r[4]["decode_ms"] for r in rows])), 1) if rows else None},
+                   "codec": args.codec, "block_tol": args.tol if sparse else None,
+                   "block_floor": args.floor if sparse else None,
+                   "plain_s_per_shot": round(tp, 3) if rows else None,
+                   "learned_s_per_shot": round(tl, 3) if rows else None,
+                   "final_image_rel_l2_max": max((r["err"] for r in rows), default=None),
+                   "final_image_rel_l2_mean": mean("err"), "raw_image_rel_l2_mean": mean("err_raw"),
+                   "snapshot_rel_err_mean": mean("st", "mean_checked_err"),
+                   "plain_checkpoint_slots": [r["slots"] for r in rows],
+                   "plain_forward_steps": [r["fsteps"] for r in rows],
+                   "store_GB_mean": round(mean("store") / 1e9, 3) if rows else None,
+                   "compression_vs_fp32_snapshots": round(nsave * n ** 3 * 4 / mean("store"), 2) if rows else None,
+                   "fit_s": round(t_fit, 2), "fit_energy_captured_64": explained, "snapshots": nsave,
+                   "exact_snapshots_mean": mean("st", "exact"), "recomputed_snapshots_mean": mean("st", "recomputed"),
+                   "encode_ms_per_shot": round(mean("st", "encode_ms"), 1) if rows else None,
+                   "decode_ms_per_shot": round(mean("st", "decode_ms"), 1) if rows else None},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
@@ -399,7 +417,14 @@ def main():
     ap.add_argument("--fused", action="store_true",
                     help="fd3d: one fused kernel per step instead of the velocity and stress kernels")
     ap.add_argument("--exact-frac", type=float, default=0.25,
-                    help="learned RTM: fraction of snapshots the exact-storage fallback may keep")
+                    help="learned RTM, latent codec: fraction of snapshots the exact-storage fallback may keep")
+    ap.add_argument("--codec", default="sparse", choices=["sparse", "latent"],
+                    help="learned RTM: error-bounded sparse block coefficients or autoencoder latents")
+    ap.add_argument("--tol", type=float, default=0.01,
+                    help="learned RTM, sparse codec: relative L2 error bound of every stored block")
+    ap.add_argument("--floor", type=float, default=1e-8,
+                    help="learned RTM, sparse codec: blocks below this fraction of the strongest block's "
+                         "energy are held to tol of that fraction")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "

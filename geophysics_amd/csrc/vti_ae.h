@@ -7,6 +7,7 @@
 #include <cstdint>
 
 #include "../../include/vti.h"
+#include "vti_bcodec.h"
 
 namespace vti {
 
@@ -24,6 +25,14 @@ hipStream_t ae_stream(vti_ae *A);
 // (scaled by its max |P|) into the Gram matrix; ae_fit_end sets the weights.
 int ae_fit_field(vti_ae *A, const float *P, long long off, long long sy, long long sx, int n, int nblk);
 int ae_fit_end(vti_ae *A, double *explained);
+// Sparse block-transform codec with the autoencoder's full basis (set by
+// ae_fit_end or ae_set_basis; V[k][j] host, 512 x 512, natural voxel order).
+int ae_set_basis(vti_ae *A, const float *V);
+int ae_bc_encode(vti_ae *A, const float *P, long long off, long long sy, long long sx, int n, const bc::Mask &m,
+                 float tol, float flo, uint32_t *ent, long long cap, uint32_t *boff, uint16_t *bcnt, long long *kept, double *err,
+                 double *energy, hipStream_t s);
+int ae_bc_decode(vti_ae *A, const uint32_t *ent, const uint32_t *boff, const uint16_t *bcnt, int n, float *out,
+                 hipStream_t s);
 
 }  // namespace vti
 

@@ -88,9 +88,16 @@ struct vti_ae {
     int *blk_idx = nullptr;
     int blk_cap = 0;
     long long fit_rows = 0;
+    // full orthonormal block basis for the sparse codec (vti_bcodec.h):
+    // host copy V[k][j] (natural voxel order) and its device bf16 splits
+    std::vector<float> basis_host;
+    uint16_t *basis = nullptr;
+    bc::Work bcw;
 
     ~vti_ae()
     {
+        bc::work_free(bcw);
+        (void)hipFree(basis);
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto *p : {master, m, v, grad, out, target, dev_loss, scale, crop_scale, gram}) (void)hipFree(p);
         (void)hipFree(blocks);
@@ -458,6 +465,39 @@ static void jacobi_eigen(int n, std::vector<double> &A, std::vector<double> &w, 
 
 namespace vti {
 
+int ae_set_basis(vti_ae *A, const float *V)
+{
+    for (int i = 0; i < 512 * 512; ++i)
+        if (!std::isfinite(V[i])) return aefail(VTI_ERR_ARG, "autoencoder: non-finite basis");
+    if (!A->basis && hipMalloc(&A->basis, 4 * 512 * 512 * sizeof(uint16_t)) != hipSuccess)
+        return aefail(VTI_ERR_ALLOC, "autoencoder: basis");
+    std::vector<uint16_t> h(4 * 512 * 512);
+    bc::basis_split(V, h.data());
+    A->basis_host.assign(V, V + 512 * 512);
+    AE_HIP(hipMemcpyAsync(A->basis, h.data(), h.size() * sizeof(uint16_t), hipMemcpyHostToDevice, A->stream));
+    AE_HIP(hipStreamSynchronize(A->stream));
+    return VTI_OK;
+}
+
+int ae_bc_encode(vti_ae *A, const float *P, long long off, long long sy, long long sx, int n, const bc::Mask &m,
+                 float tol, float flo, uint32_t *ent, long long cap, uint32_t *boff, uint16_t *bcnt, long long *kept, double *err,
+                 double *energy, hipStream_t s)
+{
+    if (!A->basis) return aefail(VTI_ERR_STATE, "sparse codec: no basis (fit or vti_ae_set_basis first)");
+    const int st = bc::encode(bc::basis_view(A->basis), P, off, sy, sx, n, m, tol, flo, A->bcw, ent, cap, boff, bcnt,
+                              kept, err, energy, s);
+    if (st) return aefail(st, "sparse codec: encode");
+    return VTI_OK;
+}
+
+int ae_bc_decode(vti_ae *A, const uint32_t *ent, const uint32_t *boff, const uint16_t *bcnt, int n, float *out,
+                 hipStream_t s)
+{
+    if (!A->basis) return aefail(VTI_ERR_STATE, "sparse codec: no basis (fit or vti_ae_set_basis first)");
+    AE_HIP(bc::launch_decode(bc::basis_view(A->basis), ent, boff, bcnt, n, out, s));
+    return VTI_OK;
+}
+
 // Weights of the linear autoencoder whose encoder projects every 8^3 block
 // on the 64 leading principal components of the fitted blocks and whose
 // decoder reconstructs from them; the latent convolutions start as identity.
@@ -501,6 +541,20 @@ int ae_fit_end(vti_ae *A, double *explained)
     f += 512 * 64;
     ident(64, 64);                            // D2
     ident(8, 8);                              // D1
+    // the full basis, voxels in natural order (ty*8 + tx)*8 + tz, for the
+    // sparse codec; the fit's block order is the encoder's tap order
+    std::vector<float> nat(512 * 512);
+    for (int ty = 0; ty < 8; ++ty)
+        for (int tx = 0; tx < 8; ++tx)
+            for (int tz = 0; tz < 8; ++tz) {
+                const int t2 = ((ty >> 2) << 2) | ((tx >> 2) << 1) | (tz >> 2);
+                const int t1 = (((ty >> 1) & 1) << 2) | (((tx >> 1) & 1) << 1) | ((tz >> 1) & 1);
+                const int t0 = ((ty & 1) << 2) | ((tx & 1) << 1) | (tz & 1);
+                const int kt = (t2 * 8 + t1) * 8 + t0, kn = (ty * 8 + tx) * 8 + tz;
+                for (int j = 0; j < 512; ++j) nat[(size_t)kn * 512 + j] = (float)V[(size_t)kt * 512 + ord[j]];
+            }
+    int st = ae_set_basis(A, nat.data());
+    if (st) return st;
     (void)hipFree(A->gram);
     A->gram = nullptr;
     A->fit_rows = 0;
@@ -586,6 +640,42 @@ int vti_ae_set_params_padded_(vti_ae *A, const float *h)
     AE_HIP(hipMemcpyAsync(A->master, h, A->nparam * sizeof(float), hipMemcpyHostToDevice, A->stream));
     AE_HIP(launch_f32_to_bf16(A->master, A->wb, (long long)A->nparam, A->stream));
     int st = rebuild_wt(A);
+    if (st) return st;
+    AE_HIP(hipStreamSynchronize(A->stream));
+    return VTI_OK;
+}
+
+int vti_ae_set_basis(vti_ae *A, const float *V)
+{
+    if (!A || !V) return aefail(VTI_ERR_ARG, "vti_ae_set_basis: null argument");
+    return ae_set_basis(A, V);
+}
+
+int vti_ae_get_basis(vti_ae *A, float *V)
+{
+    if (!A || !V) return aefail(VTI_ERR_ARG, "vti_ae_get_basis: null argument");
+    if (A->basis_host.empty()) return aefail(VTI_ERR_STATE, "vti_ae_get_basis: no basis");
+    std::copy(A->basis_host.begin(), A->basis_host.end(), V);
+    return VTI_OK;
+}
+
+int vti_bc_encode(vti_ae *A, const float *P, long long off, long long sy, long long sx, int n, float tol,
+                  float floor, unsigned *ent, long long cap, unsigned *boff, unsigned short *bcnt, long long *kept,
+                  double *err)
+{
+    if (!A || !P || !ent || !boff || !bcnt || n < 8 || n % 8 || !(tol >= 0.0f) || !(floor >= 0.0f) || cap < 0)
+        return aefail(VTI_ERR_ARG, "vti_bc_encode: bad arguments");
+    const bc::Mask m{0, 0, 0, 0, 0, 0};
+    const int st = ae_bc_encode(A, P, off, sy, sx, n, m, tol, floor, ent, cap, boff, bcnt, kept, err, nullptr, A->stream);
+    if (st) return st;
+    AE_HIP(hipStreamSynchronize(A->stream));
+    return VTI_OK;
+}
+
+int vti_bc_decode(vti_ae *A, const unsigned *ent, const unsigned *boff, const unsigned short *bcnt, int n, float *out)
+{
+    if (!A || !ent || !boff || !bcnt || !out || n < 8 || n % 8) return aefail(VTI_ERR_ARG, "vti_bc_decode: bad arguments");
+    const int st = ae_bc_decode(A, ent, boff, bcnt, n, out, A->stream);
     if (st) return st;
     AE_HIP(hipStreamSynchronize(A->stream));
     return VTI_OK;

@@ -1,0 +1,394 @@
+// vti_bcodec.hip — error-bounded sparse block-transform codec (gfx950).
+//
+// Encode: a workgroup (4 wavefronts) takes 32 blocks, stages them in LDS as
+// bf16 hi/lo pairs (x = hi + lo to 2^-16), and computes the 32 x 512
+// coefficient tile on v_mfma_f32_16x16x32_bf16 as Xh Vh + Xl Vh + Xh Vl
+// (fp32 accumulation; the dropped Xl Vl term is below 2^-16 relative); the
+// staging also yields the strongest block energy (one atomic per
+// workgroup).  Pack: one wavefront per block finds, by bisection on the
+// float bits of c^2 (31 wave reductions), the largest cut whose dropped
+// energy fits the block's budget, keeps the coefficients above it, places
+// them with a wave prefix sum and one atomic per block.  Decode: the kept
+// values (bf16, exact) are scattered into a zeroed LDS tile and multiplied
+// by V (hi and lo).
+#include "vti_bcodec.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "../../include/vti.h"
+
+namespace vti {
+namespace bc {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef unsigned short us8 __attribute__((ext_vector_type(8)));
+
+constexpr int BB = kBlocksPerWG;
+constexpr int LROW = 512 + 8;   // LDS row (bf16), padded against bank conflicts
+
+__device__ __forceinline__ bf16x8 as_bf(us8 v) { return __builtin_bit_cast(bf16x8, v); }
+
+__device__ __forceinline__ uint16_t to_bf(float f)   // round to nearest even
+{
+    const uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+__device__ __forceinline__ float from_bf(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__global__ __launch_bounds__(256) void encode_kernel(Basis V, const float *__restrict__ P, long long off,
+                                                     long long sy, long long sx, int n, Mask m,
+                                                     float *__restrict__ coef, Counters *cnt)
+{
+    __shared__ uint16_t xh[BB][LROW];
+    __shared__ uint16_t xl[BB][LROW];
+    __shared__ float eb[BB];
+    const int nb = n / 8;
+    const long long nblk = (long long)nb * nb * nb;
+    const long long b0 = (long long)blockIdx.x * BB;
+    if (threadIdx.x < BB) eb[threadIdx.x] = 0.0f;
+    __syncthreads();
+    // stage: 4 consecutive voxels (same ty, tx) per thread and iteration
+    for (int e = threadIdx.x; e < BB * 128; e += 256) {
+        const int i = e >> 7, k = (e & 127) * 4;
+        const long long b = b0 + i;
+        float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (b < nblk) {
+            const int bz = (int)(b % nb), bx = (int)((b / nb) % nb), by = (int)(b / ((long long)nb * nb));
+            const int y = 8 * by + (k >> 6), x = 8 * bx + ((k >> 3) & 7), z = 8 * bz + (k & 7);
+            const float *src = P + off + (long long)y * sy + (long long)x * sx + z;
+            const bool yin = y >= m.y0 && y < m.y0 + m.by && x >= m.x0 && x < m.x0 + m.bx;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const bool masked = yin && z + q >= m.z0 && z + q < m.z0 + m.bz;
+                v[q] = masked ? 0.0f : src[q];
+            }
+        }
+        float e2 = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint16_t h = to_bf(v[q]);
+            xh[i][k + q] = h;
+            xl[i][k + q] = to_bf(v[q] - from_bf(h));
+            e2 += v[q] * v[q];
+        }
+        atomicAdd(&eb[i], e2);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float mx = 0.0f;
+        for (int i = 0; i < BB; ++i) mx = fmaxf(mx, eb[i]);
+        atomicMax(&cnt->emax_bits, __float_as_uint(mx));   // non-negative floats order as their bits
+    }
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    f4 acc[2][8];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[r][c] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int kk = 0; kk < 512; kk += 32) {
+        const int kc = kk + 8 * (l >> 4);
+        us8 ah[2], al[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            ah[r] = *reinterpret_cast<const us8 *>(&xh[16 * r + (l & 15)][kc]);
+            al[r] = *reinterpret_cast<const us8 *>(&xl[16 * r + (l & 15)][kc]);
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int j = 128 * w + 16 * c + (l & 15);
+            const us8 bh = *reinterpret_cast<const us8 *>(V.vt_h + (long long)j * 512 + kc);
+            const us8 bl = *reinterpret_cast<const us8 *>(V.vt_l + (long long)j * 512 + kc);
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(ah[r]), as_bf(bh), acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(al[r]), as_bf(bh), acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(ah[r]), as_bf(bl), acc[r][c], 0, 0, 0);
+            }
+        }
+    }
+    // D row 4*(l/16) + e, column l%16 of each 16 x 16 tile
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const long long b = b0 + 16 * r + 4 * (l >> 4) + e;
+            if (b >= nblk) continue;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) coef[b * 512 + 128 * w + 16 * c + (l & 15)] = acc[r][c][e];
+        }
+}
+
+// wave-uniform sum: lane 0's butterfly result broadcast (the lanes' own
+// results may differ in the last bit)
+__device__ __forceinline__ float wave_sum(float x)
+{
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+    return __shfl(x, 0, 64);
+}
+
+__global__ __launch_bounds__(256) void pack_kernel(const float *__restrict__ coef, long long nblk, float tol2,
+                                                   float flo, uint32_t *ent, unsigned cap, uint32_t *boff,
+                                                   uint16_t *bcnt, Counters *cnt)
+{
+    __shared__ float sd[4], se[4];
+    const long long b = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    float dropped = 0.0f, eblk = 0.0f;
+    if (b < nblk) {   // wavefront-uniform
+        const float *c = coef + b * 512 + 8 * l;
+        float v[8];
+        unsigned q[8];
+        float e2 = 0.0f;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            v[e] = c[e];
+            q[e] = __float_as_uint(v[e] * v[e]);
+            e2 += v[e] * v[e];
+        }
+        eblk = wave_sum(e2);
+        const float emax = __uint_as_float(cnt->emax_bits);
+        const float budget = tol2 * fmaxf(eblk, flo * emax);
+        // largest cut t with sum of c^2 over bits(c^2) < t within budget
+        unsigned lo = 0u, hi = 0x7f800001u;
+        while (hi - lo > 1u) {
+            const unsigned mid = lo + (hi - lo) / 2u;
+            float part = 0.0f;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) part += q[e] < mid ? __uint_as_float(q[e]) : 0.0f;
+            if (wave_sum(part) <= budget) lo = mid;
+            else hi = mid;
+        }
+        int n_keep = 0;
+        float part = 0.0f;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            n_keep += q[e] >= lo;
+            part += q[e] < lo ? __uint_as_float(q[e]) : 0.0f;
+        }
+        dropped = wave_sum(part);
+        int incl = n_keep;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int t = __shfl_up(incl, d, 64);
+            if (l >= d) incl += t;
+        }
+        const int total = __shfl(incl, 63, 64);
+        unsigned base = 0;
+        if (l == 0 && total > 0) base = atomicAdd(&cnt->cursor, (unsigned)total);
+        base = __shfl(base, 0, 64);
+        const bool fits = (unsigned long long)base + (unsigned)total <= cap;
+        if (l == 0) {
+            boff[b] = base;
+            bcnt[b] = (uint16_t)total;
+            if (!fits) atomicOr(&cnt->overflow, 1u);
+        }
+        if (fits) {
+            unsigned pos = base + (unsigned)(incl - n_keep);
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                if (q[e] >= lo) ent[pos++] = ((uint32_t)to_bf(v[e]) << 16) | (uint32_t)(8 * l + e);
+        }
+    }
+    if (l == 0) {
+        sd[w] = dropped;
+        se[w] = eblk;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(&cnt->dropped, (double)sd[0] + sd[1] + sd[2] + sd[3]);
+        atomicAdd(&cnt->energy, (double)se[0] + se[1] + se[2] + se[3]);
+    }
+}
+
+__global__ __launch_bounds__(256) void decode_kernel(Basis V, const uint32_t *__restrict__ ent,
+                                                     const uint32_t *__restrict__ boff,
+                                                     const uint16_t *__restrict__ bcnt, int n,
+                                                     float *__restrict__ out)
+{
+    __shared__ uint16_t ch[BB][LROW];
+    const int nb = n / 8;
+    const long long nblk = (long long)nb * nb * nb;
+    const long long b0 = (long long)blockIdx.x * BB;
+    for (int i = threadIdx.x; i < BB * LROW / 2; i += 256) reinterpret_cast<uint32_t *>(&ch[0][0])[i] = 0u;
+    __syncthreads();
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    for (int i = w; i < BB; i += 4) {
+        const long long b = b0 + i;
+        if (b >= nblk) break;
+        const int cnt = bcnt[b];
+        const uint32_t *e = ent + boff[b];
+        for (int q = l; q < cnt; q += 64) {
+            const uint32_t x = e[q];
+            ch[i][x & 511u] = (uint16_t)(x >> 16);
+        }
+    }
+    __syncthreads();
+    f4 acc[2][8];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[r][c] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int kk = 0; kk < 512; kk += 32) {
+        const int kc = kk + 8 * (l >> 4);
+        us8 a[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) a[r] = *reinterpret_cast<const us8 *>(&ch[16 * r + (l & 15)][kc]);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int k = 128 * w + 16 * c + (l & 15);
+            const us8 bh = *reinterpret_cast<const us8 *>(V.v_h + (long long)k * 512 + kc);
+            const us8 bl = *reinterpret_cast<const us8 *>(V.v_l + (long long)k * 512 + kc);
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(a[r]), as_bf(bh), acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(a[r]), as_bf(bl), acc[r][c], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const long long b = b0 + 16 * r + 4 * (l >> 4) + e;
+            if (b >= nblk) continue;
+            const int bz = (int)(b % nb), bx = (int)((b / nb) % nb), by = (int)(b / ((long long)nb * nb));
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int k = 128 * w + 16 * c + (l & 15);
+                const int y = 8 * by + (k >> 6), x = 8 * bx + ((k >> 3) & 7), z = 8 * bz + (k & 7);
+                out[((long long)y * n + x) * n + z] = acc[r][c][e];
+            }
+        }
+}
+
+__global__ void zero_kernel(uint32_t *p, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+uint16_t host_bf(float f)   // round to nearest even (finite inputs)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+float host_from_bf(uint16_t h)
+{
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+}  // namespace
+
+int work_reserve(Work &w, int n)
+{
+    const long long nb = n / 8, nblk = nb * nb * nb;
+    if (!w.cnt && hipMalloc(&w.cnt, sizeof(Counters)) != hipSuccess) return VTI_ERR_ALLOC;
+    if (nblk > w.cap_blocks) {
+        (void)hipFree(w.coef);
+        w.coef = nullptr;
+        w.cap_blocks = 0;
+        if (hipMalloc(&w.coef, (size_t)nblk * 512 * sizeof(float)) != hipSuccess) return VTI_ERR_ALLOC;
+        w.cap_blocks = nblk;
+    }
+    return VTI_OK;
+}
+
+void work_free(Work &w)
+{
+    (void)hipFree(w.coef);
+    (void)hipFree(w.cnt);
+    w = Work{};
+}
+
+void basis_split(const float *V, uint16_t *out)
+{
+    uint16_t *vt_h = out, *vt_l = out + 512 * 512, *v_h = out + 2 * 512 * 512, *v_l = out + 3 * 512 * 512;
+    for (int k = 0; k < 512; ++k)
+        for (int j = 0; j < 512; ++j) {
+            const float x = V[k * 512 + j];
+            const uint16_t h = host_bf(x);
+            const uint16_t lo = host_bf(x - host_from_bf(h));
+            v_h[k * 512 + j] = h;
+            v_l[k * 512 + j] = lo;
+            vt_h[j * 512 + k] = h;
+            vt_l[j * 512 + k] = lo;
+        }
+}
+
+Basis basis_view(const uint16_t *dev)
+{
+    return Basis{dev, dev + 512 * 512, dev + 2 * 512 * 512, dev + 3 * 512 * 512};
+}
+
+hipError_t launch_encode(const Basis &V, const float *P, long long off, long long sy, long long sx, int n,
+                         const Mask &m, Work &w, hipStream_t s)
+{
+    const long long nb = n / 8, nblk = nb * nb * nb;
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t *>(w.cnt),
+                       (long long)(sizeof(Counters) / 4));
+    hipLaunchKernelGGL(encode_kernel, dim3((unsigned)((nblk + BB - 1) / BB)), dim3(256), 0, s, V, P, off, sy, sx, n, m,
+                       w.coef, w.cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(const Work &w, int n, float tol2, float flo, uint32_t *ent, long long cap, uint32_t *boff,
+                       uint16_t *bcnt, hipStream_t s)
+{
+    const long long nb = n / 8, nblk = nb * nb * nb;
+    // a snapshot has at most n^3 entries
+    const unsigned c32 = (unsigned)std::max(0LL, std::min(cap, (long long)n * n * n));
+    hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((nblk + 3) / 4)), dim3(256), 0, s, w.coef, nblk, tol2, flo, ent,
+                       c32, boff, bcnt, w.cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_decode(const Basis &V, const uint32_t *ent, const uint32_t *boff, const uint16_t *bcnt, int n,
+                         float *out, hipStream_t s)
+{
+    const long long nb = n / 8, nblk = nb * nb * nb;
+    hipLaunchKernelGGL(decode_kernel, dim3((unsigned)((nblk + BB - 1) / BB)), dim3(256), 0, s, V, ent, boff, bcnt, n,
+                       out);
+    return hipGetLastError();
+}
+
+int encode(const Basis &V, const float *P, long long off, long long sy, long long sx, int n, const Mask &m,
+           float tol, float flo, Work &w, uint32_t *ent, long long cap, uint32_t *boff, uint16_t *bcnt,
+           long long *kept, double *err, double *energy, hipStream_t s)
+{
+    if (n < 8 || n % 8 || !(tol >= 0.0f) || !(flo >= 0.0f)) return VTI_ERR_ARG;
+    int st = work_reserve(w, n);
+    if (st) return st;
+    // rounding a kept value to bf16 moves it by at most 2^-8 of itself: that
+    // share of every block's budget is set aside for the kept energy
+    const double q2 = std::ldexp(1.0, -16);
+    const double t2 = (double)tol * tol;
+    const float tol2 = t2 > q2 ? (float)(t2 - q2) : 0.0f;
+    if (launch_encode(V, P, off, sy, sx, n, m, w, s) != hipSuccess ||
+        launch_pack(w, n, tol2, flo, ent, cap, boff, bcnt, s) != hipSuccess)
+        return VTI_ERR_HIP;
+    Counters h;
+    if (hipMemcpyAsync(&h, w.cnt, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return VTI_ERR_HIP;
+    const long long k = (long long)h.cursor;
+    if (kept) *kept = (h.overflow && k <= cap) ? cap + 1 : k;
+    if (energy) *energy = h.energy;
+    if (err) *err = h.energy > 0.0 ? std::sqrt(std::min(1.0, (h.dropped + q2 * (h.energy - h.dropped)) / h.energy)) : 0.0;
+    return VTI_OK;
+}
+
+}  // namespace bc
+}  // namespace vti

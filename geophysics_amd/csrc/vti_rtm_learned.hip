@@ -1,5 +1,9 @@
-// vti_rtm_learned.hip — 3D RTM with the source wavefield kept as
-// autoencoder latents instead of being recomputed from checkpoints.
+// vti_rtm_learned.hip — 3D RTM with the source wavefield kept compressed
+// instead of being recomputed from checkpoints: as autoencoder latents
+// (codec 0), or as error-bounded sparse coefficients of the fitted block
+// basis (codec 1, vti_bcodec.h: every stored snapshot within relative L2
+// error tol, the source box exact, snapshots past the byte budget
+// recomputed from checkpoints).  The rest of this comment is codec 0.
 //
 // First sweep (the forward modelling of the shot, with record and source
 // illumination exactly as vti_rtm_shot): after every step it with
@@ -160,6 +164,9 @@ int vti_learned_defaults(vti_learned_opts *o)
     o->ck_every = 0;
     o->max_exact = 0;
     o->src_box = 8;
+    o->codec = 0;
+    o->floor = 1e-8f;
+    o->budget = 0;
     return VTI_OK;
 }
 
@@ -173,8 +180,11 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
     if (k.ndim != 3 || k.ny != n || k.nz != n || n % 8)
         return fail(VTI_ERR_ARG, "vti_rtm_shot_learned: 3D cubic interiors with a side divisible by 8 only");
     if (c->nrec <= 0) return fail(VTI_ERR_STATE, "vti_rtm_shot_learned: the shot has no receivers");
-    if (lo->save_every < 1 || lo->check_every < 0 || lo->ck_every < 0 || lo->max_exact < 0 || lo->src_box < 0)
+    if (lo->save_every < 1 || lo->check_every < 0 || lo->ck_every < 0 || lo->max_exact < 0 || lo->src_box < 0 ||
+        lo->codec < 0 || lo->codec > 1 || lo->budget < 0 ||
+        (lo->codec == 1 && (!(lo->tol >= 0.0f) || !(lo->floor >= 0.0f))))
         return fail(VTI_ERR_ARG, "vti_rtm_shot_learned: bad options");
+    const bool sparse = lo->codec == 1;
     int st;
     if (!c->img && (st = vti_rtm_reset(c))) return st;
     const int nt = k.nt, nrec = c->nrec, np = k.npml, se = lo->save_every;
@@ -184,12 +194,25 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
     const int nsave = (nt + se - 1) / se;
     const long long off = np + (long long)np * c->g.ldz + (long long)np * c->g.ldxy;
     hipStream_t s = c->stream;
-    Buf lat, scl, dense, dense2, acc, exa, boxes;
-    if (hipMalloc(&lat.p, (size_t)nsave * lat_n * sizeof(uint16_t)) != hipSuccess ||
-        hipMalloc(&scl.p, (size_t)nsave * 64) != hipSuccess ||
+    Buf lat, scl, dense, dense2, acc, exa, boxes, ents, boffs, bcnts;
+    if ((!sparse && (hipMalloc(&lat.p, (size_t)nsave * lat_n * sizeof(uint16_t)) != hipSuccess ||
+                     hipMalloc(&scl.p, (size_t)nsave * 64) != hipSuccess)) ||
         hipMalloc(&dense.p, (size_t)n * n * n * sizeof(float)) != hipSuccess ||
         hipMalloc(&acc.p, 2 * sizeof(double)) != hipSuccess)
         return fail(VTI_ERR_ALLOC, "vti_rtm_shot_learned: latent store");
+    // sparse codec store: entries (4 bytes each) within the budget, block tables per snapshot
+    const long long nblk = (long long)(n / 8) * (n / 8) * (n / 8);
+    const long long budget = lo->budget > 0 ? lo->budget : (long long)nsave * n * n * n;
+    const long long cap = sparse ? budget / 4 : 0;
+    if (sparse && (hipMalloc(&ents.p, (size_t)std::max(cap, 1LL) * 4) != hipSuccess ||
+                   hipMalloc(&boffs.p, (size_t)nsave * nblk * 4) != hipSuccess ||
+                   hipMalloc(&bcnts.p, (size_t)nsave * nblk * 2) != hipSuccess))
+        return fail(VTI_ERR_ALLOC, "vti_rtm_shot_learned: sparse store");
+    uint32_t *ENT = static_cast<uint32_t *>(ents.p);
+    uint32_t *BOFF = static_cast<uint32_t *>(boffs.p);
+    uint16_t *BCNT = static_cast<uint16_t *>(bcnts.p);
+    std::vector<long long> ebase(nsave, 0);
+    long long used = 0;
     uint16_t *L = static_cast<uint16_t *>(lat.p);
     float *Sc = static_cast<float *>(scl.p);
     float *D = static_cast<float *>(dense.p);
@@ -257,6 +280,28 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         const int j = it / se;
         const float *P = W.f[VTI_F_P];
         const double t0 = now_ms();
+        if (sparse) {   // error-bounded sparse coefficients; the source box exactly
+            if (box_n)
+                hipLaunchKernelGGL(box_save_kernel, dim3(64), dim3(256), 0, s, c->g, np, P, bx, BX + (size_t)j * box_n);
+            const bc::Mask m{bx.y0, bx.x0, bx.z0, bx.by, bx.bx, bx.bz};
+            long long need = 0;
+            double e = 0.0;
+            if ((st = ae_bc_encode(A, P, off, c->g.ldxy, c->g.ldz, n, m, lo->tol, lo->floor, ENT + used, cap - used,
+                                   BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, &need, &e, nullptr, s)))
+                return st;
+            t_enc += now_ms() - t0;
+            if (need > cap - used) {   // over budget: recompute this snapshot from a checkpoint
+                if (!nck) return fail(VTI_ERR_ALLOC, "vti_rtm_shot_learned: sparse store budget exhausted (set ck_every)");
+                bad[j] = 1;
+                ++nbad;
+            } else {
+                ebase[j] = used;
+                used += need;
+                err_sum += e;
+                ++nchecked;
+            }
+            continue;
+        }
         if (box_n) {   // the source box is kept exactly; the autoencoder sees the rest
             hipLaunchKernelGGL(box_save_kernel, dim3(64), dim3(256), 0, s, c->g, np, P, bx, BX + (size_t)j * box_n);
             hipLaunchKernelGGL(masked_interior_kernel, dim3(2048), dim3(256), 0, s, c->g, np, n, P, bx, D);
@@ -329,7 +374,9 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
             if ((st = hip(launch_image(c, W.f[VTI_F_P], G.f[VTI_F_P], c->img), "image"))) return st;
         } else {
             const double t0 = now_ms();
-            if ((st = ae_decode(A, L + (size_t)j * lat_n, Sc + (size_t)j * 16, n, D, s))) return st;
+            if (sparse) st = ae_bc_decode(A, ENT + ebase[j], BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, n, D, s);
+            else st = ae_decode(A, L + (size_t)j * lat_n, Sc + (size_t)j * 16, n, D, s);
+            if (st) return st;
             if (box_n)
                 hipLaunchKernelGGL(box_restore_kernel, dim3(64), dim3(256), 0, s, n, bx, BX + (size_t)j * box_n, D);
             t_dec += now_ms() - t0;
@@ -346,6 +393,11 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         stats[3] = nbad;
         stats[4] = nsave;
         stats[5] = nexact;
+        int nrec_snap = 0;
+        for (char b : bad) nrec_snap += b;
+        stats[6] = sparse ? (double)used * 4 + (double)nsave * nblk * 6 + (double)nsave * box_n * 4
+                          : (double)nsave * lat_n * 2 + (double)nexact * nn3 * 4 + (double)nsave * box_n * 4;
+        stats[7] = nrec_snap;
     }
     return VTI_OK;
 }

@@ -60,7 +60,8 @@ class RtmOpts(C.Structure):
 _lib = None
 class LearnedOpts(C.Structure):
     _fields_ = [("save_every", C.c_int), ("check_every", C.c_int), ("tol", C.c_float), ("ck_every", C.c_int),
-                ("max_exact", C.c_int), ("src_box", C.c_int)]
+                ("max_exact", C.c_int), ("src_box", C.c_int), ("codec", C.c_int), ("floor", C.c_float),
+                ("budget", C.c_longlong)]
 
 
 FP = C.POINTER(C.c_float)
@@ -136,6 +137,11 @@ def _declare(L):
         "vti_ae_latent_elems": (C.c_longlong, [C.c_int]),
         "vti_ae_fit_field": (C.c_int, [VP, VP, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_int]),
         "vti_ae_fit_end": (C.c_int, [VP, DP]),
+        "vti_ae_set_basis": (C.c_int, [VP, FP]),
+        "vti_ae_get_basis": (C.c_int, [VP, FP]),
+        "vti_bc_encode": (C.c_int, [VP, VP, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_float,
+                                    C.c_float, VP, C.c_longlong, VP, VP, C.POINTER(C.c_longlong), DP]),
+        "vti_bc_decode": (C.c_int, [VP, VP, VP, VP, C.c_int, VP]),
         "vti_sym_eigen": (C.c_int, [C.c_int, DP, DP, DP]),
         "vti_ae_fit_shot": (C.c_int, [VP, VP, C.c_int, C.c_int, C.c_int]),
         "vti_learned_defaults": (C.c_int, [C.POINTER(LearnedOpts)]),
@@ -381,11 +387,12 @@ class Propagator:
         return img, ill
 
     def rtm_shot_learned(self, ae, opts=None, lopts=None, obs=None):
-        """RTM of the current shot with the source field stored as
-        autoencoder latents; returns the stats dict."""
+        """RTM of the current shot with the source field stored compressed
+        (autoencoder latents, or sparse block-transform coefficients with
+        lopts.codec = 1); returns the stats dict."""
         o = opts if opts is not None else self.rtm_defaults()
         lo = lopts if lopts is not None else learned_defaults()
-        st = np.zeros(6, np.float64)
+        st = np.zeros(8, np.float64)
         p = None
         if obs is not None:
             a, p = _f32(obs)
@@ -393,7 +400,7 @@ class Propagator:
         check(lib().vti_rtm_shot_learned(self.h, ae.h, C.byref(o), C.byref(lo), p, st.ctypes.data_as(DP)),
               "rtm_shot_learned")
         return {"encode_ms": st[0], "decode_ms": st[1], "mean_checked_err": st[2], "over_tol": int(st[3]),
-                "snapshots": int(st[4]), "exact": int(st[5])}
+                "snapshots": int(st[4]), "exact": int(st[5]), "store_bytes": st[6], "recomputed": int(st[7])}
 
     def ae_fit_shot(self, ae, save_every=4, nblocks=4096, src_box=8):
         check(lib().vti_ae_fit_shot(self.h, ae.h, save_every, nblocks, src_box), "ae_fit_shot")
@@ -695,6 +702,31 @@ class Autoencoder:
         e = C.c_double()
         check(lib().vti_ae_fit_end(self.h, C.byref(e)), "ae_fit_end")
         return e.value
+
+    def set_basis(self, V):
+        """Orthonormal 512 x 512 block basis V[k][j] of the sparse codec."""
+        a, p = _f32(V)
+        if a.size != 512 * 512:
+            raise VtiError("basis must be 512 x 512")
+        check(lib().vti_ae_set_basis(self.h, p), "ae_set_basis")
+
+    def basis(self):
+        out = np.zeros((512, 512), np.float32)
+        check(lib().vti_ae_get_basis(self.h, out.ctypes.data_as(FP)), "ae_get_basis")
+        return out
+
+    def bc_encode(self, ptr, off, sy, sx, n, tol, ent, cap, boff, bcnt, floor=1e-8):
+        """Sparse codec encode of the n^3 box at device pointer ptr into the
+        device buffers ent (cap uint32), boff (uint32), bcnt (uint16) of
+        (n/8)^3 blocks; returns (entries needed, error bound)."""
+        kept = C.c_longlong()
+        err = C.c_double()
+        check(lib().vti_bc_encode(self.h, ptr, off, sy, sx, n, tol, floor, ent, cap, boff, bcnt, C.byref(kept),
+                                  C.byref(err)), "bc_encode")
+        return kept.value, err.value
+
+    def bc_decode(self, ent, boff, bcnt, n, out):
+        check(lib().vti_bc_decode(self.h, ent, boff, bcnt, n, out), "bc_decode")
 
     @staticmethod
     def latent_elems(n):

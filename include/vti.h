@@ -303,6 +303,33 @@ long long vti_ae_latent_elems(int n);
 int vti_ae_fit_field(vti_ae *ae, const float *P, long long off, long long sy, long long sx, int n,
                      int nblocks);
 int vti_ae_fit_end(vti_ae *ae, double *explained);
+/* The fit also keeps the full orthonormal 512 x 512 basis of 8^3 blocks
+ * (V[k][j], k = (ty*8 + tx)*8 + tz the voxel, j the component, leading
+ * components first) for the sparse codec below; set_basis replaces it
+ * (any orthonormal V, host, row-major). */
+int vti_ae_set_basis(vti_ae *ae, const float *V);
+int vti_ae_get_basis(vti_ae *ae, float *V);
+/* Sparse block-transform codec (device pointers; n^3 box at
+ * P[off + y*sy + x*sx + z], n a multiple of 8).  Every 8^3 block is
+ * projected on the basis and its smallest coefficients are dropped while
+ * the block's L2 error stays <= tol x max(block norm, sqrt(floor) x the
+ * strongest block's norm) (Parseval: the dropped energy is the error; the
+ * bf16 rounding of the kept values is budgeted as 2^-8 of their magnitude,
+ * so tol <= 2^-8 keeps every non-zero coefficient and bounds the error by
+ * 2^-8).  Kept coefficients are entries
+ * ent[] = bf16 value << 16 | voxel-order index; per block (z fastest, then
+ * x, then y over the (n/8)^3 blocks) boff = first entry, bcnt = count.
+ * *kept = entries needed; when *kept > cap the entries are incomplete
+ * (never written past cap).  *err = the bound on the box's relative L2
+ * error.  Both calls run on the autoencoder's
+ * stream and return after it has finished; work producing P must be
+ * complete before the call.  No counterpart in the reference. */
+int vti_bc_encode(vti_ae *ae, const float *P, long long off, long long sy, long long sx, int n, float tol,
+                  float floor, unsigned *ent, long long cap, unsigned *boff, unsigned short *bcnt, long long *kept,
+                  double *err);
+/* Dense n^3 box (z fastest) from the entries of vti_bc_encode. */
+int vti_bc_decode(vti_ae *ae, const unsigned *ent, const unsigned *boff, const unsigned short *bcnt, int n,
+                  float *out);
 /* Symmetric eigen-decomposition used by the fit (cyclic Jacobi, host):
  * a n x n row-major, w eigenvalues, v eigenvectors as columns, unsorted. */
 int vti_sym_eigen(int n, const double *a, double *w, double *v);
@@ -329,11 +356,22 @@ typedef struct {
     int ck_every;      /* 0: no fallback checkpoints */
     int max_exact;     /* 0: no exactly kept snapshots */
     int src_box;       /* 8: half-width of the exactly kept box around the source (0: none) */
+    int codec;         /* 0: autoencoder latents; 1: sparse block transform, every 8^3 block
+                          of every stored snapshot within relative L2 error tol of
+                          max(its energy, floor x the strongest block's energy)
+                          (check_every, max_exact unused) */
+    float floor;       /* codec 1: 1e-8 */
+    long long budget;  /* codec 1: bytes for kept coefficients (0: nsnap * n^3, a quarter of the
+                          fp32 snapshots); past it, snapshots are recomputed from the ck_every
+                          checkpoints (or the call fails without them) */
 } vti_learned_opts;
 int vti_learned_defaults(vti_learned_opts *o);
-/* stats (optional, 6 doubles): encode ms, decode ms (host-timed), mean
- * checked relative error, snapshots above tol, snapshots, snapshots kept
- * exactly. */
+/* stats (optional, 8 doubles): encode ms, decode ms (host-timed), mean
+ * checked relative error (codec 1: mean error bound of the stored
+ * snapshots), snapshots above tol (codec 1: over budget), snapshots,
+ * snapshots kept exactly, bytes of the compressed store (latents or
+ * entries + block tables, exactly kept snapshots, source boxes), snapshots
+ * recomputed from checkpoints. */
 int vti_rtm_shot_learned(vti_ctx *ctx, vti_ae *ae, const vti_rtm_opts *opts,
                          const vti_learned_opts *lo, const float *obs, double *stats);
 /* Forward modelling of the current shot; every save_every-th snapshot, the
