@@ -322,16 +322,17 @@ def bench_rtm3d_learned(args, torch, vti):
         prop.set_shot(vti.shot_ref3d(cfg, 0, cx, 0, cy, 0, 1, 0, rec_depth=3, rec_step=2))
         if k < ntrain:
             t0 = time.perf_counter()
-            prop.ae_fit_shot(A, save_every=4, nblocks=2048)
+            prop.ae_fit_shot(A, save_every=4, nblocks=2048, src_box=args.src_box)
             if k == ntrain - 1:
                 explained = A.fit_end()
             t_fit += time.perf_counter() - t0
             log(f"[bench] fit shot {k} done ({time.perf_counter() - t0:.1f} s)")
             continue
         lo = vti.learned_defaults()
+        lo.src_box = args.src_box
         if sparse:
-            # store sized for 2x compression; only the bytes used count
-            lo.codec, lo.tol, lo.floor, lo.budget = 1, args.tol, args.floor, nsave * n ** 3 * 2
+            lo.codec, lo.tol, lo.floor = 1, args.tol, args.floor
+            lo.budget = int(nsave * n ** 3 * 4 * args.store_frac)
         else:
             lo.check_every, lo.tol, lo.ck_every, lo.max_exact = 1, 0.02, 0, max_exact
         prop.rtm_reset()
@@ -378,7 +379,7 @@ def bench_rtm3d_learned(args, torch, vti):
         "config": {"workload": f"3D VTI RTM {n}^3, {nt} steps, dt 0.8 ms, 15 Hz, {ntrain} fitting shots then "
                                f"{len(rows)} shots imaged both ways",
                    "codec": args.codec, "block_tol": args.tol if sparse else None,
-                   "block_floor": args.floor if sparse else None,
+                   "block_floor": args.floor if sparse else None, "src_box": args.src_box,
                    "plain_s_per_shot": round(tp, 3) if rows else None,
                    "learned_s_per_shot": round(tl, 3) if rows else None,
                    "final_image_rel_l2_max": max((r["err"] for r in rows), default=None),
@@ -422,6 +423,11 @@ def main():
                     help="learned RTM: error-bounded sparse block coefficients or autoencoder latents")
     ap.add_argument("--tol", type=float, default=0.01,
                     help="learned RTM, sparse codec: relative L2 error bound of every stored block")
+    ap.add_argument("--src-box", type=int, default=8,
+                    help="learned RTM: half-width of the exactly kept box around the source")
+    ap.add_argument("--store-frac", type=float, default=0.5,
+                    help="learned RTM, sparse codec: store allocated as this fraction of the fp32 snapshots "
+                         "(only the bytes used count)")
     ap.add_argument("--floor", type=float, default=1e-8,
                     help="learned RTM, sparse codec: blocks below this fraction of the strongest block's "
                          "energy are held to tol of that fraction")

@@ -14,7 +14,8 @@
 // to tol of that floor.
 //
 // Voxel order inside a block is natural: k = (ty*8 + tx)*8 + tz.
-// A kept coefficient is one 32-bit entry: bf16 value << 16 | k.  Per block:
+// A kept coefficient is one 32-bit entry: the top 23 bits of its fp32
+// pattern (rounded; relative error <= 2^-15) | k in the low 9 bits.  Per block:
 // the offset of its first entry (uint32, from the snapshot's base) and the
 // number of entries (uint16).
 #pragma once
@@ -80,8 +81,8 @@ hipError_t launch_decode(const Basis &V, const uint32_t *ent, const uint32_t *bo
 
 // Whole encode of one snapshot: kernels, counter read-back (host sync).
 // Every block's relative L2 error (to the floor above) is within tol: the
-// dropped energy plus the bf16 rounding of the kept values (2^-8 of their
-// magnitude), so tol <= 2^-8 keeps every non-zero coefficient.  *kept =
+// dropped energy plus 2^-26 of the budget set aside for the value rounding
+// and the transforms, so tol <= 2^-13 keeps every non-zero coefficient.  *kept =
 // entries needed; when *kept > cap the entries are incomplete (the caller
 // falls back).  *err = the bound on the box's relative L2 error; *energy =
 // the box's energy.

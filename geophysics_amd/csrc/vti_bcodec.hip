@@ -9,8 +9,8 @@
 // float bits of c^2 (31 wave reductions), the largest cut whose dropped
 // energy fits the block's budget, keeps the coefficients above it, places
 // them with a wave prefix sum and one atomic per block.  Decode: the kept
-// values (bf16, exact) are scattered into a zeroed LDS tile and multiplied
-// by V (hi and lo).
+// values (15 significant bits = bf16 hi + lo exactly) are scattered into
+// zeroed LDS tiles and multiplied by V: Ch Vh + Cl Vh + Ch Vl.
 #include "vti_bcodec.h"
 
 #include <algorithm>
@@ -41,6 +41,16 @@ __device__ __forceinline__ uint16_t to_bf(float f)   // round to nearest even
 }
 
 __device__ __forceinline__ float from_bf(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// A kept coefficient as the top 23 bits of its fp32 pattern (sign, exponent,
+// 14 mantissa bits), rounded to nearest even: relative error <= 2^-15.  The
+// low 9 bits of the entry hold the coefficient's index.
+__device__ __forceinline__ uint32_t to_v23(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return u & 0xfffffe00u;
+    return (u + 0xffu + ((u >> 9) & 1u)) & 0xfffffe00u;
+}
 
 __global__ __launch_bounds__(256) void encode_kernel(Basis V, const float *__restrict__ P, long long off,
                                                      long long sy, long long sx, int n, Mask m,
@@ -194,7 +204,7 @@ __global__ __launch_bounds__(256) void pack_kernel(const float *__restrict__ coe
             unsigned pos = base + (unsigned)(incl - n_keep);
 #pragma unroll
             for (int e = 0; e < 8; ++e)
-                if (q[e] >= lo) ent[pos++] = ((uint32_t)to_bf(v[e]) << 16) | (uint32_t)(8 * l + e);
+                if (q[e] >= lo) ent[pos++] = to_v23(v[e]) | (uint32_t)(8 * l + e);
         }
     }
     if (l == 0) {
@@ -214,10 +224,14 @@ __global__ __launch_bounds__(256) void decode_kernel(Basis V, const uint32_t *__
                                                      float *__restrict__ out)
 {
     __shared__ uint16_t ch[BB][LROW];
+    __shared__ uint16_t cl[BB][LROW];
     const int nb = n / 8;
     const long long nblk = (long long)nb * nb * nb;
     const long long b0 = (long long)blockIdx.x * BB;
-    for (int i = threadIdx.x; i < BB * LROW / 2; i += 256) reinterpret_cast<uint32_t *>(&ch[0][0])[i] = 0u;
+    for (int i = threadIdx.x; i < BB * LROW / 2; i += 256) {
+        reinterpret_cast<uint32_t *>(&ch[0][0])[i] = 0u;
+        reinterpret_cast<uint32_t *>(&cl[0][0])[i] = 0u;
+    }
     __syncthreads();
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     for (int i = w; i < BB; i += 4) {
@@ -227,7 +241,10 @@ __global__ __launch_bounds__(256) void decode_kernel(Basis V, const uint32_t *__
         const uint32_t *e = ent + boff[b];
         for (int q = l; q < cnt; q += 64) {
             const uint32_t x = e[q];
-            ch[i][x & 511u] = (uint16_t)(x >> 16);
+            const float f = __uint_as_float(x & 0xfffffe00u);   // = hi + lo exactly (15 significant bits)
+            const uint16_t h = to_bf(f);
+            ch[i][x & 511u] = h;
+            cl[i][x & 511u] = to_bf(f - from_bf(h));
         }
     }
     __syncthreads();
@@ -238,9 +255,12 @@ __global__ __launch_bounds__(256) void decode_kernel(Basis V, const uint32_t *__
         for (int c = 0; c < 8; ++c) acc[r][c] = f4{0.0f, 0.0f, 0.0f, 0.0f};
     for (int kk = 0; kk < 512; kk += 32) {
         const int kc = kk + 8 * (l >> 4);
-        us8 a[2];
+        us8 a[2], al[2];
 #pragma unroll
-        for (int r = 0; r < 2; ++r) a[r] = *reinterpret_cast<const us8 *>(&ch[16 * r + (l & 15)][kc]);
+        for (int r = 0; r < 2; ++r) {
+            a[r] = *reinterpret_cast<const us8 *>(&ch[16 * r + (l & 15)][kc]);
+            al[r] = *reinterpret_cast<const us8 *>(&cl[16 * r + (l & 15)][kc]);
+        }
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             const int k = 128 * w + 16 * c + (l & 15);
@@ -249,6 +269,7 @@ __global__ __launch_bounds__(256) void decode_kernel(Basis V, const uint32_t *__
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
                 acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(a[r]), as_bf(bh), acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(al[r]), as_bf(bh), acc[r][c], 0, 0, 0);
                 acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(a[r]), as_bf(bl), acc[r][c], 0, 0, 0);
             }
         }
@@ -371,9 +392,10 @@ int encode(const Basis &V, const float *P, long long off, long long sy, long lon
     if (n < 8 || n % 8 || !(tol >= 0.0f) || !(flo >= 0.0f)) return VTI_ERR_ARG;
     int st = work_reserve(w, n);
     if (st) return st;
-    // rounding a kept value to bf16 moves it by at most 2^-8 of itself: that
-    // share of every block's budget is set aside for the kept energy
-    const double q2 = std::ldexp(1.0, -16);
+    // a kept value is stored to 2^-15 of itself and the two transforms add
+    // ~2^-16 of the block norm each: 2^-26 of every block's budget is set
+    // aside for them
+    const double q2 = std::ldexp(1.0, -26);
     const double t2 = (double)tol * tol;
     const float tol2 = t2 > q2 ? (float)(t2 - q2) : 0.0f;
     if (launch_encode(V, P, off, sy, sx, n, m, w, s) != hipSuccess ||

@@ -313,11 +313,11 @@ int vti_ae_get_basis(vti_ae *ae, float *V);
  * P[off + y*sy + x*sx + z], n a multiple of 8).  Every 8^3 block is
  * projected on the basis and its smallest coefficients are dropped while
  * the block's L2 error stays <= tol x max(block norm, sqrt(floor) x the
- * strongest block's norm) (Parseval: the dropped energy is the error; the
- * bf16 rounding of the kept values is budgeted as 2^-8 of their magnitude,
- * so tol <= 2^-8 keeps every non-zero coefficient and bounds the error by
- * 2^-8).  Kept coefficients are entries
- * ent[] = bf16 value << 16 | voxel-order index; per block (z fastest, then
+ * strongest block's norm) (Parseval: the dropped energy is the error; 2^-26
+ * of the budget covers the rounding of the kept values and the transforms,
+ * so tol <= 2^-13 keeps every non-zero coefficient).  Kept coefficients are
+ * entries ent[] = top 23 bits of the fp32 value (rounded, relative error
+ * <= 2^-15) | 9-bit voxel-order index; per block (z fastest, then
  * x, then y over the (n/8)^3 blocks) boff = first entry, bcnt = count.
  * *kept = entries needed; when *kept > cap the entries are incomplete
  * (never written past cap).  *err = the bound on the box's relative L2

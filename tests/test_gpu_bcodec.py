@@ -27,8 +27,9 @@ def _unblocks(b, n):
     return b.reshape(nb, nb, nb, 8, 8, 8).transpose(0, 3, 1, 4, 2, 5).reshape(n, n, n)
 
 
-def _bf16_to_f32(h):
-    return (h.astype(np.uint32) << 16).view(np.float32)
+def _entry_value(e):
+    """Top 23 bits of an entry = the kept value's fp32 pattern."""
+    return (e & np.uint32(0xFFFFFE00)).view(np.float32)
 
 
 def _orthonormal(seed):
@@ -74,7 +75,7 @@ def test_codec_error_bound_and_entries(tol, floor):
     """Random orthonormal basis, an FD snapshot: every block's decoded error
     is within tol of max(its norm, sqrt(floor) x the strongest block's), the
     whole box within the reported bound, the entries of a block are its
-    largest coefficients (bf16 of the float64 projection), and the decode
+    largest coefficients (the float64 projection to 2^-15), and the decode
     equals numpy's reconstruction from the same entries."""
     import torch
     V = _orthonormal(11)
@@ -86,14 +87,14 @@ def test_codec_error_bound_and_entries(tol, floor):
         kept, err, out, ent, boff, bcnt = _codec(A, torch, P, N, tol, N ** 3, floor=floor)
     e = rel_l2(out, box)
     _say(f"tol {tol} floor {floor}: kept {kept} of {N ** 3}, bound {err:.4e}, error {e:.4e}")
-    assert e <= err * (1 + 1e-3) + 1e-6
+    assert e <= err * (1 + 1e-3) + 1e-5
     assert int(bcnt.astype(np.int64).sum()) == kept
     xb = _blocks(box)
     eb = (xb ** 2).sum(axis=1)
     ref = np.maximum(eb, floor * eb.max())
-    assert err <= max(tol, 2.0 ** -8) * np.sqrt(ref.sum() / eb.sum()) * (1 + 1e-5)
+    assert err <= max(tol, 2.0 ** -13) * np.sqrt(ref.sum() / eb.sum()) * (1 + 1e-5)
     errb = ((_blocks(out.astype(np.float64)) - xb) ** 2).sum(axis=1)
-    assert np.all(errb <= max(tol * tol, 2.0 ** -16) * ref * 1.01 + 1e-8 * eb + 1e-30)
+    assert np.all(errb <= max(tol * tol, 2.0 ** -26) * ref * 1.01 + 1e-9 * eb + 1e-30)
     # entries against the float64 projection
     c = xb @ V.astype(np.float64)
     bnorm = np.sqrt(eb)
@@ -106,20 +107,20 @@ def test_codec_error_bound_and_entries(tol, floor):
         ee = ent[s0:s0 + k]
         idx = (ee & 511).astype(np.int64)
         assert len(np.unique(idx)) == k
-        vals = _bf16_to_f32((ee >> 16).astype(np.uint16)).astype(np.float64)
+        vals = _entry_value(ee).astype(np.float64)
         dense[b, idx] = vals
         if not k:
             continue
-        # bf16 rounding (<= 2^-8 of the value) plus the transform's own
+        # value rounding (<= 2^-15 of the value) plus the transform's own
         # error, bounded by ~2^-16 of the block's norm (hi/lo splits)
-        dev = np.abs(vals - c[b, idx]) - 2.0 ** -8 * np.abs(c[b, idx])
+        dev = np.abs(vals - c[b, idx]) - 2.0 ** -15 * np.abs(c[b, idx])
         worst = max(worst, dev.max() / max(bnorm[b], 1e-30))
         if k < 512:                          # a magnitude cut, up to the transform's error
             keep = np.zeros(512, bool)
             keep[idx] = True
             slack = 4e-5 * bnorm[b]
             assert np.abs(c[b, keep]).min() + slack >= np.abs(c[b, ~keep]).max() - slack
-    _say(f"  largest coefficient deviation beyond bf16 rounding: {worst:.3e} of the block norm")
+    _say(f"  largest coefficient deviation beyond the value rounding: {worst:.3e} of the block norm")
     assert worst < 4e-5
     ranges.sort()
     for (a0, a1), (b0, _) in zip(ranges, ranges[1:]):
@@ -158,7 +159,7 @@ def _shot(cfg, cx, cy):
 
 def test_sparse_rtm_image_error_and_fallback():
     """Basis fitted on two shots, a third shot imaged from sparse
-    coefficients at tol 1 % and 0.5 %, against the plain RTM image of the same
+    coefficients at tol 1 % and 0.2 %, against the plain RTM image of the same
     steps; a budget below the need with checkpoints recomputes every
     snapshot (bit-identical image), without checkpoints it fails."""
     cfg, vp, e, d = _setup()
@@ -177,7 +178,7 @@ def test_sparse_rtm_image_error_and_fallback():
         img_p, ill_p = prop.rtm_image(finalize=False)
         fin_p, _ = prop.rtm_image(finalize=True)
         res = {}
-        for tol in (0.01, 0.005):
+        for tol in (0.01, 0.002):
             lo = vti.learned_defaults()
             lo.codec, lo.tol = 1, tol
             lo.budget = (NT // 4) * N ** 3 * 4   # room for every coefficient: no recomputation
@@ -189,7 +190,7 @@ def test_sparse_rtm_image_error_and_fallback():
             res[tol] = (rel_l2(img_l, img_p), rel_l2(fin_l, fin_p), st)
             _say(f"tol {tol}: raw image {res[tol][0]:.3e}, final image {res[tol][1]:.3e}, {st}")
             assert st["recomputed"] == 0 and st["snapshots"] == NT // 4
-            assert st["mean_checked_err"] <= max(tol, 2.0 ** -8)
+            assert st["mean_checked_err"] <= tol
         lo = vti.learned_defaults()      # room for one entry: every non-zero snapshot is recomputed
         lo.codec, lo.tol, lo.budget, lo.ck_every = 1, 0.0, 4, 16
         prop.rtm_reset()
@@ -199,7 +200,8 @@ def test_sparse_rtm_image_error_and_fallback():
         prop.rtm_reset()
         with pytest.raises(vti.VtiError):
             prop.rtm_shot_learned(A, prop.rtm_defaults(), lo, None)
-    assert res[0.01][0] < 1e-2 and res[0.01][1] < 2e-2
-    assert res[0.005][0] < res[0.01][0]
+    assert res[0.01][0] < 5e-3 and res[0.01][1] < 3e-2       # measured 1.1e-3, 2.0e-2
+    assert res[0.002][0] < 1e-3 and res[0.002][1] < 1e-2     # measured 2.1e-4, 5.1e-3
+    assert res[0.002][0] < res[0.01][0] and res[0.002][1] < res[0.01][1]
     assert st["recomputed"] > 0
     assert np.array_equal(img_r, img_p), rel_l2(img_r, img_p)
