@@ -211,33 +211,45 @@ def bench_fwi_grad(args, torch, vti):
     v0 = gaussian_filter(vp.astype(np.float64), 15.0, mode="nearest")
     cfg = vti.cfg_ref2d(vti.MODE_GPU_REF, nx, nz, 12, 12, 20, 0, 10, nt, 1000)
     cfg.dx = cfg.dz = dx                           # the reference helpers take integer spacings
-    log(f"[bench] FWI gradient: {nshot} shots, nt {nt}, grid {nx}x{nz}")
-    G = vti.Gradient(cfg, fp64=False)
-    total = np.zeros((nx, nz))
-    misfit = 0.0
-    t_obs = t_grad = 0.0
-    steps = 0
-    for k in range(nshot):
-        sx = 20 + 15 * k
-        if sx >= nx - 20:
-            break
-        shot = vti.shot_ref2d(cfg, 1, sx, 0, 2, rec_depth=2)
-        G.set_model(vp, ep, dl)
-        G.set_shot(shot)
-        t0 = time.perf_counter()
-        dobs = G.forward()
-        t1 = time.perf_counter()
-        G.set_model(v0, ep, dl)
-        r = G.run(dobs=dobs, nsnap=40)
-        t2 = time.perf_counter()
-        t_obs += t1 - t0
-        t_grad += t2 - t1
-        total += r["grad_vp"]
-        misfit += r["misfit"]
-        steps = r["fwd_steps"]
-        log(f"[bench] shot {k}: misfit {r['misfit']:.4e}, {t2 - t1:.2f} s")
-    G.close()
-    done = k + 1
+    nstream = max(1, args.streams)
+    log(f"[bench] FWI gradient: {nshot} shots, nt {nt}, grid {nx}x{nz}, {nstream} concurrent contexts")
+    shots = [20 + 15 * k for k in range(nshot) if 20 + 15 * k < nx - 20]
+    done = len(shots)
+    # one gradient context (own HIP stream) per worker thread; the library
+    # call releases the interpreter lock, so the shots' kernels overlap
+    import threading
+    lock = threading.Lock()
+    acc = {"total": np.zeros((nx, nz)), "misfit": 0.0, "t_obs": 0.0, "t_grad": 0.0, "steps": 0}
+
+    def worker(w):
+        with vti.Gradient(cfg, fp64=False) as G:
+            for k in range(w, done, nstream):
+                shot = vti.shot_ref2d(cfg, 1, shots[k], 0, 2, rec_depth=2)
+                G.set_model(vp, ep, dl)
+                G.set_shot(shot)
+                t0 = time.perf_counter()
+                dobs = G.forward()
+                t1 = time.perf_counter()
+                G.set_model(v0, ep, dl)
+                r = G.run(dobs=dobs, nsnap=40)
+                t2 = time.perf_counter()
+                with lock:
+                    acc["t_obs"] += t1 - t0
+                    acc["t_grad"] += t2 - t1
+                    acc["total"] += r["grad_vp"]
+                    acc["misfit"] += r["misfit"]
+                    acc["steps"] = r["fwd_steps"]
+                log(f"[bench] shot {k}: misfit {r['misfit']:.4e}, {t2 - t1:.2f} s")
+
+    t_wall0 = time.perf_counter()
+    threads = [threading.Thread(target=worker, args=(w,)) for w in range(nstream)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    t_wall = time.perf_counter() - t_wall0
+    misfit, steps = acc["misfit"], acc["steps"]
+    t_obs, t_grad = acc["t_obs"], acc["t_grad"]
     cells = (nx + 40 - 8) * (nz + 40 - 8)
     cpu = None
     if not args.no_cpu:
@@ -267,15 +279,17 @@ def bench_fwi_grad(args, torch, vti):
                          f"same {nt_c}-step case (random adjoint source) against it: rel-L2 {err:.2e}"}
         log(f"[bench] CPU FP64 oracle {t_cpu:.2f} s for {nt_c} steps; GPU FP32 vs FP64 rel-L2 {err:.2e}")
     out = {
-        "metric": "2D VTI FWI gradient time per shot (s), FP32 adjoint with 40 checkpoint slots",
-        "value": round(t_grad / done, 4), "unit": "s/shot", "n_gpus": 1, "steps": nt, "warmup": 0,
-        "ms_per_step": round(t_grad / done / nt * 1e3, 4), "higher_is_better": False, "scaling": "weak",
+        "metric": "2D VTI FWI gradient time per shot (s: wall time / shots), FP32 adjoint with 40 checkpoint slots",
+        "value": round(t_wall / done, 4), "unit": "s/shot", "n_gpus": 1, "steps": nt, "warmup": 0,
+        "ms_per_step": round(t_wall / done / nt * 1e3, 4), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{nx}x{nz} grid, dx {dx} m, {nt} steps, dt 1 ms, {done} shots 15 cells apart, "
                                "layered model with two Gaussian anomalies, start = smoothed (sigma 15)",
                    "shots": done, "forward_steps_per_shot": steps, "misfit": misfit,
                    "observed_data_s_per_shot": round(t_obs / done, 4),
-                   "cells_per_step": cells, "grad_norm": float(np.linalg.norm(total))},
+                   "concurrent_contexts": nstream, "wall_s": round(t_wall, 3),
+                   "gradient_s_per_shot_one_context": round(t_grad / done, 4),
+                   "cells_per_step": cells, "grad_norm": float(np.linalg.norm(acc["total"]))},
         "roofline": None, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -412,6 +426,8 @@ def main():
     ap.add_argument("--workload", default="fd3d", choices=["fd3d", "batch-rtm", "fwi-grad", "rtm3d-learned"],
                     help="fd3d: the headline 3D FD line (default); the others time the batched 2D RTM, "
                          "the 2D FWI gradient and the 3D RTM with learned snapshot compression")
+    ap.add_argument("--streams", type=int, default=4,
+                    help="FWI gradient: shots run concurrently, one context and HIP stream each")
     ap.add_argument("--shots", type=int, default=0, help="shots of the batch / FWI / learned-RTM workloads "
                     "(0: the workload's own count)")
     ap.add_argument("--nt", type=int, default=0, help="time steps of the non-default workloads (0: their own)")
@@ -421,14 +437,14 @@ def main():
                     help="learned RTM, latent codec: fraction of snapshots the exact-storage fallback may keep")
     ap.add_argument("--codec", default="sparse", choices=["sparse", "latent"],
                     help="learned RTM: error-bounded sparse block coefficients or autoencoder latents")
-    ap.add_argument("--tol", type=float, default=0.01,
+    ap.add_argument("--tol", type=float, default=0.0007,
                     help="learned RTM, sparse codec: relative L2 error bound of every stored block")
-    ap.add_argument("--src-box", type=int, default=8,
+    ap.add_argument("--src-box", type=int, default=32,
                     help="learned RTM: half-width of the exactly kept box around the source")
-    ap.add_argument("--store-frac", type=float, default=0.5,
+    ap.add_argument("--store-frac", type=float, default=0.9,
                     help="learned RTM, sparse codec: store allocated as this fraction of the fp32 snapshots "
                          "(only the bytes used count)")
-    ap.add_argument("--floor", type=float, default=1e-8,
+    ap.add_argument("--floor", type=float, default=1e-6,
                     help="learned RTM, sparse codec: blocks below this fraction of the strongest block's "
                          "energy are held to tol of that fraction")
     args = ap.parse_args()
