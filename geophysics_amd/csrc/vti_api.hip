@@ -349,6 +349,8 @@ int vti_destroy(vti_ctx *c)
     hipFree(c->dtap);
     hipFree(c->rec);
     hipFree(c->rec_t);
+    for (hipEvent_t e : c->ev_acc)
+        if (e) hipEventDestroy(e);
     hipFree(c->alt);
     hipFree(c->img);
     hipFree(c->illum);
@@ -635,6 +637,42 @@ int vti_step(vti_ctx *c, int it0, int nsteps)
         HIP_TRY(launch_stress(c->cfg.ndim, stress_args(c, it), c->stream));
         if (c->prof) HIP_TRY(hipEventRecord(e2, c->stream));
     }
+    return VTI_OK;
+}
+
+long long vti_state_bytes(const vti_ctx *c)
+{
+    if (!c) return 0;
+    long long n = 0;
+    for (int k = 0; k < 11; ++k)
+        if (c->f[k]) n += (long long)c->nvol * (long long)sizeof(float);
+    return n;
+}
+
+// The wavefield state (every allocated field of u, v, w, px .. qz, P, Q in
+// that order, padded volumes) to / from a device buffer of vti_state_bytes.
+int vti_save_state(vti_ctx *c, void *dst)
+{
+    if (!c || !dst) return fail(VTI_ERR_ARG, "vti_save_state: null argument");
+    char *p = static_cast<char *>(dst);
+    for (int k = 0; k < 11; ++k)
+        if (c->f[k]) {
+            HIP_TRY(hipMemcpyAsync(p, c->f[k], c->nvol * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+            p += c->nvol * sizeof(float);
+        }
+    return VTI_OK;
+}
+
+int vti_load_state(vti_ctx *c, const void *src)
+{
+    if (!c || !src) return fail(VTI_ERR_ARG, "vti_load_state: null argument");
+    const char *p = static_cast<const char *>(src);
+    for (int k = 0; k < 11; ++k)
+        if (c->f[k]) {
+            HIP_TRY(hipMemcpyAsync(c->f[k], p, c->nvol * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+            p += c->nvol * sizeof(float);
+        }
+    c->alt_valid = false;
     return VTI_OK;
 }
 

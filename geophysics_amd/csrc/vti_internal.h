@@ -137,6 +137,8 @@ struct vti_ctx {
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
+    // ordering of vti_image_accumulate against the other context's stream
+    hipEvent_t ev_acc[2] = {nullptr, nullptr};
 
     // second buffers of u, v, w, P, Q for the fused 3D step (vti_step), which
     // reads one set and writes the other; the pointers in f swap each step

@@ -261,6 +261,35 @@ int vti_rtm_shot(vti_ctx *c, const vti_rtm_opts *o, const float *obs, long long 
     return hip(hipStreamSynchronize(c->stream), "sync");
 }
 
+// One imaging step of a caller-driven RTM (cal_migration
+// Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:679-698, 3D
+// Toa_gpu_3dvti_rtm_adcigs_1orderfunction.cu:508-521; cal_illumination
+// :654-677): img += P_src * P_rcv and illum += P_src^2 + Q_src^2 over the
+// interior (device arrays, z fastest, then x, then y), clamp: an
+// accumulated illumination <= 0 becomes 1 (the 2D reference's form).  Runs
+// on the receiver context's stream after the source context's queued work;
+// the source context's later work waits for it.
+int vti_image_accumulate(vti_ctx *src, vti_ctx *rcv, float *img, float *illum, int clamp)
+{
+    if (!src || !rcv || !img) return fail(VTI_ERR_ARG, "vti_image_accumulate: null argument");
+    const vti_cfg &a = src->cfg, &b = rcv->cfg;
+    if (a.ndim != b.ndim || a.nx != b.nx || a.nz != b.nz || (a.ndim == 3 && a.ny != b.ny) || a.npml != b.npml)
+        return fail(VTI_ERR_ARG, "vti_image_accumulate: the contexts' grids differ");
+    for (hipEvent_t &e : rcv->ev_acc)
+        if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            return fail(VTI_ERR_HIP, "vti_image_accumulate: event");
+    if (src != rcv && (hipEventRecord(rcv->ev_acc[0], src->stream) != hipSuccess ||
+                       hipStreamWaitEvent(rcv->stream, rcv->ev_acc[0], 0) != hipSuccess))
+        return fail(VTI_ERR_HIP, "vti_image_accumulate: ordering after the source context");
+    if (launch_image(rcv, src->f[VTI_F_P], rcv->f[VTI_F_P], img) != hipSuccess ||
+        (illum && launch_illum(rcv, src->f[VTI_F_P], src->f[VTI_F_Q], illum, clamp ? 1 : 0) != hipSuccess))
+        return fail(VTI_ERR_HIP, "vti_image_accumulate: launch");
+    if (src != rcv && (hipEventRecord(rcv->ev_acc[1], rcv->stream) != hipSuccess ||
+                       hipStreamWaitEvent(src->stream, rcv->ev_acc[1], 0) != hipSuccess))
+        return fail(VTI_ERR_HIP, "vti_image_accumulate: ordering of the source context");
+    return VTI_OK;
+}
+
 int vti_rtm_image(vti_ctx *c, int finalize, float *img, float *illum)
 {
     if (!c || !c->img) return fail(VTI_ERR_STATE, "vti_rtm_image: no image");

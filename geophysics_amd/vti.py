@@ -100,6 +100,10 @@ def _declare(L):
         "vti_padded_dims": (C.c_int, [VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "vti_prof_enable": (C.c_int, [VP, C.c_int]),
         "vti_set_fused": (C.c_int, [VP, C.c_int]),
+        "vti_state_bytes": (C.c_longlong, [VP]),
+        "vti_save_state": (C.c_int, [VP, VP]),
+        "vti_load_state": (C.c_int, [VP, VP]),
+        "vti_image_accumulate": (C.c_int, [VP, VP, VP, VP, C.c_int]),
         "vti_prof_read": (C.c_int, [VP, DP, DP, C.POINTER(C.c_int)]),
         "vti_calib_copy": (C.c_int, [VP, VP, C.c_size_t, VP]),
         "vti_sync": (C.c_int, [VP]),
@@ -409,6 +413,24 @@ class Propagator:
         loss = np.zeros(1, np.float32)
         check(lib().vti_ae_train_shot(self.h, ae.h, save_every, nsteps, loss.ctypes.data_as(FP)), "ae_train_shot")
         return float(loss[0])
+
+    def state_bytes(self):
+        return int(lib().vti_state_bytes(self.h))
+
+    def save_state(self, dst_ptr):
+        """Copy the wavefield state to a device buffer of state_bytes() (waits)."""
+        check(lib().vti_save_state(self.h, dst_ptr), "save_state")
+        self.sync()
+
+    def load_state(self, src_ptr):
+        check(lib().vti_load_state(self.h, src_ptr), "load_state")
+        self.sync()
+
+    def image_accumulate(self, src, img_ptr, illum_ptr=None, clamp=False):
+        """img += P(src) * P(self), illum += P(src)^2 + Q(src)^2 on device
+        arrays of the interior (waits)."""
+        check(lib().vti_image_accumulate(src.h, self.h, img_ptr, illum_ptr, 1 if clamp else 0), "image_accumulate")
+        self.sync()
 
     def set_fused(self, on=True):
         """3D: one fused kernel per step, or the separate velocity and stress kernels (default)."""

@@ -411,6 +411,26 @@ int vti_grad_run(vti_grad *g, const double *dobs, const double *resid, int nsnap
                  double *misfit, double *grad_vp, double *grad_src, double *d_out,
                  long long *fwd_steps);
 
+/* Wavefield state for caller-driven checkpointing (no reference
+ * counterpart: the reference re-runs the forward operator,
+ * Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:1423-1437): every allocated field of
+ * u, v, w, px, py, pz, qx, qy, qz, P, Q (padded volumes, in that order) to
+ * or from a device buffer of vti_state_bytes bytes, enqueued on the
+ * context's stream (vti_sync waits). */
+long long vti_state_bytes(const vti_ctx *ctx);
+int vti_save_state(vti_ctx *ctx, void *dst);
+int vti_load_state(vti_ctx *ctx, const void *src);
+/* One imaging step of a caller-driven RTM between a source-side and a
+ * receiver-side context on the same grid (cal_migration
+ * Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:679-698 / 3D
+ * Toa_gpu_3dvti_rtm_adcigs_1orderfunction.cu:508-521, cal_illumination
+ * :654-677): img += P_src * P_rcv, and when illum is not NULL illum +=
+ * P_src^2 + Q_src^2 (clamp: a sum <= 0 becomes 1), over the interior;
+ * img and illum are device arrays (z fastest, then x, then y).  Ordered
+ * after both contexts' queued work; later work of the source context
+ * waits for it. */
+int vti_image_accumulate(vti_ctx *src, vti_ctx *rcv, float *img, float *illum, int clamp);
+
 /* 3D only: vti_step runs each step as one fused kernel (on = 1) or as the
  * velocity and stress kernels of vti_update_vel / vti_update_stress (on =
  * 0, the default and the faster one on MI355X: 4.47 against 7.3 ms per
