@@ -172,7 +172,9 @@ static VelArgs vel_args(vti_ctx *c, int it) { return vel_args_for(c, main_state(
 static StressArgs stress_args(vti_ctx *c, int it)
 {
     float *row = c->nrec > 0 ? c->rec + (size_t)it * c->nrec : nullptr;
-    return stress_args_for(c, main_state(c), it, true, c->cfg.sv != 0, row);
+    StressArgs a = stress_args_for(c, main_state(c), it, true, c->cfg.sv != 0, row);
+    a.unsplit = c->unsplit && c->cfg.ndim == 3 && c->cfg.mode == VTI_MODE_GPU_REF;
+    return a;
 }
 
 extern "C" {
@@ -589,7 +591,7 @@ int vti_step(vti_ctx *c, int it0, int nsteps)
     int st = check_step(c, it0);
     if (st) return st;
     if (nsteps < 0 || it0 + nsteps > c->cfg.nt) return fail(VTI_ERR_ARG, "vti_step: range past nt");
-    if (c->cfg.ndim == 3 && c->fused && nsteps > 0) {
+    if (c->cfg.ndim == 3 && c->fused && !c->unsplit && nsteps > 0) {
         // fused steps: u, v, w, P, Q alternate between c->f and the second
         // buffers; the cells outside the updated region are read by the
         // stencils but never written, so the second buffers are refreshed
@@ -673,6 +675,15 @@ int vti_load_state(vti_ctx *c, const void *src)
             p += c->nvol * sizeof(float);
         }
     c->alt_valid = false;
+    return VTI_OK;
+}
+
+int vti_set_unsplit(vti_ctx *c, int on)
+{
+    if (!c) return fail(VTI_ERR_ARG, "vti_set_unsplit: null ctx");
+    if (on && (c->cfg.ndim != 3 || c->cfg.mode != VTI_MODE_GPU_REF))
+        return fail(VTI_ERR_ARG, "vti_set_unsplit: 3D point-source form only");
+    c->unsplit = on != 0;
     return VTI_OK;
 }
 

@@ -55,6 +55,9 @@ struct StressArgs {
     // receivers: row rz, x = rx0 + k*rdx (k < rnx), y = ry0 + j*rdy (j < rny)
     float *rec;      // row of this step, nrec floats (nullptr: no record)
     int rec_kind, rz, rx0, rdx, rnx, ry0, rdy, rny;
+    // 3D point-source form: unsplit interior update (P, Q directly where every
+    // damping factor is 1; the split fields only in the absorbing layer)
+    int unsplit;
 };
 
 // One fused 3D step (vti_fused3d.h): velocity update into new u, v, w from
@@ -145,6 +148,7 @@ struct vti_ctx {
     float *alt = nullptr;
     float *altf[11] = {};
     bool fused = false;
+    bool unsplit = false;   // vti_step: unsplit interior (3D point-source form)
     bool alt_valid = false;   // altf equals f outside the updated region
 
     // imaging (RTM): interior-sized image and illumination, accumulated over shots

@@ -263,7 +263,11 @@ hipError_t launch_stress(int ndim, const StressArgs &a, hipStream_t s)
     } else {
         int nzt, nt, ntp;
         const int grd = s3d::grid3d(a.g, kYChunk, nzt, nt, ntp);
-        hipLaunchKernelGGL((s3d::stress3d<kYChunk, true>), dim3(grd), dim3(s3d::TZ, s3d::TX), 0, s, a, nzt, nt, ntp);
+        if (a.unsplit && !a.gsrc)
+            hipLaunchKernelGGL((s3d::stress3d<kYChunk, true, true>), dim3(grd), dim3(s3d::TZ, s3d::TX), 0, s, a, nzt,
+                               nt, ntp);
+        else
+            hipLaunchKernelGGL((s3d::stress3d<kYChunk, true>), dim3(grd), dim3(s3d::TZ, s3d::TX), 0, s, a, nzt, nt, ntp);
     }
     return hipGetLastError();
 }

@@ -129,7 +129,13 @@ __global__ __launch_bounds__(256) void vel3d(VelArgs a, int nzt, int ntile, int 
 }
 
 // ------------------------------------------------------------- stress ----
-template <int YCHUNK, bool NT>
+// US (unsplit interior): where every damping factor of the cell is exactly
+// 1 (outside the absorbing layer) the split fields are redundant: P and Q
+// are updated directly, P -= (tx + ty + tz), Q likewise, with the same
+// per-direction increments as the split form, and px .. qz are neither read
+// nor written there (40 instead of 80 bytes per cell).  The absorbing layer
+// keeps the split form.  Point-source (load-time) form only.
+template <int YCHUNK, bool NT, bool US = false>
 __global__ __launch_bounds__(256) void stress3d(StressArgs a, int nzt, int ntile, int ntile_p)
 {
     const Grid g = a.g;
@@ -150,6 +156,7 @@ __global__ __launch_bounds__(256) void stress3d(StressArgs a, int nzt, int ntile
     }
     const float ax1 = a.ax1[ix], ax2 = a.ax2[ix], az1 = a.az1[iz], az2 = a.az2[iz];
     const float dtx = a.dtx, dty = a.dty, dtz = a.dtz;
+    const bool col_int = US && ax1 == 1.0f && ax2 == 1.0f && az1 == 1.0f && az2 == 1.0f;
     float vq[8];   // vq[k] = v at y-4+k
 #pragma unroll
     for (int k = 0; k < 8; ++k) vq[k] = v[col + (long long)(c.y0 - 4 + k) * L];
@@ -162,10 +169,16 @@ __global__ __launch_bounds__(256) void stress3d(StressArgs a, int nzt, int ntile
         for (int k = 0; k < 8; ++k) wz[k] = w[id + (k - 4)];                   // z-4 .. z+3
         const float vv = ld<NT>(a.vp + id);
         float ee = ld<NT>(a.ep + id), dd = ld<NT>(a.de + id);
-        const float opx = ld<NT>(a.px + id), opy = ld<NT>(a.py + id), opz = ld<NT>(a.pz + id);
-        const float oqx = ld<NT>(a.qx + id), oqy = ld<NT>(a.qy + id), oqz = ld<NT>(a.qz + id);
-        const float vn = iy + 1 < c.y1 ? v[id + 4 * L] : 0.0f;
         const float ay1 = a.ay1[iy], ay2 = a.ay2[iy];
+        const bool un = col_int && ay1 == 1.0f && ay2 == 1.0f;
+        float opx = 0.0f, opy = 0.0f, opz = 0.0f, oqx = 0.0f, oqy = 0.0f, oqz = 0.0f, oP = 0.0f, oQ = 0.0f;
+        if (!un) {
+            opx = ld<NT>(a.px + id); opy = ld<NT>(a.py + id); opz = ld<NT>(a.pz + id);
+            oqx = ld<NT>(a.qx + id); oqy = ld<NT>(a.qy + id); oqz = ld<NT>(a.qz + id);
+        } else {
+            oP = ld<NT>(a.P + id); oQ = ld<NT>(a.Q + id);
+        }
+        const float vn = iy + 1 < c.y1 ? v[id + 4 * L] : 0.0f;
         compiler_fence();
         float xx = 0.0f, yy = 0.0f, zz = 0.0f;
 #pragma unroll
@@ -185,27 +198,42 @@ __global__ __launch_bounds__(256) void stress3d(StressArgs a, int nzt, int ntile
         }
         const float e2 = 1.0f + 2.0f * ee;
         const float sd = sqrtf(1.0f + 2.0f * dd);
-        const float px = ax2 * opx - ax1 * vv * vv * e2 * dtx * xx;
-        const float py = ay2 * opy - ay1 * vv * vv * e2 * dty * yy;
-        const float pz = az2 * opz - az1 * vv * vv * sd * dtz * zz;
-        const float qx = ax2 * oqx - ax1 * vv * vv * sd * dtx * xx;
-        const float qy = ay2 * oqy - ay1 * vv * vv * sd * dty * yy;
-        const float qz = az2 * oqz - az1 * vv * vv * dtz * zz;
-        st<NT>(a.px + id, px); st<NT>(a.py + id, py); st<NT>(a.pz + id, pz);
-        st<NT>(a.qx + id, qx); st<NT>(a.qy + id, qy); st<NT>(a.qz + id, qz);
-        float Pn = px + py + pz, Qn = qx + qy + qz;
-        if (src_col && iy == a.gy) {
-            const float s = (float)((double)a.gamp * a.gw[(ix - a.gx + 3) * 7 + (iz - a.gz + 3)]);
-            Pn = px + py + pz + s;
-            Qn = qx + qy + qz + s;
-        }
-        a.P[id] = Pn;
-        a.Q[id] = Qn;
-        if (rk >= 0) {
-            const int j = iy - a.ry0;
-            if (j >= 0 && j % a.rdy == 0 && j / a.rdy < a.rny) {
-                const float r = a.rec_kind == 0 ? Pn : (a.rec_kind == 1 ? Pn + Pn : Pn + Qn);
-                a.rec[rk + (long long)a.rnx * (j / a.rdy)] = r;
+        if (un) {
+            const float aa = vv * vv;
+            const float Pn = oP - (aa * e2 * dtx * xx + aa * e2 * dty * yy + aa * sd * dtz * zz);
+            const float Qn = oQ - (aa * sd * dtx * xx + aa * sd * dty * yy + aa * dtz * zz);
+            st<NT>(a.P + id, Pn);
+            st<NT>(a.Q + id, Qn);
+            if (rk >= 0) {
+                const int j = iy - a.ry0;
+                if (j >= 0 && j % a.rdy == 0 && j / a.rdy < a.rny) {
+                    const float r = a.rec_kind == 0 ? Pn : (a.rec_kind == 1 ? Pn + Pn : Pn + Qn);
+                    a.rec[rk + (long long)a.rnx * (j / a.rdy)] = r;
+                }
+            }
+        } else {
+            const float px = ax2 * opx - ax1 * vv * vv * e2 * dtx * xx;
+            const float py = ay2 * opy - ay1 * vv * vv * e2 * dty * yy;
+            const float pz = az2 * opz - az1 * vv * vv * sd * dtz * zz;
+            const float qx = ax2 * oqx - ax1 * vv * vv * sd * dtx * xx;
+            const float qy = ay2 * oqy - ay1 * vv * vv * sd * dty * yy;
+            const float qz = az2 * oqz - az1 * vv * vv * dtz * zz;
+            st<NT>(a.px + id, px); st<NT>(a.py + id, py); st<NT>(a.pz + id, pz);
+            st<NT>(a.qx + id, qx); st<NT>(a.qy + id, qy); st<NT>(a.qz + id, qz);
+            float Pn = px + py + pz, Qn = qx + qy + qz;
+            if (src_col && iy == a.gy) {
+                const float s = (float)((double)a.gamp * a.gw[(ix - a.gx + 3) * 7 + (iz - a.gz + 3)]);
+                Pn = px + py + pz + s;
+                Qn = qx + qy + qz + s;
+            }
+            a.P[id] = Pn;
+            a.Q[id] = Qn;
+            if (rk >= 0) {
+                const int j = iy - a.ry0;
+                if (j >= 0 && j % a.rdy == 0 && j / a.rdy < a.rny) {
+                    const float r = a.rec_kind == 0 ? Pn : (a.rec_kind == 1 ? Pn + Pn : Pn + Qn);
+                    a.rec[rk + (long long)a.rnx * (j / a.rdy)] = r;
+                }
             }
         }
 #pragma unroll

@@ -100,6 +100,7 @@ def _declare(L):
         "vti_padded_dims": (C.c_int, [VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "vti_prof_enable": (C.c_int, [VP, C.c_int]),
         "vti_set_fused": (C.c_int, [VP, C.c_int]),
+        "vti_set_unsplit": (C.c_int, [VP, C.c_int]),
         "vti_state_bytes": (C.c_longlong, [VP]),
         "vti_save_state": (C.c_int, [VP, VP]),
         "vti_load_state": (C.c_int, [VP, VP]),
@@ -431,6 +432,10 @@ class Propagator:
         arrays of the interior (waits)."""
         check(lib().vti_image_accumulate(src.h, self.h, img_ptr, illum_ptr, 1 if clamp else 0), "image_accumulate")
         self.sync()
+
+    def set_unsplit(self, on=True):
+        """3D point-source form: unsplit interior update (faster; ~1e-6 from the split form)."""
+        check(lib().vti_set_unsplit(self.h, 1 if on else 0), "set_unsplit")
 
     def set_fused(self, on=True):
         """3D: one fused kernel per step, or the separate velocity and stress kernels (default)."""

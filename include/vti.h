@@ -431,6 +431,19 @@ int vti_load_state(vti_ctx *ctx, const void *src);
  * waits for it. */
 int vti_image_accumulate(vti_ctx *src, vti_ctx *rcv, float *img, float *illum, int clamp);
 
+/* 3D point-source form only: vti_step / vti_run / vti_update_stress update
+ * P and Q directly wherever every damping factor of the cell is exactly 1
+ * (outside the absorbing layer), P -= tx + ty + tz and Q likewise with the
+ * split form's per-direction increments, and keep the split fields only in
+ * the absorbing layer (80 -> 40 bytes per interior cell in the stress
+ * update).  Not the reference's arithmetic (update_stress3d
+ * GPU_vti3dfd_kernels.cu:131-201 keeps all six splits everywhere): the
+ * records agree with it to ~1e-6 rel-L2 and are as close to an FP64 run.
+ * Switch before a run (after vti_reset): the interior split fields are not
+ * maintained while it is on.  Default off.  RTM and gradient contexts keep
+ * the split form. */
+int vti_set_unsplit(vti_ctx *ctx, int on);
+
 /* 3D only: vti_step runs each step as one fused kernel (on = 1) or as the
  * velocity and stress kernels of vti_update_vel / vti_update_stress (on =
  * 0, the default and the faster one on MI355X: 4.47 against 7.3 ms per

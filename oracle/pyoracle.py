@@ -22,7 +22,7 @@ class OrcParams(C.Structure):
             "sx", "sy", "sz", "sv", "sv_kind", "sv_R", "sv_r",
             "sv_cx", "sv_cy", "sv_cz",
             "rec_z", "rec_x0", "rec_dx", "rec_nx", "rec_y0", "rec_dy", "rec_ny",
-            "rec_kind")]
+            "rec_kind", "unsplit")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

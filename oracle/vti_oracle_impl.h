@@ -347,6 +347,15 @@ static void FN(stress)(SIM *s, int it, int inject, int sv)
                     REAL vv = vp[id];
                     REAL e2 = 1 + 2 * ee;
                     REAL sd = SQRT_R(1 + 2 * dd);
+                    if (s->p.unsplit && ax1[ix] == 1 && ax2[ix] == 1 && ay1[iy] == 1 && ay2[iy] == 1 &&
+                        az1[iz] == 1 && az2[iz] == 1) {
+                        /* unsplit interior (product option, not the reference):
+                         * P -= sum of the three split increments, Q likewise */
+                        REAL a = vv * vv;
+                        P[id] = P[id] - (a * e2 * dtx * xx + a * e2 * dty * yy + a * sd * dtz * zz);
+                        Q[id] = Q[id] - (a * sd * dtx * xx + a * sd * dty * yy + a * dtz * zz);
+                        continue;
+                    }
                     px[id] = ax2[ix] * px[id] - ax1[ix] * vv * vv * e2 * dtx * xx;
                     py[id] = ay2[iy] * py[id] - ay1[iy] * vv * vv * e2 * dty * yy;
                     pz[id] = az2[iz] * pz[id] - az1[iz] * vv * vv * sd * dtz * zz;
@@ -398,6 +407,19 @@ static int FN(step_ex)(SIM *s, int it, int inject, int sv, REAL *rec)
         /* add_source2d GPU_vti2dfd_kernels.cu:200-206: before update_vel */
         long id = s->p.sz + (long)s->p.sx * s->nnz + (long)s->p.sy * s->nnz * s->nnx * (s->p.ndim == 3);
         REAL a = s->amp[it];
+        if (a != 0 && s->p.unsplit) {
+            /* unsplit interior: the injected value must not persist in P, Q
+             * (in the split form the stress update rebuilds them) */
+            const REAL p0 = s->f[9][id], q0 = s->f[10][id];
+            s->f[9][id] += a;
+            s->f[10][id] += a;
+            FN(vel)(s);
+            s->f[9][id] = p0;
+            s->f[10][id] = q0;
+            FN(stress)(s, it, inject, sv);
+            if (rec) FN(record)(s, rec);
+            return 0;
+        }
         if (a != 0) {
             s->f[9][id] += a;
             s->f[10][id] += a;

@@ -310,3 +310,40 @@ def test_image_accumulate_two_contexts():
     assert np.abs(ref_img).max() > 0
     assert np.array_equal(got_img, ref_img)
     assert np.array_equal(got_ill, ref_ill)
+
+
+def test_3d_unsplit_interior_matches_oracle():
+    """Unsplit interior (vti_set_unsplit): bit for bit the oracle's unsplit
+    mode, and within 1e-5 rel-L2 of the reference's split arithmetic (the
+    oracle's default), records and P; the FP64 error is stated."""
+    nx, ny, nz, npml, nt = 44, 40, 36, 8, 300
+    vp, e, d = smooth_model3d(nx, ny, nz, seed=5)
+    d = np.minimum(d, e)
+    cfg = vti.cfg_ref3d(nx, ny, nz, 10, 10, 10, npml, 1, 25, nt, 500)
+    with vti.Propagator(cfg, vp, e, d) as prop:
+        prop.set_unsplit(True)
+        prop.set_shot(vti.shot_ref3d(cfg, 0, nx // 2, 0, ny // 2, 0, 2, 0, rec_depth=3, rec_step=2))
+        prop.run()
+        rec = prop.record()
+        P = prop.field("P")
+    p = po.params_gpu3d(nx, ny, nz, 10, 10, 10, npml, 1, 25, 0, nx // 2, 0, ny // 2, 0, 2, 0, nt, 500, 3, 2)
+    p.unsplit = 1
+    su = po.Sim(p, vp, e, d, "f32")
+    ref_u = su.run()
+    assert np.array_equal(rec, ref_u), rel_l2(rec, ref_u)
+    assert np.array_equal(P, su.field("P"))
+    p.unsplit = 0
+    ref_s = po.Sim(p, vp, e, d, "f32").run()
+    r64 = po.Sim(p, vp, e, d, "f64").run()
+    e_split = rel_l2(rec, ref_s)
+    _say(f"3D unsplit: rel-L2 vs split FP32 {e_split:.3e}; vs FP64 {rel_l2(rec, r64):.3e} "
+         f"(split FP32 vs FP64 {rel_l2(ref_s, r64):.3e})")
+    assert 0 < e_split <= 1e-5
+
+
+def test_unsplit_rejected_outside_3d_point_source():
+    vp, e, d = layered_model(40, 30, seed=1)
+    cfg = vti.cfg_ref2d(vti.MODE_GPU_REF, 40, 30, 10, 10, 8, 0, 20, 50, 1000)
+    with vti.Propagator(cfg, vp, e, d) as prop:
+        with pytest.raises(vti.VtiError):
+            prop.set_unsplit(True)
