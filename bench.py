@@ -316,6 +316,7 @@ def bench_rtm3d_learned(args, torch, vti):
     torch.cuda.synchronize()
     cfg = vti.cfg_ref3d(n, n, n, 10, 10, 10, npml, 1, 15, nt, 800)
     prop = vti.Propagator(cfg)
+    prop.set_unsplit(not args.split)      # source-side propagation of both paths
     prop.set_model(vp.data_ptr(), eps.data_ptr(), dl.data_ptr(), on_device=True)
     del vp, eps, dl
     torch.cuda.empty_cache()
@@ -393,6 +394,7 @@ def bench_rtm3d_learned(args, torch, vti):
         "config": {"workload": f"3D VTI RTM {n}^3, {nt} steps, dt 0.8 ms, 15 Hz, {ntrain} fitting shots then "
                                f"{len(rows)} shots imaged both ways",
                    "codec": args.codec, "block_tol": args.tol if sparse else None,
+                   "source_side_stress": "split" if args.split else "unsplit interior",
                    "block_floor": args.floor if sparse else None, "src_box": args.src_box,
                    "plain_s_per_shot": round(tp, 3) if rows else None,
                    "learned_s_per_shot": round(tl, 3) if rows else None,

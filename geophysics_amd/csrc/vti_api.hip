@@ -165,6 +165,13 @@ StressArgs stress_args_for(vti_ctx *c, const VtiState &s, int it, bool inject, b
     return a;
 }
 
+StressArgs src_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, float *rec_row)
+{
+    StressArgs a = stress_args_for(c, s, it, true, sv, rec_row);
+    a.unsplit = c->unsplit && c->cfg.ndim == 3 && c->cfg.mode == VTI_MODE_GPU_REF;
+    return a;
+}
+
 }  // namespace vti
 
 static VelArgs vel_args(vti_ctx *c, int it) { return vel_args_for(c, main_state(c), it, true); }
@@ -172,9 +179,7 @@ static VelArgs vel_args(vti_ctx *c, int it) { return vel_args_for(c, main_state(
 static StressArgs stress_args(vti_ctx *c, int it)
 {
     float *row = c->nrec > 0 ? c->rec + (size_t)it * c->nrec : nullptr;
-    StressArgs a = stress_args_for(c, main_state(c), it, true, c->cfg.sv != 0, row);
-    a.unsplit = c->unsplit && c->cfg.ndim == 3 && c->cfg.mode == VTI_MODE_GPU_REF;
-    return a;
+    return src_stress_args(c, main_state(c), it, c->cfg.sv != 0, row);
 }
 
 extern "C" {

@@ -171,4 +171,7 @@ VtiState main_state(vti_ctx *c);
 // update in CPU mode); sv: SV taper; rec_row: receiver row of this step or null
 VelArgs vel_args_for(vti_ctx *c, const VtiState &s, int it, bool inject);
 StressArgs stress_args_for(vti_ctx *c, const VtiState &s, int it, bool inject, bool sv, float *rec_row);
+// Source-side stress arguments (point source injected): the context's
+// unsplit-interior setting applies (3D point-source form).
+StressArgs src_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, float *rec_row);
 }  // namespace vti

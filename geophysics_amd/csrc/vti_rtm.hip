@@ -218,12 +218,12 @@ int vti_rtm_shot(vti_ctx *c, const vti_rtm_opts *o, const float *obs, long long 
     op.restore = [&](int i) { return hip(state_copy(c, W, rc.slots[i]), "restore"); };
     op.step = [&](int it) {
         if (launch_vel(nd, vel_args_for(c, W, it, true), c->stream) != hipSuccess) return fail(VTI_ERR_HIP, "vel");
-        return hip(launch_stress(nd, stress_args_for(c, W, it, true, svf, nullptr), c->stream), "stress");
+        return hip(launch_stress(nd, src_stress_args(c, W, it, svf, nullptr), c->stream), "stress");
     };
     op.first_step = [&](int it) {
         if (launch_vel(nd, vel_args_for(c, W, it, true), c->stream) != hipSuccess) return fail(VTI_ERR_HIP, "vel");
         float *row = c->rec + (size_t)it * nrec;
-        if (launch_stress(nd, stress_args_for(c, W, it, true, svf, row), c->stream) != hipSuccess)
+        if (launch_stress(nd, src_stress_args(c, W, it, svf, row), c->stream) != hipSuccess)
             return fail(VTI_ERR_HIP, "stress");
         return hip(launch_illum(c, W.f[VTI_F_P], W.f[VTI_F_Q], c->illum, clamp), "illum");
     };

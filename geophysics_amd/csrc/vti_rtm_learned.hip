@@ -274,7 +274,7 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
             return st;
         if (launch_vel(3, vel_args_for(c, W, it, true), s) != hipSuccess) return fail(VTI_ERR_HIP, "vel");
         float *row = c->rec + (size_t)it * nrec;
-        if (launch_stress(3, stress_args_for(c, W, it, true, svf, row), s) != hipSuccess) return fail(VTI_ERR_HIP, "stress");
+        if (launch_stress(3, src_stress_args(c, W, it, svf, row), s) != hipSuccess) return fail(VTI_ERR_HIP, "stress");
         if ((st = hip(launch_illum(c, W.f[VTI_F_P], W.f[VTI_F_Q], c->illum, clamp), "illum"))) return st;
         if (it % se) continue;
         const int j = it / se;
@@ -368,7 +368,7 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
             if ((st = hip(state_copy(c, W, ck[ci]), "restore"))) return st;
             for (int work = ci * lo->ck_every; work < target; ++work) {
                 if (launch_vel(3, vel_args_for(c, W, work, true), s) != hipSuccess) return fail(VTI_ERR_HIP, "vel");
-                if (launch_stress(3, stress_args_for(c, W, work, true, svf, nullptr), s) != hipSuccess)
+                if (launch_stress(3, src_stress_args(c, W, work, svf, nullptr), s) != hipSuccess)
                     return fail(VTI_ERR_HIP, "stress");
             }
             if ((st = hip(launch_image(c, W.f[VTI_F_P], G.f[VTI_F_P], c->img), "image"))) return st;
@@ -437,7 +437,7 @@ int vti_ae_fit_shot(vti_ctx *c, vti_ae *A, int save_every, int nblocks, int src_
     if (state_zero(c, W) != hipSuccess) return fail(VTI_ERR_HIP, "zero");
     for (int it = 0; it < k.nt; ++it) {
         if (launch_vel(3, vel_args_for(c, W, it, true), c->stream) != hipSuccess ||
-            launch_stress(3, stress_args_for(c, W, it, true, k.sv != 0, nullptr), c->stream) != hipSuccess)
+            launch_stress(3, src_stress_args(c, W, it, k.sv != 0, nullptr), c->stream) != hipSuccess)
             return fail(VTI_ERR_HIP, "step");
         if (it % save_every) continue;
         if (bx.size())
@@ -466,7 +466,7 @@ int vti_ae_train_shot(vti_ctx *c, vti_ae *A, int save_every, int nsteps, float *
     int cnt = 0;
     for (int it = 0; it < k.nt; ++it) {
         if (launch_vel(3, vel_args_for(c, W, it, true), c->stream) != hipSuccess ||
-            launch_stress(3, stress_args_for(c, W, it, true, k.sv != 0, nullptr), c->stream) != hipSuccess)
+            launch_stress(3, src_stress_args(c, W, it, k.sv != 0, nullptr), c->stream) != hipSuccess)
             return fail(VTI_ERR_HIP, "step");
         if (it % save_every) continue;
         if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(VTI_ERR_HIP, "sync");   // training runs on its own stream

@@ -440,8 +440,9 @@ int vti_image_accumulate(vti_ctx *src, vti_ctx *rcv, float *img, float *illum, i
  * GPU_vti3dfd_kernels.cu:131-201 keeps all six splits everywhere): the
  * records agree with it to ~1e-6 rel-L2 and are as close to an FP64 run.
  * Switch before a run (after vti_reset): the interior split fields are not
- * maintained while it is on.  Default off.  RTM and gradient contexts keep
- * the split form. */
+ * maintained while it is on.  Default off.  vti_rtm_shot and
+ * vti_rtm_shot_learned use it for the source-side propagation (the
+ * receiver side stays split); vti_grad keeps the split form. */
 int vti_set_unsplit(vti_ctx *ctx, int on);
 
 /* 3D only: vti_step runs each step as one fused kernel (on = 1) or as the

@@ -210,14 +210,20 @@ def laplace_adj(img):
 
 
 def rtm(params, vp, eps, dl, sv_fwd=1, sv_bwd=0, inject_z=None, illum_mode=0, image_every=1,
-        obs=None, mute=None, prec="f32"):
+        obs=None, mute=None, prec="f32", unsplit_source=False):
     """One-shot RTM with every forward snapshot kept (no checkpointing).
 
     mute: callable(modelled_record) applied in place before it is used as
-    data when obs is None.  Returns (img, illum, modelled) shaped like the
-    interior ((nx, nz) or (ny, nx, nz)) and (nrec, nt)."""
-    s = Sim(params, vp, eps, dl, prec)
-    g = Sim(params, vp, eps, dl, prec)
+    data when obs is None.  unsplit_source: the source side in the unsplit
+    interior mode (the receiver side stays split).  Returns (img, illum,
+    modelled) shaped like the interior ((nx, nz) or (ny, nx, nz)) and
+    (nrec, nt)."""
+    ps = OrcParams.from_buffer_copy(params)
+    ps.unsplit = 1 if unsplit_source else 0
+    pg = OrcParams.from_buffer_copy(params)
+    pg.unsplit = 0
+    s = Sim(ps, vp, eps, dl, prec)
+    g = Sim(pg, vp, eps, dl, prec)
     dt = s.dtype
     shape = (params.ny, params.nx, params.nz) if params.ndim == 3 else (params.nx, params.nz)
     img = np.zeros(shape, dtype=dt)
