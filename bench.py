@@ -166,6 +166,7 @@ def bench_batch_rtm(args, torch, vti):
         models.append(layered_fault_model(nx, nz, rng))
     log(f"[bench] batch RTM: {nshot} shots, nt {nt}")
     B = vti.Batch(cfgs, shots, models)
+    B.set_unsplit(not args.split)
     shape, ms_tune = B.autotune(steps=min(10, nt))
     cfgd = B.get_config()
     log(f"[bench] autotuned tile {cfgd['tile']} grid_mult {cfgd['grid_mult']}: {ms_tune:.3f} ms/step")
@@ -186,6 +187,7 @@ def bench_batch_rtm(args, torch, vti):
         "config": {"workload": f"{nshot} independent 2D shots, nx in [300,1200], nz in [200,800], dx 10 m, "
                                f"nt {nt}, dt 1 ms, Ricker 10-25 Hz, layered models with a dipping fault",
                    "shots": nshot, "cells_per_step": cells, "forward_steps": fwd, "checkpoint_slots": o.nsnap,
+                   "stress_update": "split" if args.split else "unsplit interior (source side)",
                    "tile": list(cfgd["tile"]), "grid_mult": cfgd["grid_mult"], "seconds": round(secs, 3),
                    "shots_per_s": round(nshot / secs, 3)},
         "roofline": None, "cpu_baseline": None,
@@ -350,10 +352,12 @@ def bench_rtm3d_learned(args, torch, vti):
             lo.budget = int(nsave * n ** 3 * 4 * args.store_frac)
         else:
             lo.check_every, lo.tol, lo.ck_every, lo.max_exact = 1, 0.02, 0, max_exact
+        ro = prop.rtm_defaults()
+        ro.inject_z = npml - 1            # receiver row of Toa_gpu_3dvti_rtm_adcigs_1orderfunction.cu:334-335
         prop.rtm_reset()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        st = prop.rtm_shot_learned(A, prop.rtm_defaults(), lo, None)
+        st = prop.rtm_shot_learned(A, ro, lo, None)
         t_learn = time.perf_counter() - t0
         img_l, _ = prop.rtm_image(finalize=False)
         fin_l, _ = prop.rtm_image(finalize=True)
@@ -362,6 +366,7 @@ def bench_rtm3d_learned(args, torch, vti):
         o = prop.rtm_defaults()
         o.image_every = 4
         o.nsnap = nsnap_equal
+        o.inject_z = npml - 1
         prop.rtm_reset()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -454,8 +459,8 @@ def main():
                          "energy are held to tol of that fraction")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(shots would shard across GPUs; see DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: the library has no multi-GPU path "
+              "(independent shots would shard across GPUs, one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import torch

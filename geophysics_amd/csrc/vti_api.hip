@@ -168,7 +168,16 @@ StressArgs stress_args_for(vti_ctx *c, const VtiState &s, int it, bool inject, b
 StressArgs src_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, float *rec_row)
 {
     StressArgs a = stress_args_for(c, s, it, true, sv, rec_row);
-    a.unsplit = c->unsplit && c->cfg.ndim == 3 && c->cfg.mode == VTI_MODE_GPU_REF;
+    a.unsplit = c->unsplit && c->cfg.mode == VTI_MODE_GPU_REF;
+    return a;
+}
+
+StressArgs rcv_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, int inj_z)
+{
+    StressArgs a = stress_args_for(c, s, it, false, sv, nullptr);
+    const size_t M = (size_t)c->nnmax;
+    const bool damped_row = c->htab[10 * M + inj_z] != 1.0f || c->htab[11 * M + inj_z] != 1.0f;
+    a.unsplit = c->unsplit && c->cfg.mode == VTI_MODE_GPU_REF && damped_row;
     return a;
 }
 
@@ -686,8 +695,8 @@ int vti_load_state(vti_ctx *c, const void *src)
 int vti_set_unsplit(vti_ctx *c, int on)
 {
     if (!c) return fail(VTI_ERR_ARG, "vti_set_unsplit: null ctx");
-    if (on && (c->cfg.ndim != 3 || c->cfg.mode != VTI_MODE_GPU_REF))
-        return fail(VTI_ERR_ARG, "vti_set_unsplit: 3D point-source form only");
+    if (on && c->cfg.mode != VTI_MODE_GPU_REF)
+        return fail(VTI_ERR_ARG, "vti_set_unsplit: point-source (GPU) form only");
     c->unsplit = on != 0;
     return VTI_OK;
 }

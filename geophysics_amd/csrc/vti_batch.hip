@@ -41,6 +41,7 @@ struct BShot {
     const double *tap;
     float *rec;         // [it][nrec]
     int nrec, rz, rx0, rdx, inj_z;
+    int inj_damped;     // the injection row lies in the absorbing layer
     float *img, *illum; // interior nx*nz
     int tile0, ntz, ntx;
 };
@@ -102,11 +103,13 @@ __global__ __launch_bounds__(TZ *TXB) void bvel(const BShot *__restrict__ S, int
 
 // MODE 0: recompute (source); 1: source with record (and, illum != 0, the
 // illumination P*P+Q*Q, clamped to 1 where <= 0 when illum == 2);
-// 2: receiver (imaging with the source P)
+// 2: receiver (imaging with the source P).  unsplit: P and Q updated
+// directly where the cell's damping factors are all 1, the split fields
+// only in the absorbing layer (as stress2d_kernel<true>).
 template <int TZ, int TXB, int MODE>
 __global__ __launch_bounds__(TZ *TXB) void bstress(const BShot *__restrict__ S, int ns, int ntile,
                                                   const int *__restrict__ itp, int sv_on, int image_every,
-                                                  int illum)
+                                                  int illum, int unsplit)
 {
     const int it = *itp;
     for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
@@ -139,12 +142,24 @@ __global__ __launch_bounds__(TZ *TXB) void bstress(const BShot *__restrict__ S, 
         const float *tb = b.tab;
         const float ax1 = tb[6 * b.M + ix], ax2 = tb[7 * b.M + ix];
         const float az1 = tb[10 * b.M + iz], az2 = tb[11 * b.M + iz];
-        const float px = ax2 * F[2][id] - ax1 * vv * vv * e2 * b.dtx * xx;
-        const float pz = az2 * F[3][id] - az1 * vv * vv * sd * b.dtz * zz;
-        const float qx = ax2 * F[4][id] - ax1 * vv * vv * sd * b.dtx * xx;
-        const float qz = az2 * F[5][id] - az1 * vv * vv * b.dtz * zz;
-        F[2][id] = px; F[3][id] = pz; F[4][id] = qx; F[5][id] = qz;
-        const float Pn = px + pz, Qn = qx + qz;
+        float Pn, Qn;
+        // receiver side: only when the records are written on a damped row,
+        // where the split form rebuilds P and Q (the written values do not
+        // persist), as in the single-shot RTM
+        const bool us = unsplit && (MODE != 2 || b.inj_damped);
+        if (us && ax1 == 1.0f && ax2 == 1.0f && az1 == 1.0f && az2 == 1.0f) {
+            const float aa = vv * vv;
+            Pn = F[6][id] - (aa * e2 * b.dtx * xx + aa * sd * b.dtz * zz);
+            Qn = F[7][id] - (aa * sd * b.dtx * xx + aa * b.dtz * zz);
+        } else {
+            const float px = ax2 * F[2][id] - ax1 * vv * vv * e2 * b.dtx * xx;
+            const float pz = az2 * F[3][id] - az1 * vv * vv * sd * b.dtz * zz;
+            const float qx = ax2 * F[4][id] - ax1 * vv * vv * sd * b.dtx * xx;
+            const float qz = az2 * F[5][id] - az1 * vv * vv * b.dtz * zz;
+            F[2][id] = px; F[3][id] = pz; F[4][id] = qx; F[5][id] = qz;
+            Pn = px + pz;
+            Qn = qx + qz;
+        }
         F[6][id] = Pn;
         F[7][id] = Qn;
         if (MODE == 1) {
@@ -238,6 +253,7 @@ struct vti_batch {
     hipGraphExec_t gexec[4] = {};   // forward+record, forward, receiver, forward+record+illumination
     int image_every = 1, illum_mode = 0, sv_fwd = 1, sv_bwd = 0;
     long long fwd_steps = 0;
+    bool unsplit = false;           // source-side unsplit interior (point-source form)
 
     ~vti_batch()
     {
@@ -259,13 +275,14 @@ template <int TZ, int TXB>
 void launch_phase(vti_batch *B, int phase, int sv_on, int grid)
 {
     const dim3 blk(TZ, TXB);
+    const int us = B->unsplit ? 1 : 0;
     switch (phase) {
     case 0: hipLaunchKernelGGL((bvel<TZ, TXB, 0>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, 1); break;
-    case 1: hipLaunchKernelGGL((bstress<TZ, TXB, 0>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 0); break;
-    case 2: hipLaunchKernelGGL((bstress<TZ, TXB, 1>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 0); break;
+    case 1: hipLaunchKernelGGL((bstress<TZ, TXB, 0>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 0, us); break;
+    case 2: hipLaunchKernelGGL((bstress<TZ, TXB, 1>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 0, us); break;
     case 3: hipLaunchKernelGGL((bvel<TZ, TXB, 1>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, 0); break;
-    case 4: hipLaunchKernelGGL((bstress<TZ, TXB, 2>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 0); break;
-    case 5: hipLaunchKernelGGL((bstress<TZ, TXB, 1>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 2); break;
+    case 4: hipLaunchKernelGGL((bstress<TZ, TXB, 2>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 0, us); break;
+    case 5: hipLaunchKernelGGL((bstress<TZ, TXB, 1>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 2, us); break;
     }
 }
 
@@ -504,6 +521,7 @@ int vti_batch_create(int nshot, const vti_cfg *cfgs, const vti_shot *shots, cons
         b.tap = B->taps + tapoff[s];
         b.nrec = sh.rec_nx; b.rz = sh.rec_z; b.rx0 = sh.rec_x0; b.rdx = sh.rec_dx;
         b.inj_z = sh.rec_z;
+        b.inj_damped = htab[10 * (size_t)M + b.inj_z] != 1.0f || htab[11 * (size_t)M + b.inj_z] != 1.0f;
         b.rec = B->recbuf + rec_off;
         rec_off += (size_t)sh.rec_nx * B->nt;
         B->cells += (long long)(b.nnx - 8) * (b.nnz - 8);
@@ -544,6 +562,18 @@ int vti_batch_get_config(vti_batch *B, int *tile_z, int *tile_x, int *grid_mult,
 }
 
 long long vti_batch_cells(const vti_batch *B) { return B ? B->cells : 0; }
+
+int vti_batch_set_unsplit(vti_batch *B, int on)
+{
+    if (!B) return bfail(VTI_ERR_ARG, "vti_batch_set_unsplit: null");
+    if (on)
+        for (const vti_cfg &c : B->cfgs)
+            if (c.mode != VTI_MODE_GPU_REF) return bfail(VTI_ERR_ARG, "vti_batch_set_unsplit: point-source (GPU) form only");
+    B->unsplit = on != 0;
+    for (auto &e : B->gexec)   // captured launches carry the old setting
+        if (e) { (void)hipGraphExecDestroy(e); e = nullptr; }
+    return VTI_OK;
+}
 
 // Forward of all shots over nsteps steps from a zero state (the modelled
 // records); used by the autotuner and the forward-only benchmark.

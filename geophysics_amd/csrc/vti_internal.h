@@ -174,4 +174,9 @@ StressArgs stress_args_for(vti_ctx *c, const VtiState &s, int it, bool inject, b
 // Source-side stress arguments (point source injected): the context's
 // unsplit-interior setting applies (3D point-source form).
 StressArgs src_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, float *rec_row);
+// Receiver-side stress arguments of an RTM whose records are written into
+// P and Q at row inj_z: the unsplit interior applies only when that row is
+// in the damped layer (there the split form rebuilds P and Q, so the written
+// values do not persist, as in the split form everywhere).
+StressArgs rcv_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, int inj_z);
 }  // namespace vti

@@ -74,6 +74,10 @@ __device__ __forceinline__ void taper(const StressArgs &a, int r2, float &ee, fl
     }
 }
 
+// US: the unsplit interior (point-source form): where the cell's damping
+// factors are all exactly 1, P and Q are updated directly and px .. qz are
+// neither read nor written (36 instead of 60 bytes per cell).
+template <bool US>
 __global__ __launch_bounds__(256) void stress2d_kernel(StressArgs a)
 {
     const Grid g = a.g;
@@ -95,18 +99,27 @@ __global__ __launch_bounds__(256) void stress2d_kernel(StressArgs a)
     }
     const float e2 = 1.0f + 2.0f * ee;
     const float sd = sqrtf(1.0f + 2.0f * dd);
-    const float px = a.ax2[ix] * a.px[id] - a.ax1[ix] * vv * vv * e2 * a.dtx * xx;
-    const float pz = a.az2[iz] * a.pz[id] - a.az1[iz] * vv * vv * sd * a.dtz * zz;
-    const float qx = a.ax2[ix] * a.qx[id] - a.ax1[ix] * vv * vv * sd * a.dtx * xx;
-    const float qz = a.az2[iz] * a.qz[id] - a.az1[iz] * vv * vv * a.dtz * zz;
-    a.px[id] = px; a.pz[id] = pz; a.qx[id] = qx; a.qz[id] = qz;
-    float Pn = px + pz, Qn = qx + qz;
-    if (a.gsrc) {   // P = px+pz+s (CPU_vti2dfd_kernels.c:173-174), 7x7 footprint
-        const int dx = ix - a.gx, dz = iz - a.gz;
-        if (dx >= -3 && dx <= 3 && dz >= -3 && dz <= 3) {
-            const float s = (float)((double)a.gamp * a.gw[(dx + 3) * 7 + (dz + 3)]);
-            Pn = px + pz + s;
-            Qn = qx + qz + s;
+    const float ax1 = a.ax1[ix], ax2 = a.ax2[ix], az1 = a.az1[iz], az2 = a.az2[iz];
+    float Pn, Qn;
+    if (US && ax1 == 1.0f && ax2 == 1.0f && az1 == 1.0f && az2 == 1.0f) {
+        const float aa = vv * vv;
+        Pn = a.P[id] - (aa * e2 * a.dtx * xx + aa * sd * a.dtz * zz);
+        Qn = a.Q[id] - (aa * sd * a.dtx * xx + aa * a.dtz * zz);
+    } else {
+        const float px = ax2 * a.px[id] - ax1 * vv * vv * e2 * a.dtx * xx;
+        const float pz = az2 * a.pz[id] - az1 * vv * vv * sd * a.dtz * zz;
+        const float qx = ax2 * a.qx[id] - ax1 * vv * vv * sd * a.dtx * xx;
+        const float qz = az2 * a.qz[id] - az1 * vv * vv * a.dtz * zz;
+        a.px[id] = px; a.pz[id] = pz; a.qx[id] = qx; a.qz[id] = qz;
+        Pn = px + pz;
+        Qn = qx + qz;
+        if (a.gsrc) {   // P = px+pz+s (CPU_vti2dfd_kernels.c:173-174), 7x7 footprint
+            const int dx = ix - a.gx, dz = iz - a.gz;
+            if (dx >= -3 && dx <= 3 && dz >= -3 && dz <= 3) {
+                const float s = (float)((double)a.gamp * a.gw[(dx + 3) * 7 + (dz + 3)]);
+                Pn = px + pz + s;
+                Qn = qx + qz + s;
+            }
         }
     }
     a.P[id] = Pn;
@@ -259,7 +272,8 @@ hipError_t launch_stress(int ndim, const StressArgs &a, hipStream_t s)
 {
     if (ndim == 2) {
         dim3 blk(TZ, TX), grd((a.g.nnz + TZ - 1) / TZ, (a.g.nnx + TX - 1) / TX);
-        hipLaunchKernelGGL(stress2d_kernel, grd, blk, 0, s, a);
+        if (a.unsplit && !a.gsrc) hipLaunchKernelGGL(stress2d_kernel<true>, grd, blk, 0, s, a);
+        else hipLaunchKernelGGL(stress2d_kernel<false>, grd, blk, 0, s, a);
     } else {
         int nzt, nt, ntp;
         const int grd = s3d::grid3d(a.g, kYChunk, nzt, nt, ntp);

@@ -248,7 +248,7 @@ int vti_rtm_shot(vti_ctx *c, const vti_rtm_opts *o, const float *obs, long long 
             return fail(VTI_ERR_HIP, "inject");
         if (launch_vel(nd, vel_args_for(c, rc.g, it, false), c->stream) != hipSuccess)
             return fail(VTI_ERR_HIP, "vel");
-        if (launch_stress(nd, stress_args_for(c, rc.g, it, false, svb, nullptr), c->stream) != hipSuccess)
+        if (launch_stress(nd, rcv_stress_args(c, rc.g, it, svb, inj_z), c->stream) != hipSuccess)
             return fail(VTI_ERR_HIP, "stress");
         if (o->illum_mode == 1 && launch_illum(c, rc.g.f[VTI_F_P], rc.g.f[VTI_F_Q], c->illum, 0) != hipSuccess)
             return fail(VTI_ERR_HIP, "illum");

@@ -101,6 +101,7 @@ def _declare(L):
         "vti_prof_enable": (C.c_int, [VP, C.c_int]),
         "vti_set_fused": (C.c_int, [VP, C.c_int]),
         "vti_set_unsplit": (C.c_int, [VP, C.c_int]),
+        "vti_batch_set_unsplit": (C.c_int, [VP, C.c_int]),
         "vti_state_bytes": (C.c_longlong, [VP]),
         "vti_save_state": (C.c_int, [VP, VP]),
         "vti_load_state": (C.c_int, [VP, VP]),
@@ -434,7 +435,7 @@ class Propagator:
         self.sync()
 
     def set_unsplit(self, on=True):
-        """3D point-source form: unsplit interior update (faster; ~1e-6 from the split form)."""
+        """Point-source form: unsplit interior update (faster; ~1e-6 from the split form)."""
         check(lib().vti_set_unsplit(self.h, 1 if on else 0), "set_unsplit")
 
     def set_fused(self, on=True):
@@ -615,6 +616,10 @@ class Batch:
         v = [C.c_int() for _ in range(4)]
         check(lib().vti_batch_get_config(self.h, *[C.byref(x) for x in v]), "batch_get_config")
         return {"tile": (v[0].value, v[1].value), "grid_mult": v[2].value, "graph_steps": v[3].value}
+
+    def set_unsplit(self, on=True):
+        """Source-side unsplit interior for every shot (point-source form)."""
+        check(lib().vti_batch_set_unsplit(self.h, 1 if on else 0), "batch_set_unsplit")
 
     def autotune(self, steps=4):
         ms = C.c_double()

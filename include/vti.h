@@ -243,6 +243,11 @@ int vti_batch_get_config(vti_batch *b, int *tile_z, int *tile_x, int *grid_mult,
 int vti_batch_autotune(vti_batch *b, int steps, double *best_ms);
 /* Updated cells per step summed over the shots. */
 long long vti_batch_cells(const vti_batch *b);
+/* Every shot's propagation with the unsplit interior, with the same rule as
+ * the single-shot RTM (see vti_set_unsplit): source side always, receiver
+ * side when its records are written on a row of the absorbing layer.
+ * Point-source form only; default off. */
+int vti_batch_set_unsplit(vti_batch *b, int on);
 /* nsteps forward steps of every shot from rest; record: keep the records. */
 int vti_batch_forward(vti_batch *b, int nsteps, int record);
 /* RTM of all shots in lockstep (source illumination, illum_mode 0). */
@@ -431,18 +436,21 @@ int vti_load_state(vti_ctx *ctx, const void *src);
  * waits for it. */
 int vti_image_accumulate(vti_ctx *src, vti_ctx *rcv, float *img, float *illum, int clamp);
 
-/* 3D point-source form only: vti_step / vti_run / vti_update_stress update
- * P and Q directly wherever every damping factor of the cell is exactly 1
- * (outside the absorbing layer), P -= tx + ty + tz and Q likewise with the
- * split form's per-direction increments, and keep the split fields only in
- * the absorbing layer (80 -> 40 bytes per interior cell in the stress
- * update).  Not the reference's arithmetic (update_stress3d
- * GPU_vti3dfd_kernels.cu:131-201 keeps all six splits everywhere): the
- * records agree with it to ~1e-6 rel-L2 and are as close to an FP64 run.
+/* Point-source (GPU) form, 2D and 3D: vti_step / vti_run /
+ * vti_update_stress update P and Q directly wherever every damping factor
+ * of the cell is exactly 1 (outside the absorbing layer), P -= tx + ty + tz
+ * and Q likewise with the split form's per-direction increments, and keep
+ * the split fields only in the absorbing layer (3D: 80 -> 40, 2D: 60 -> 36
+ * bytes per interior cell in the stress update).  Not the reference's
+ * arithmetic (update_stress3d GPU_vti3dfd_kernels.cu:131-201,
+ * update_stress2d GPU_vti2dfd_kernels.cu:240-299 keep every split
+ * everywhere): the records agree with it to ~1e-6 rel-L2 and are as close
+ * to an FP64 run.
  * Switch before a run (after vti_reset): the interior split fields are not
  * maintained while it is on.  Default off.  vti_rtm_shot and
- * vti_rtm_shot_learned use it for the source-side propagation (the
- * receiver side stays split); vti_grad keeps the split form. */
+ * vti_rtm_shot_learned use it for the source-side propagation, and for the
+ * receiver side when the records are written on a row of the absorbing
+ * layer (the 3D reference's npml-1); vti_grad keeps the split form. */
 int vti_set_unsplit(vti_ctx *ctx, int on);
 
 /* 3D only: vti_step runs each step as one fused kernel (on = 1) or as the

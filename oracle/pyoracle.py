@@ -210,18 +210,18 @@ def laplace_adj(img):
 
 
 def rtm(params, vp, eps, dl, sv_fwd=1, sv_bwd=0, inject_z=None, illum_mode=0, image_every=1,
-        obs=None, mute=None, prec="f32", unsplit_source=False):
+        obs=None, mute=None, prec="f32", unsplit_source=False, unsplit_receiver=False):
     """One-shot RTM with every forward snapshot kept (no checkpointing).
 
     mute: callable(modelled_record) applied in place before it is used as
-    data when obs is None.  unsplit_source: the source side in the unsplit
-    interior mode (the receiver side stays split).  Returns (img, illum,
+    data when obs is None.  unsplit_source / unsplit_receiver: the source /
+    receiver simulation in the unsplit interior mode.  Returns (img, illum,
     modelled) shaped like the interior ((nx, nz) or (ny, nx, nz)) and
     (nrec, nt)."""
     ps = OrcParams.from_buffer_copy(params)
     ps.unsplit = 1 if unsplit_source else 0
     pg = OrcParams.from_buffer_copy(params)
-    pg.unsplit = 0
+    pg.unsplit = 1 if unsplit_receiver else 0
     s = Sim(ps, vp, eps, dl, prec)
     g = Sim(pg, vp, eps, dl, prec)
     dt = s.dtype

@@ -46,7 +46,7 @@ typedef struct orc_params {
     int sv_cx, sv_cy, sv_cz; /* taper centre */
     int rec_z, rec_x0, rec_dx, rec_nx, rec_y0, rec_dy, rec_ny;
     int rec_kind;          /* 0 P, 1 P+P, 2 P+Q */
-    int unsplit;           /* 3D point-source form only: the product's unsplit
+    int unsplit;           /* point-source form only: the product's unsplit
                               interior update (not the reference's arithmetic:
                               where every damping factor is exactly 1, P and Q
                               are updated directly and the split fields are not

@@ -323,6 +323,14 @@ static void FN(stress)(SIM *s, int it, int inject, int sv)
                 REAL vv = vp[id];
                 REAL e2 = 1 + 2 * ee;
                 REAL sd = SQRT_R(1 + 2 * dd);
+                if (s->p.unsplit && s->p.src_kind == 0 && ax1[ix] == 1 && ax2[ix] == 1 && az1[iz] == 1 &&
+                    az2[iz] == 1) {
+                    /* unsplit interior (product option, not the reference) */
+                    REAL a = vv * vv;
+                    P[id] = P[id] - (a * e2 * dtx * xx + a * sd * dtz * zz);
+                    Q[id] = Q[id] - (a * sd * dtx * xx + a * dtz * zz);
+                    continue;
+                }
                 px[id] = ax2[ix] * px[id] - ax1[ix] * vv * vv * e2 * dtx * xx;
                 pz[id] = az2[iz] * pz[id] - az1[iz] * vv * vv * sd * dtz * zz;
                 qx[id] = ax2[ix] * qx[id] - ax1[ix] * vv * vv * sd * dtx * xx;

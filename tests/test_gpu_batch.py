@@ -5,6 +5,7 @@ import pytest
 
 from conftest import layered_model, rel_l2
 from geophysics_amd import vti
+from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 
@@ -89,3 +90,44 @@ def test_batch_rtm_modelled_records_with_mute_equal_single_shot():
             prop.rtm_shot(oo, None)
             img, _ = prop.rtm_image(finalize=False)
         assert np.array_equal(imgs[s][0], img), (s, rel_l2(imgs[s][0], img))
+
+
+@pytest.mark.parametrize("graph", [0, 4])
+def test_batch_rtm_unsplit_equals_single_shot_unsplit(graph):
+    """Unsplit interior in the batch (vti_batch_set_unsplit): each shot's
+    image and illumination equal the single-shot RTM with vti_set_unsplit
+    bit for bit; against the FP64 oracle RTM the unsplit image is as close
+    as the split FP32 image (these images correlate noise records, so the
+    split image's own FP32 error is ~1e-5)."""
+    cfgs, shots, models = _batch(nt=220)
+    rng = np.random.default_rng(0)
+    obs = [(rng.standard_normal((s.rec_nx, c.nt)) * 0.1).astype(np.float32) for c, s in zip(cfgs, shots)]
+    with vti.Batch(cfgs, shots, models) as B:
+        o = vti.rtm_defaults(2)
+        o.nsnap = 3
+        B.config(0, 8, graph)
+        B.set_unsplit(True)
+        B.rtm(o, obs)
+        imgs = [B.image(s) for s in range(B.n)]
+    for s, (cfg, shot, m) in enumerate(zip(cfgs, shots, models)):
+        with vti.Propagator(cfg, *m) as prop:
+            prop.set_shot(shot)
+            oo = prop.rtm_defaults()
+            oo.nsnap = 3
+            ref = {}
+            for us in (True, False):
+                prop.set_unsplit(us)
+                prop.rtm_reset()
+                prop.rtm_shot(oo, obs[s])
+                ref[us] = prop.rtm_image(finalize=False)
+        assert np.array_equal(imgs[s][1], ref[True][1]), s
+        assert np.array_equal(imgs[s][0], ref[True][0]), (s, rel_l2(imgs[s][0], ref[True][0]))
+        nx, nz = SIZES[s]                  # the oracle shot of _batch's shot s
+        p = po.params_gpu2d(nx, nz, 10, 10, 10 + s, 1, 15 + 2 * s, 1, nx // 2 - s, 0, 2 + s % 2, cfg.nt, 1000)
+        i32, _, _ = po.rtm(p, *m, sv_fwd=1, sv_bwd=0, illum_mode=0, obs=obs[s])
+        assert np.array_equal(i32, ref[False][0]), s      # the same shot
+        i64, _, _ = po.rtm(p, *m, sv_fwd=1, sv_bwd=0, illum_mode=0, obs=obs[s].astype(np.float64), prec="f64")
+        e_us, e_sp = rel_l2(imgs[s][0], i64), rel_l2(ref[False][0], i64)
+        print(f"shot {s}: unsplit vs FP64 {e_us:.2e}, split vs FP64 {e_sp:.2e}, "
+              f"unsplit vs split {rel_l2(imgs[s][0], ref[False][0]):.2e}", flush=True)
+        assert e_us <= max(1e-5, 1.5 * e_sp), s

@@ -341,9 +341,39 @@ def test_3d_unsplit_interior_matches_oracle():
     assert 0 < e_split <= 1e-5
 
 
-def test_unsplit_rejected_outside_3d_point_source():
+def test_unsplit_rejected_for_the_cpu_form():
+    """The in-stress Gaussian source (CPU form) adds to P: no unsplit update."""
     vp, e, d = layered_model(40, 30, seed=1)
-    cfg = vti.cfg_ref2d(vti.MODE_GPU_REF, 40, 30, 10, 10, 8, 0, 20, 50, 1000)
+    cfg = vti.cfg_ref2d(vti.MODE_CPU_REF, 40, 30, 10, 10, 8, 0, 20, 50, 1000)
     with vti.Propagator(cfg, vp, e, d) as prop:
         with pytest.raises(vti.VtiError):
             prop.set_unsplit(True)
+
+
+@pytest.mark.parametrize("SV", [0, 1])
+def test_2d_unsplit_interior_matches_oracle(SV):
+    """2D unsplit interior: bit for bit the oracle's unsplit mode, within
+    1e-5 of the reference's split arithmetic; the FP64 error is stated."""
+    nx, nz, npd, nt = 120, 90, 20, 800
+    vp, e, d = layered_model(nx, nz, seed=4)
+    d = np.minimum(d, e)
+    cfg = vti.cfg_ref2d(vti.MODE_GPU_REF, nx, nz, 10, 10, npd, SV, 20, nt, 1000)
+    with vti.Propagator(cfg, vp, e, d) as prop:
+        prop.set_unsplit(True)
+        prop.set_shot(vti.shot_ref2d(cfg, 1, 60, 0, 2))
+        prop.run()
+        rec = prop.record()
+        P = prop.field("P")
+    p = po.params_gpu2d(nx, nz, 10, 10, npd, SV, 20, 1, 60, 0, 2, nt, 1000)
+    p.unsplit = 1
+    su = po.Sim(p, vp, e, d, "f32")
+    ref_u = su.run()
+    assert np.array_equal(rec, ref_u), rel_l2(rec, ref_u)
+    assert np.array_equal(P, su.field("P"))
+    p.unsplit = 0
+    ref_s = po.Sim(p, vp, e, d, "f32").run()
+    r64 = po.Sim(p, vp, e, d, "f64").run()
+    e_split = rel_l2(rec, ref_s)
+    _say(f"2D unsplit (SV {SV}): rel-L2 vs split FP32 {e_split:.3e}; vs FP64 {rel_l2(rec, r64):.3e} "
+         f"(split FP32 vs FP64 {rel_l2(ref_s, r64):.3e})")
+    assert 0 < e_split <= 1e-5

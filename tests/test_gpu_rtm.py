@@ -88,10 +88,11 @@ def test_rtm_3d_matches_oracle_bitwise():
     assert np.array_equal(img, rimg), rel_l2(img, rimg)
 
 
-def test_rtm_3d_unsplit_source_side():
-    """3D RTM with the unsplit interior on the source side (vti_set_unsplit):
-    bit for bit the oracle's RTM with an unsplit source simulation, and
-    within 1e-5 rel-L2 of the split RTM image."""
+def test_rtm_3d_unsplit_interior():
+    """3D RTM with the unsplit interior (vti_set_unsplit): the records are
+    written at row npml-1, inside the damped layer, so both sides use it;
+    bit for bit the oracle's RTM with unsplit source and receiver
+    simulations, and within 1e-5 rel-L2 of the split RTM image."""
     nx, ny, nz, npml, nt = 28, 26, 24, 6, 140
     vp, e, d = smooth_model3d(nx, ny, nz, seed=7)
     cfg = vti.cfg_ref3d(nx, ny, nz, 10, 10, 10, npml, 1, 25, nt, 800)
@@ -111,10 +112,10 @@ def test_rtm_3d_unsplit_source_side():
     p = po.params_gpu3d(nx, ny, nz, 10, 10, 10, npml, 1, 25, 0, 14, 0, 13, 0, 1, 0, nt, 800, 1, 2)
     p.rec_kind = 2
     rimg, rill, _ = po.rtm(p, vp, e, d, sv_fwd=1, sv_bwd=0, inject_z=npml - 1, illum_mode=1, obs=obs,
-                           unsplit_source=True)
+                           unsplit_source=True, unsplit_receiver=True)
     assert np.array_equal(ill, rill)
     assert np.array_equal(img, rimg), rel_l2(img, rimg)
     simg, _, _ = po.rtm(p, vp, e, d, sv_fwd=1, sv_bwd=0, inject_z=npml - 1, illum_mode=1, obs=obs)
     err = rel_l2(img, simg)
-    print(f"3D RTM, unsplit source side: image rel-L2 vs split {err:.3e}", flush=True)
+    print(f"3D RTM, unsplit interior: image rel-L2 vs split {err:.3e}", flush=True)
     assert err <= 1e-5
