@@ -172,6 +172,17 @@ StressArgs src_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, float
     return a;
 }
 
+StressArgs with_illum(vti_ctx *c, StressArgs a, int clamp)
+{
+    a.illum = c->illum;
+    a.illum_clamp = clamp;
+    a.inp = c->cfg.npml;
+    a.inx = c->cfg.nx;
+    a.iny = c->cfg.ndim == 3 ? c->cfg.ny : 1;
+    a.inz = c->cfg.nz;
+    return a;
+}
+
 StressArgs rcv_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, int inj_z)
 {
     StressArgs a = stress_args_for(c, s, it, false, sv, nullptr);

@@ -55,9 +55,14 @@ struct StressArgs {
     // receivers: row rz, x = rx0 + k*rdx (k < rnx), y = ry0 + j*rdy (j < rny)
     float *rec;      // row of this step, nrec floats (nullptr: no record)
     int rec_kind, rz, rx0, rdx, rnx, ry0, rdy, rny;
-    // 3D point-source form: unsplit interior update (P, Q directly where every
+    // point-source form: unsplit interior update (P, Q directly where every
     // damping factor is 1; the split fields only in the absorbing layer)
     int unsplit;
+    // source illumination accumulated in the epilogue (RTM first sweep):
+    // illum[interior cell] += P^2 + Q^2, a sum <= 0 set to 1 when illum_clamp;
+    // the interior is inx x iny x inz cells from padded index inp (y only in 3D)
+    float *illum;
+    int illum_clamp, inp, inx, iny, inz;
 };
 
 // One fused 3D step (vti_fused3d.h): velocity update into new u, v, w from
@@ -179,4 +184,6 @@ StressArgs src_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, float
 // in the damped layer (there the split form rebuilds P and Q, so the written
 // values do not persist, as in the split form everywhere).
 StressArgs rcv_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, int inj_z);
+// a with the context's source illumination accumulated in its epilogue
+StressArgs with_illum(vti_ctx *c, StressArgs a, int clamp);
 }  // namespace vti

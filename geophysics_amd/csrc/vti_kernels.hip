@@ -124,6 +124,7 @@ __global__ __launch_bounds__(256) void stress2d_kernel(StressArgs a)
     }
     a.P[id] = Pn;
     a.Q[id] = Qn;
+    illum_add(a, ix, a.inp, iz, Pn, Qn);
     if (a.rec && iz == a.rz) {
         const int k = ix - a.rx0;
         if (k >= 0 && k % a.rdx == 0 && k / a.rdx < a.rnx) {

@@ -223,9 +223,8 @@ int vti_rtm_shot(vti_ctx *c, const vti_rtm_opts *o, const float *obs, long long 
     op.first_step = [&](int it) {
         if (launch_vel(nd, vel_args_for(c, W, it, true), c->stream) != hipSuccess) return fail(VTI_ERR_HIP, "vel");
         float *row = c->rec + (size_t)it * nrec;
-        if (launch_stress(nd, src_stress_args(c, W, it, svf, row), c->stream) != hipSuccess)
-            return fail(VTI_ERR_HIP, "stress");
-        return hip(launch_illum(c, W.f[VTI_F_P], W.f[VTI_F_Q], c->illum, clamp), "illum");
+        return hip(launch_stress(nd, with_illum(c, src_stress_args(c, W, it, svf, row), clamp), c->stream),
+                   "stress");
     };
     op.after_first = [&]() {
         if (obs) {   // observed record [irec][it] replaces the modelled one (:1411-1415)

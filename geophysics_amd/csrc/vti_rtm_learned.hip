@@ -274,8 +274,8 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
             return st;
         if (launch_vel(3, vel_args_for(c, W, it, true), s) != hipSuccess) return fail(VTI_ERR_HIP, "vel");
         float *row = c->rec + (size_t)it * nrec;
-        if (launch_stress(3, src_stress_args(c, W, it, svf, row), s) != hipSuccess) return fail(VTI_ERR_HIP, "stress");
-        if ((st = hip(launch_illum(c, W.f[VTI_F_P], W.f[VTI_F_Q], c->illum, clamp), "illum"))) return st;
+        if (launch_stress(3, with_illum(c, src_stress_args(c, W, it, svf, row), clamp), s) != hipSuccess)
+            return fail(VTI_ERR_HIP, "stress");
         if (it % se) continue;
         const int j = it / se;
         const float *P = W.f[VTI_F_P];

@@ -19,6 +19,21 @@
 #include "vti_internal.h"
 
 namespace vti {
+// Illumination epilogue of a stress update (cal_illumination
+// Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:654-677, 3D
+// Toa_gpu_3dvti_rtm_adcigs_1orderfunction.cu:523-537): the same float
+// operations as a separate pass over the stored P and Q.
+__device__ __forceinline__ void illum_add(const StressArgs &a, int ix, int iy, int iz, float Pn, float Qn)
+{
+    if (!a.illum) return;
+    const int jx = ix - a.inp, jy = iy - a.inp, jz = iz - a.inp;
+    if (jx < 0 || jx >= a.inx || jy < 0 || jy >= a.iny || jz < 0 || jz >= a.inz) return;
+    const long long o = jz + (long long)jx * a.inz + (long long)jy * a.inz * a.inx;
+    float v = a.illum[o] + (Pn * Pn + Qn * Qn);
+    if (a.illum_clamp && v <= 0.0f) v = 1.0f;
+    a.illum[o] = v;
+}
+
 namespace s3d {
 
 __device__ __forceinline__ float cst(int m)
@@ -204,6 +219,7 @@ __global__ __launch_bounds__(256) void stress3d(StressArgs a, int nzt, int ntile
             const float Qn = oQ - (aa * sd * dtx * xx + aa * sd * dty * yy + aa * dtz * zz);
             st<NT>(a.P + id, Pn);
             st<NT>(a.Q + id, Qn);
+            illum_add(a, ix, iy, iz, Pn, Qn);
             if (rk >= 0) {
                 const int j = iy - a.ry0;
                 if (j >= 0 && j % a.rdy == 0 && j / a.rdy < a.rny) {
@@ -228,6 +244,7 @@ __global__ __launch_bounds__(256) void stress3d(StressArgs a, int nzt, int ntile
             }
             a.P[id] = Pn;
             a.Q[id] = Qn;
+            illum_add(a, ix, iy, iz, Pn, Qn);
             if (rk >= 0) {
                 const int j = iy - a.ry0;
                 if (j >= 0 && j % a.rdy == 0 && j / a.rdy < a.rny) {
