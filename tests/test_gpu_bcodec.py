@@ -205,3 +205,32 @@ def test_sparse_rtm_image_error_and_fallback():
     assert res[0.002][0] < res[0.01][0] and res[0.002][1] < res[0.01][1]
     assert st["recomputed"] > 0
     assert np.array_equal(img_r, img_p), rel_l2(img_r, img_p)
+
+
+def test_fitted_basis_keeps_fewer_coefficients_than_untrained_bases():
+    """What the learning gains: the basis fitted on two shots' snapshots
+    needs fewer coefficients for the same per-block error bound (1 %) on a
+    third shot's snapshot than the identity basis (the raw samples) or a
+    random orthonormal basis."""
+    import torch
+    cfg, vp, e, d = _setup()
+    with vti.Propagator(cfg, vp, e, d) as prop, vti.Autoencoder(crop=32, batch=1, seed=5) as A:
+        for cx, cy in ((10, 12), (22, 20)):
+            prop.set_shot(_shot(cfg, cx, cy))
+            prop.ae_fit_shot(A, save_every=4, nblocks=64, src_box=0)
+        A.fit_end()
+        fitted = A.basis()
+        prop.set_shot(_shot(cfg, 16, 9))
+        prop.reset()
+        prop.step(0, 150)
+        P = prop.field("P")
+        kept = {}
+        for name, V in (("fitted", fitted), ("identity", np.eye(512, dtype=np.float32)),
+                        ("random", _orthonormal(3))):
+            A.set_basis(V)
+            k, err, out, *_ = _codec(A, torch, P, N, 0.01, N ** 3)
+            box = P[NPML:NPML + N, NPML:NPML + N, NPML:NPML + N]
+            assert rel_l2(out, box) <= err * (1 + 1e-3) + 1e-6
+            kept[name] = k
+    _say(f"coefficients kept at 1 % per block: {kept}")
+    assert kept["fitted"] < kept["identity"] and kept["fitted"] < kept["random"]
