@@ -14,6 +14,9 @@
  *   2D-vti-fd+rtm-cuda/Toa_gpu2DvtiFdRtmAdcigsLaplace.cu (RTM imaging)
  * Two instantiations: *_f32 (float, reference rounding order, no FMA
  * contraction) and *_f64 (everything in double: the error reference).
+ * One mode is not the reference's arithmetic: orc_params.unsplit restates
+ * the product's optional unsplit-interior update (checker for that mode);
+ * in double it equals the split form to 1e-12 (tests/test_oracle_unsplit.py).
  */
 #ifndef VTI_ORACLE_H
 #define VTI_ORACLE_H
