@@ -564,8 +564,9 @@ def main():
             "cells_updated_per_step": cells, "bytes_per_cell_step_algorithmic": BYTES_STEP,
             "pct_hbm_peak_step": round(100.0 * BYTES_STEP * gcells / HBM_PEAK_GBS, 2),
             "fused_step": fused,
-            "stress_update": "unsplit interior (split fields in the absorbing layer only; records within "
-                             "~1e-6 rel-L2 of the split form)" if unsplit else "split (bit-identical to the oracle)",
+            "stress_update": "reduced split (P, Q only in the interior; split fields of the damped axes only "
+                             "in the absorbing layer; records within ~1e-6 rel-L2 of the split form)"
+                             if unsplit else "split (bit-identical to the oracle)",
             "vel_kernel_ms": None if fused else round(vel_avg * 1e3, 4),
             "stress_kernel_ms": None if fused else round(str_avg * 1e3, 4),
             "fused_kernel_ms": round(str_avg * 1e3, 4) if fused else None,

@@ -55,8 +55,10 @@ struct StressArgs {
     // receivers: row rz, x = rx0 + k*rdx (k < rnx), y = ry0 + j*rdy (j < rny)
     float *rec;      // row of this step, nrec floats (nullptr: no record)
     int rec_kind, rz, rx0, rdx, rnx, ry0, rdy, rny;
-    // point-source form: unsplit interior update (P, Q directly where every
-    // damping factor is 1; the split fields only in the absorbing layer)
+    // point-source form, reduced split (unsplit): in the interior P and Q
+    // are updated directly; in 3D a cell of the absorbing layer keeps the
+    // split fields only of its damped axes (all of them in the corners) and
+    // carries the undamped rest in P, Q.
     int unsplit;
     // source illumination accumulated in the epilogue (RTM first sweep):
     // illum[interior cell] += P^2 + Q^2, a sum <= 0 set to 1 when illum_clamp;
@@ -180,9 +182,11 @@ StressArgs stress_args_for(vti_ctx *c, const VtiState &s, int it, bool inject, b
 // unsplit-interior setting applies (3D point-source form).
 StressArgs src_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, float *rec_row);
 // Receiver-side stress arguments of an RTM whose records are written into
-// P and Q at row inj_z: the unsplit interior applies only when that row is
-// in the damped layer (there the split form rebuilds P and Q, so the written
-// values do not persist, as in the split form everywhere).
+// P and Q at row inj_z: the reduced split applies only when that row is in
+// the damped layer.  There the row's own stress result is never read (the
+// next load overwrites it before the velocity update, and it lies outside
+// the image), so whether the written values persist in P, Q does not
+// matter; an interior row would carry them into the image.
 StressArgs rcv_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, int inj_z);
 // a with the context's source illumination accumulated in its epilogue
 StressArgs with_illum(vti_ctx *c, StressArgs a, int clamp);

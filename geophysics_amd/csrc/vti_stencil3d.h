@@ -144,12 +144,16 @@ __global__ __launch_bounds__(256) void vel3d(VelArgs a, int nzt, int ntile, int 
 }
 
 // ------------------------------------------------------------- stress ----
-// US (unsplit interior): where every damping factor of the cell is exactly
-// 1 (outside the absorbing layer) the split fields are redundant: P and Q
-// are updated directly, P -= (tx + ty + tz), Q likewise, with the same
+// US (reduced split): where every damping factor of the cell is exactly 1
+// (outside the absorbing layer) the split fields are redundant: P and Q are
+// updated directly, P -= (tx + ty + tz), Q likewise, with the same
 // per-direction increments as the split form, and px .. qz are neither read
-// nor written there (40 instead of 80 bytes per cell).  The absorbing layer
-// keeps the split form.  Point-source (load-time) form only.
+// nor written there (40 instead of 80 bytes per cell).  A cell of the
+// absorbing layer with one or two damped axes keeps only those axes' split
+// components and carries the rest in P: P' = (px' + py' + pz') + ((P - px -
+// py - pz) - (tx + ty + tz)), each term present only for the kept (px ..) or
+// the undamped (tx ..) axes (56 or 72 bytes per cell).  Corners (three damped
+// axes) keep the full split form.  Point-source (load-time) form only.
 template <int YCHUNK, bool NT, bool US = false>
 __global__ __launch_bounds__(256) void stress3d(StressArgs a, int nzt, int ntile, int ntile_p)
 {
@@ -171,7 +175,8 @@ __global__ __launch_bounds__(256) void stress3d(StressArgs a, int nzt, int ntile
     }
     const float ax1 = a.ax1[ix], ax2 = a.ax2[ix], az1 = a.az1[iz], az2 = a.az2[iz];
     const float dtx = a.dtx, dty = a.dty, dtz = a.dtz;
-    const bool col_int = US && ax1 == 1.0f && ax2 == 1.0f && az1 == 1.0f && az2 == 1.0f;
+    const bool dxc = !(ax1 == 1.0f && ax2 == 1.0f), dzc = !(az1 == 1.0f && az2 == 1.0f);
+    const bool col_int = US && !dxc && !dzc;
     float vq[8];   // vq[k] = v at y-4+k
 #pragma unroll
     for (int k = 0; k < 8; ++k) vq[k] = v[col + (long long)(c.y0 - 4 + k) * L];
@@ -185,14 +190,16 @@ __global__ __launch_bounds__(256) void stress3d(StressArgs a, int nzt, int ntile
         const float vv = ld<NT>(a.vp + id);
         float ee = ld<NT>(a.ep + id), dd = ld<NT>(a.de + id);
         const float ay1 = a.ay1[iy], ay2 = a.ay2[iy];
-        const bool un = col_int && ay1 == 1.0f && ay2 == 1.0f;
+        const bool dy = !(ay1 == 1.0f && ay2 == 1.0f);
+        const bool un = col_int && !dy;
+        // kept split components per axis; full: P, Q rebuilt from them alone
+        const bool full = !US || (dxc && dy && dzc);
+        const bool fx = full || dxc, fy = full || dy, fz = full || dzc;
         float opx = 0.0f, opy = 0.0f, opz = 0.0f, oqx = 0.0f, oqy = 0.0f, oqz = 0.0f, oP = 0.0f, oQ = 0.0f;
-        if (!un) {
-            opx = ld<NT>(a.px + id); opy = ld<NT>(a.py + id); opz = ld<NT>(a.pz + id);
-            oqx = ld<NT>(a.qx + id); oqy = ld<NT>(a.qy + id); oqz = ld<NT>(a.qz + id);
-        } else {
-            oP = ld<NT>(a.P + id); oQ = ld<NT>(a.Q + id);
-        }
+        if (fx) { opx = ld<NT>(a.px + id); oqx = ld<NT>(a.qx + id); }
+        if (fy) { opy = ld<NT>(a.py + id); oqy = ld<NT>(a.qy + id); }
+        if (fz) { opz = ld<NT>(a.pz + id); oqz = ld<NT>(a.qz + id); }
+        if (!full) { oP = ld<NT>(a.P + id); oQ = ld<NT>(a.Q + id); }
         const float vn = iy + 1 < c.y1 ? v[id + 4 * L] : 0.0f;
         compiler_fence();
         float xx = 0.0f, yy = 0.0f, zz = 0.0f;
@@ -217,6 +224,35 @@ __global__ __launch_bounds__(256) void stress3d(StressArgs a, int nzt, int ntile
             const float aa = vv * vv;
             const float Pn = oP - (aa * e2 * dtx * xx + aa * e2 * dty * yy + aa * sd * dtz * zz);
             const float Qn = oQ - (aa * sd * dtx * xx + aa * sd * dty * yy + aa * dtz * zz);
+            st<NT>(a.P + id, Pn);
+            st<NT>(a.Q + id, Qn);
+            illum_add(a, ix, iy, iz, Pn, Qn);
+            if (rk >= 0) {
+                const int j = iy - a.ry0;
+                if (j >= 0 && j % a.rdy == 0 && j / a.rdy < a.rny) {
+                    const float r = a.rec_kind == 0 ? Pn : (a.rec_kind == 1 ? Pn + Pn : Pn + Qn);
+                    a.rec[rk + (long long)a.rnx * (j / a.rdy)] = r;
+                }
+            }
+        } else if (US && !full) {
+            const float aa = vv * vv;
+            const float px = fx ? ax2 * opx - ax1 * vv * vv * e2 * dtx * xx : 0.0f;
+            const float py = fy ? ay2 * opy - ay1 * vv * vv * e2 * dty * yy : 0.0f;
+            const float pz = fz ? az2 * opz - az1 * vv * vv * sd * dtz * zz : 0.0f;
+            const float qx = fx ? ax2 * oqx - ax1 * vv * vv * sd * dtx * xx : 0.0f;
+            const float qy = fy ? ay2 * oqy - ay1 * vv * vv * sd * dty * yy : 0.0f;
+            const float qz = fz ? az2 * oqz - az1 * vv * vv * dtz * zz : 0.0f;
+            if (fx) { st<NT>(a.px + id, px); st<NT>(a.qx + id, qx); }
+            if (fy) { st<NT>(a.py + id, py); st<NT>(a.qy + id, qy); }
+            if (fz) { st<NT>(a.pz + id, pz); st<NT>(a.qz + id, qz); }
+            const float rp = ((oP - opx) - opy) - opz;   // unkept components are 0
+            const float rq = ((oQ - oqx) - oqy) - oqz;
+            const float tp = ((fx ? 0.0f : aa * e2 * dtx * xx) + (fy ? 0.0f : aa * e2 * dty * yy)) +
+                             (fz ? 0.0f : aa * sd * dtz * zz);
+            const float tq = ((fx ? 0.0f : aa * sd * dtx * xx) + (fy ? 0.0f : aa * sd * dty * yy)) +
+                             (fz ? 0.0f : aa * dtz * zz);
+            const float Pn = ((px + py) + pz) + (rp - tp);
+            const float Qn = ((qx + qy) + qz) + (rq - tq);
             st<NT>(a.P + id, Pn);
             st<NT>(a.Q + id, Qn);
             illum_add(a, ix, iy, iz, Pn, Qn);

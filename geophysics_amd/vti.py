@@ -435,7 +435,9 @@ class Propagator:
         self.sync()
 
     def set_unsplit(self, on=True):
-        """Point-source form: unsplit interior update (faster; ~1e-6 from the split form)."""
+        """Point-source form: P, Q updated directly in the interior and, in 3D, only the
+        damped axes' split fields kept in the absorbing layer (faster; ~1e-6 from the
+        split form)."""
         check(lib().vti_set_unsplit(self.h, 1 if on else 0), "set_unsplit")
 
     def set_fused(self, on=True):

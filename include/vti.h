@@ -439,9 +439,11 @@ int vti_image_accumulate(vti_ctx *src, vti_ctx *rcv, float *img, float *illum, i
 /* Point-source (GPU) form, 2D and 3D: vti_step / vti_run /
  * vti_update_stress update P and Q directly wherever every damping factor
  * of the cell is exactly 1 (outside the absorbing layer), P -= tx + ty + tz
- * and Q likewise with the split form's per-direction increments, and keep
- * the split fields only in the absorbing layer (3D: 80 -> 40, 2D: 60 -> 36
- * bytes per interior cell in the stress update).  Not the reference's
+ * and Q likewise with the split form's per-direction increments (3D: 80 ->
+ * 40, 2D: 60 -> 36 bytes per interior cell in the stress update).  In 3D a
+ * cell of the absorbing layer keeps only the split fields of its damped
+ * axes and carries the rest in P and Q (56 or 72 bytes; corners 80); in 2D
+ * the absorbing layer keeps the split form.  Not the reference's
  * arithmetic (update_stress3d GPU_vti3dfd_kernels.cu:131-201,
  * update_stress2d GPU_vti2dfd_kernels.cu:240-299 keep every split
  * everywhere): the records agree with it to ~1e-6 rel-L2 and are as close

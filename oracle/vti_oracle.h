@@ -15,8 +15,12 @@
  * Two instantiations: *_f32 (float, reference rounding order, no FMA
  * contraction) and *_f64 (everything in double: the error reference).
  * One mode is not the reference's arithmetic: orc_params.unsplit restates
- * the product's optional unsplit-interior update (checker for that mode);
- * in double it equals the split form to 1e-12 (tests/test_oracle_unsplit.py).
+ * the product's optional reduced-split update (checker for that mode): in
+ * 3D, a cell keeps the split fields only of its damped axes (none in the
+ * interior, all in the corners of the absorbing layer) and carries the
+ * undamped rest in P, Q; in 2D only
+ * the interior is unsplit.  In double it equals the split form to 1e-12
+ * (tests/test_oracle_unsplit.py).
  */
 #ifndef VTI_ORACLE_H
 #define VTI_ORACLE_H

@@ -355,13 +355,35 @@ static void FN(stress)(SIM *s, int it, int inject, int sv)
                     REAL vv = vp[id];
                     REAL e2 = 1 + 2 * ee;
                     REAL sd = SQRT_R(1 + 2 * dd);
-                    if (s->p.unsplit && ax1[ix] == 1 && ax2[ix] == 1 && ay1[iy] == 1 && ay2[iy] == 1 &&
-                        az1[iz] == 1 && az2[iz] == 1) {
+                    const int dX = !(ax1[ix] == 1 && ax2[ix] == 1), dY = !(ay1[iy] == 1 && ay2[iy] == 1);
+                    const int dZ = !(az1[iz] == 1 && az2[iz] == 1);
+                    if (s->p.unsplit && dX + dY + dZ == 0) {
                         /* unsplit interior (product option, not the reference):
                          * P -= sum of the three split increments, Q likewise */
                         REAL a = vv * vv;
                         P[id] = P[id] - (a * e2 * dtx * xx + a * e2 * dty * yy + a * sd * dtz * zz);
                         Q[id] = Q[id] - (a * sd * dtx * xx + a * sd * dty * yy + a * dtz * zz);
+                        continue;
+                    }
+                    if (s->p.unsplit && dX + dY + dZ < 3) {
+                        /* one or two damped axes: their split components are
+                         * kept, the undamped rest is P minus them */
+                        REAL a = vv * vv, z0 = 0;
+                        REAL npx = dX ? ax2[ix] * px[id] - ax1[ix] * vv * vv * e2 * dtx * xx : z0;
+                        REAL npy = dY ? ay2[iy] * py[id] - ay1[iy] * vv * vv * e2 * dty * yy : z0;
+                        REAL npz = dZ ? az2[iz] * pz[id] - az1[iz] * vv * vv * sd * dtz * zz : z0;
+                        REAL nqx = dX ? ax2[ix] * qx[id] - ax1[ix] * vv * vv * sd * dtx * xx : z0;
+                        REAL nqy = dY ? ay2[iy] * qy[id] - ay1[iy] * vv * vv * sd * dty * yy : z0;
+                        REAL nqz = dZ ? az2[iz] * qz[id] - az1[iz] * vv * vv * dtz * zz : z0;
+                        REAL rp = ((P[id] - (dX ? px[id] : z0)) - (dY ? py[id] : z0)) - (dZ ? pz[id] : z0);
+                        REAL rq = ((Q[id] - (dX ? qx[id] : z0)) - (dY ? qy[id] : z0)) - (dZ ? qz[id] : z0);
+                        REAL tp = ((dX ? z0 : a * e2 * dtx * xx) + (dY ? z0 : a * e2 * dty * yy)) + (dZ ? z0 : a * sd * dtz * zz);
+                        REAL tq = ((dX ? z0 : a * sd * dtx * xx) + (dY ? z0 : a * sd * dty * yy)) + (dZ ? z0 : a * dtz * zz);
+                        if (dX) px[id] = npx, qx[id] = nqx;
+                        if (dY) py[id] = npy, qy[id] = nqy;
+                        if (dZ) pz[id] = npz, qz[id] = nqz;
+                        P[id] = ((npx + npy) + npz) + (rp - tp);
+                        Q[id] = ((nqx + nqy) + nqz) + (rq - tq);
                         continue;
                     }
                     px[id] = ax2[ix] * px[id] - ax1[ix] * vv * vv * e2 * dtx * xx;
