@@ -412,8 +412,8 @@ def bench_rtm3d_learned(args, torch, vti):
                    "compression_vs_fp32_snapshots": round(nsave * n ** 3 * 4 / mean("store"), 2) if rows else None,
                    "fit_s": round(t_fit, 2), "fit_energy_captured_64": explained, "snapshots": nsave,
                    "exact_snapshots_mean": mean("st", "exact"), "recomputed_snapshots_mean": mean("st", "recomputed"),
-                   "encode_ms_per_shot": round(mean("st", "encode_ms"), 1) if rows else None,
-                   "decode_ms_per_shot": round(mean("st", "decode_ms"), 1) if rows else None},
+                   "encode_gpu_ms_per_shot": round(mean("st", "encode_ms"), 1) if rows else None,
+                   "decode_gpu_ms_per_shot": round(mean("st", "decode_ms"), 1) if rows else None},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

@@ -23,7 +23,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -145,10 +144,53 @@ struct Buf {
     ~Buf() { (void)hipFree(p); }
 };
 
-double now_ms()
-{
-    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
+// GPU time of bracketed stream sections, summed without stalling the
+// stream: a ring of event pairs, each pair read when its slot is reused
+// (long complete by then) or by total() after the stream is synchronised.
+struct EvRing {
+    static constexpr int K = 8;
+    hipEvent_t a[K] = {}, b[K] = {};
+    bool live[K] = {};
+    int k = 0;
+    double ms = 0.0;
+    bool ok = true;
+    EvRing()
+    {
+        for (int i = 0; i < K; ++i)
+            if (hipEventCreate(&a[i]) != hipSuccess || hipEventCreate(&b[i]) != hipSuccess) ok = false;
+    }
+    ~EvRing()
+    {
+        for (int i = 0; i < K; ++i) {
+            if (a[i]) (void)hipEventDestroy(a[i]);
+            if (b[i]) (void)hipEventDestroy(b[i]);
+        }
+    }
+    void take(int i)
+    {
+        float t = 0.0f;
+        if (live[i] && hipEventSynchronize(b[i]) == hipSuccess && hipEventElapsedTime(&t, a[i], b[i]) == hipSuccess)
+            ms += t;
+        live[i] = false;
+    }
+    hipError_t begin(hipStream_t s)
+    {
+        take(k);
+        return hipEventRecord(a[k], s);
+    }
+    hipError_t end(hipStream_t s)
+    {
+        live[k] = true;
+        const hipError_t e = hipEventRecord(b[k], s);
+        k = (k + 1) % K;
+        return e;
+    }
+    double total()
+    {
+        for (int i = 0; i < K; ++i) take(i);
+        return ms;
+    }
+};
 
 }  // namespace
 
@@ -264,7 +306,9 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
     const int clamp = o->illum_mode == 0;
     auto hip = [](hipError_t e, const char *w) { return e == hipSuccess ? VTI_OK : fail(VTI_ERR_HIP, w); };
     std::vector<char> bad(nsave, 0);
-    double t_enc = 0.0, t_dec = 0.0, err_sum = 0.0;
+    EvRing t_enc, t_dec;   // the codec's GPU time (encode: up to its host synchronisation)
+    if (!t_enc.ok || !t_dec.ok) return fail(VTI_ERR_HIP, "vti_rtm_shot_learned: events");
+    double err_sum = 0.0;
     int nchecked = 0, nbad = 0;
 
     // ---- first sweep
@@ -279,7 +323,7 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         if (it % se) continue;
         const int j = it / se;
         const float *P = W.f[VTI_F_P];
-        const double t0 = now_ms();
+        if ((st = hip(t_enc.begin(s), "event"))) return st;
         if (sparse) {   // error-bounded sparse coefficients; the source box exactly
             if (box_n)
                 hipLaunchKernelGGL(box_save_kernel, dim3(64), dim3(256), 0, s, c->g, np, P, bx, BX + (size_t)j * box_n);
@@ -289,7 +333,7 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
             if ((st = ae_bc_encode(A, P, off, c->g.ldxy, c->g.ldz, n, m, lo->tol, lo->floor, ENT + used, cap - used,
                                    BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, &need, &e, nullptr, s)))
                 return st;
-            t_enc += now_ms() - t0;
+            if ((st = hip(t_enc.end(s), "event"))) return st;
             if (need > cap - used) {   // over budget: recompute this snapshot from a checkpoint
                 if (!nck) return fail(VTI_ERR_ALLOC, "vti_rtm_shot_learned: sparse store budget exhausted (set ck_every)");
                 bad[j] = 1;
@@ -310,7 +354,7 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
             st = ae_encode(A, P, off, c->g.ldxy, c->g.ldz, n, L + (size_t)j * lat_n, Sc + (size_t)j * 16, s);
         }
         if (st) return st;
-        t_enc += now_ms() - t0;
+        if ((st = hip(t_enc.end(s), "event"))) return st;
         if (lo->check_every > 0 && j % lo->check_every == 0) {
             if ((st = ae_decode(A, L + (size_t)j * lat_n, Sc + (size_t)j * 16, n, D2, s))) return st;
             if (box_n)
@@ -373,13 +417,13 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
             }
             if ((st = hip(launch_image(c, W.f[VTI_F_P], G.f[VTI_F_P], c->img), "image"))) return st;
         } else {
-            const double t0 = now_ms();
+            if ((st = hip(t_dec.begin(s), "event"))) return st;
             if (sparse) st = ae_bc_decode(A, ENT + ebase[j], BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, n, D, s);
             else st = ae_decode(A, L + (size_t)j * lat_n, Sc + (size_t)j * 16, n, D, s);
             if (st) return st;
             if (box_n)
                 hipLaunchKernelGGL(box_restore_kernel, dim3(64), dim3(256), 0, s, n, bx, BX + (size_t)j * box_n, D);
-            t_dec += now_ms() - t0;
+            if ((st = hip(t_dec.end(s), "event"))) return st;
             hipLaunchKernelGGL(image_dense_kernel, grd, blk, 0, s, c->g, np, k.nx, k.ny, k.nz, D, G.f[VTI_F_P], c->img);
         }
         if ((st = hip(hipGetLastError(), "image"))) return st;
@@ -387,8 +431,8 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
     c->rtm_illum_mode = o->illum_mode;
     if ((st = hip(hipStreamSynchronize(s), "sync"))) return st;
     if (stats) {
-        stats[0] = t_enc;
-        stats[1] = t_dec;
+        stats[0] = t_enc.total();
+        stats[1] = t_dec.total();
         stats[2] = nchecked ? err_sum / nchecked : -1.0;
         stats[3] = nbad;
         stats[4] = nsave;

@@ -371,7 +371,9 @@ typedef struct {
                           checkpoints (or the call fails without them) */
 } vti_learned_opts;
 int vti_learned_defaults(vti_learned_opts *o);
-/* stats (optional, 8 doubles): encode ms, decode ms (host-timed), mean
+/* stats (optional, 8 doubles): encode ms, decode ms (GPU time between
+ * HIP events around the codec's launches; the encode's up to the host
+ * synchronisation it needs to place the entries), mean
  * checked relative error (codec 1: mean error bound of the stored
  * snapshots), snapshots above tol (codec 1: over budget), snapshots,
  * snapshots kept exactly, bytes of the compressed store (latents or

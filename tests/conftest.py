@@ -16,15 +16,14 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: longer CPU oracle runs")
     # Tests that hand torch device buffers to libvti.so need torch's HIP
     # runtime initialised before the library is loaded (one runtime per
-    # process, the one torch ships), as bench.py does; on a machine without
-    # a device this is a no-op.
-    if "gpu" in (config.getoption("-m", default="") or "") and "not gpu" not in config.getoption("-m", default=""):
-        try:
-            import torch
-            if torch.cuda.is_available():
-                torch.cuda.init()
-        except Exception:
-            pass
+    # process, the one torch ships), as bench.py does, whichever tests are
+    # selected; on a machine without a device this is a no-op.
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except Exception:
+        pass
 
 
 def rel_l2(a, b):
