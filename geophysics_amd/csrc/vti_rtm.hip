@@ -247,10 +247,11 @@ int vti_rtm_shot(vti_ctx *c, const vti_rtm_opts *o, const float *obs, long long 
             return fail(VTI_ERR_HIP, "inject");
         if (launch_vel(nd, vel_args_for(c, rc.g, it, false), c->stream) != hipSuccess)
             return fail(VTI_ERR_HIP, "vel");
-        if (launch_stress(nd, rcv_stress_args(c, rc.g, it, svb, inj_z), c->stream) != hipSuccess)
-            return fail(VTI_ERR_HIP, "stress");
-        if (o->illum_mode == 1 && launch_illum(c, rc.g.f[VTI_F_P], rc.g.f[VTI_F_Q], c->illum, 0) != hipSuccess)
-            return fail(VTI_ERR_HIP, "illum");
+        // illumination mode 1 adds the receiver side's P^2 + Q^2 in the
+        // stress update's epilogue (same float operations as a separate pass)
+        StressArgs ra = rcv_stress_args(c, rc.g, it, svb, inj_z);
+        if (o->illum_mode == 1) ra = with_illum(c, ra, 0);
+        if (launch_stress(nd, ra, c->stream) != hipSuccess) return fail(VTI_ERR_HIP, "stress");
         if (it % o->image_every == 0)
             return hip(launch_image(c, W.f[VTI_F_P], rc.g.f[VTI_F_P], c->img), "image");
         return VTI_OK;

@@ -397,9 +397,9 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
     for (int it = nt - 1; it >= 0; --it) {
         if (launch_inject_rec(c, G, c->rec + (size_t)it * nrec, inj_z) != hipSuccess) return fail(VTI_ERR_HIP, "inject");
         if (launch_vel(3, vel_args_for(c, G, it, false), s) != hipSuccess) return fail(VTI_ERR_HIP, "vel");
-        if (launch_stress(3, rcv_stress_args(c, G, it, svb, inj_z), s) != hipSuccess) return fail(VTI_ERR_HIP, "stress");
-        if (o->illum_mode == 1 && launch_illum(c, G.f[VTI_F_P], G.f[VTI_F_Q], c->illum, 0) != hipSuccess)
-            return fail(VTI_ERR_HIP, "illum");
+        StressArgs ra = rcv_stress_args(c, G, it, svb, inj_z);
+        if (o->illum_mode == 1) ra = with_illum(c, ra, 0);   // receiver illumination in the epilogue
+        if (launch_stress(3, ra, s) != hipSuccess) return fail(VTI_ERR_HIP, "stress");
         if (it % se) continue;
         const int j = it / se;
         const dim3 blk(64), grd((k.nz + 63) / 64, k.nx, k.ny);

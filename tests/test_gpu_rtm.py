@@ -45,6 +45,30 @@ def test_rtm_2d_matches_oracle_bitwise(nsnap):
         assert steps > nt
 
 
+def test_rtm_2d_receiver_illumination_bitwise():
+    """Illumination mode 1 (source and receiver sides summed, no clamp): the
+    receiver side's P^2 + Q^2 is added in the stress update's epilogue; bit
+    for bit the oracle's separate pass."""
+    nx, nz, npd, nt = 72, 60, 12, 300
+    vp, e, d = layered_model(nx, nz, seed=8, elo=0.1, ehi=0.2, dlo=0.0, dhi=0.1)
+    cfg = vti.cfg_ref2d(vti.MODE_GPU_REF, nx, nz, 10, 10, npd, 0, 20, nt, 1000)
+    rng = np.random.default_rng(4)
+    obs = (rng.standard_normal((nx, nt)) * 0.1).astype(np.float32)
+    with vti.Propagator(cfg, vp, e, d) as prop:
+        prop.set_shot(vti.shot_ref2d(cfg, 1, 36, 0, 1))
+        o = prop.rtm_defaults()
+        o.nsnap = 3
+        o.illum_mode = 1
+        prop.rtm_reset()
+        prop.rtm_shot(o, obs)
+        img, ill = prop.rtm_image(finalize=False)
+    p = po.params_gpu2d(nx, nz, 10, 10, npd, 0, 20, 1, 36, 0, 1, nt, 1000)
+    rimg, rill, _ = po.rtm(p, vp, e, d, sv_fwd=0, sv_bwd=0, illum_mode=1, obs=obs)
+    assert np.abs(rill).max() > 0
+    assert np.array_equal(ill, rill), rel_l2(ill, rill)
+    assert np.array_equal(img, rimg), rel_l2(img, rimg)
+
+
 def test_rtm_2d_modelled_record_with_mute():
     """obs=None: the modelled record, muted with the windowed form (tt=20)."""
     nx, nz, npd, nt = 80, 60, 12, 350
