@@ -8,7 +8,8 @@
 // workgroup).  Pack: one wavefront per block finds, by bisection on the
 // float bits of c^2 (31 wave reductions), the largest cut whose dropped
 // energy fits the block's budget, keeps the coefficients above it, places
-// them with a wave prefix sum and one atomic per block.  Decode: the kept
+// them with a wave prefix sum, a prefix over the workgroup's 8 blocks and
+// one atomic per workgroup.  Decode: the kept
 // values (15 significant bits = bf16 hi + lo exactly) are scattered into
 // zeroed LDS tiles and multiplied by V: Ch Vh + Cl Vh + Ch Vl.
 #include "vti_bcodec.h"
@@ -144,18 +145,28 @@ __device__ __forceinline__ float wave_sum(float x)
     return __shfl(x, 0, 64);
 }
 
-__global__ __launch_bounds__(256) void pack_kernel(const float *__restrict__ coef, long long nblk, float tol2,
-                                                   float flo, uint32_t *ent, unsigned cap, uint32_t *boff,
-                                                   uint16_t *bcnt, Counters *cnt)
+constexpr int PW = 8;   // blocks (wavefronts) per pack workgroup
+
+// One wavefront per block, PW blocks per workgroup; the workgroup reserves
+// its entries with one atomic on the cursor (same-address atomics serialise
+// in L2, so one per block would dominate the kernel) and hands each block
+// its range from a prefix over the workgroup's block totals.
+__global__ __launch_bounds__(64 * PW) void pack_kernel(const float *__restrict__ coef, long long nblk, float tol2,
+                                                       float flo, uint32_t *ent, unsigned cap, uint32_t *boff,
+                                                       uint16_t *bcnt, Counters *cnt)
 {
-    __shared__ float sd[4], se[4];
-    const long long b = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    __shared__ float sd[PW], se[PW];
+    __shared__ unsigned tot[PW], pre[PW];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const long long b = (long long)blockIdx.x * PW + w;
+    const bool live = b < nblk;   // wavefront-uniform
     float dropped = 0.0f, eblk = 0.0f;
-    if (b < nblk) {   // wavefront-uniform
+    float v[8];
+    unsigned q[8];
+    unsigned lo = 0u;
+    int n_keep = 0, incl = 0, total = 0;
+    if (live) {
         const float *c = coef + b * 512 + 8 * l;
-        float v[8];
-        unsigned q[8];
         float e2 = 0.0f;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
@@ -167,7 +178,7 @@ __global__ __launch_bounds__(256) void pack_kernel(const float *__restrict__ coe
         const float emax = __uint_as_float(cnt->emax_bits);
         const float budget = tol2 * fmaxf(eblk, flo * emax);
         // largest cut t with sum of c^2 over bits(c^2) < t within budget
-        unsigned lo = 0u, hi = 0x7f800001u;
+        unsigned hi = 0x7f800001u;
         while (hi - lo > 1u) {
             const unsigned mid = lo + (hi - lo) / 2u;
             float part = 0.0f;
@@ -176,7 +187,6 @@ __global__ __launch_bounds__(256) void pack_kernel(const float *__restrict__ coe
             if (wave_sum(part) <= budget) lo = mid;
             else hi = mid;
         }
-        int n_keep = 0;
         float part = 0.0f;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
@@ -184,16 +194,37 @@ __global__ __launch_bounds__(256) void pack_kernel(const float *__restrict__ coe
             part += q[e] < lo ? __uint_as_float(q[e]) : 0.0f;
         }
         dropped = wave_sum(part);
-        int incl = n_keep;
+        incl = n_keep;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
             const int t = __shfl_up(incl, d, 64);
             if (l >= d) incl += t;
         }
-        const int total = __shfl(incl, 63, 64);
-        unsigned base = 0;
-        if (l == 0 && total > 0) base = atomicAdd(&cnt->cursor, (unsigned)total);
-        base = __shfl(base, 0, 64);
+        total = __shfl(incl, 63, 64);
+    }
+    if (l == 0) {
+        sd[w] = dropped;
+        se[w] = eblk;
+        tot[w] = (unsigned)total;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned run = 0;
+        double dsum = 0.0, esum = 0.0;
+        for (int i = 0; i < PW; ++i) {
+            pre[i] = run;
+            run += tot[i];
+            dsum += (double)sd[i];
+            esum += (double)se[i];
+        }
+        const unsigned base = run > 0 ? atomicAdd(&cnt->cursor, run) : 0u;
+        for (int i = 0; i < PW; ++i) pre[i] += base;
+        atomicAdd(&cnt->dropped, dsum);
+        atomicAdd(&cnt->energy, esum);
+    }
+    __syncthreads();
+    if (live) {
+        const unsigned base = pre[w];
         const bool fits = (unsigned long long)base + (unsigned)total <= cap;
         if (l == 0) {
             boff[b] = base;
@@ -206,15 +237,6 @@ __global__ __launch_bounds__(256) void pack_kernel(const float *__restrict__ coe
             for (int e = 0; e < 8; ++e)
                 if (q[e] >= lo) ent[pos++] = to_v23(v[e]) | (uint32_t)(8 * l + e);
         }
-    }
-    if (l == 0) {
-        sd[w] = dropped;
-        se[w] = eblk;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        atomicAdd(&cnt->dropped, (double)sd[0] + sd[1] + sd[2] + sd[3]);
-        atomicAdd(&cnt->energy, (double)se[0] + se[1] + se[2] + se[3]);
     }
 }
 
@@ -371,8 +393,8 @@ hipError_t launch_pack(const Work &w, int n, float tol2, float flo, uint32_t *en
     const long long nb = n / 8, nblk = nb * nb * nb;
     // a snapshot has at most n^3 entries
     const unsigned c32 = (unsigned)std::max(0LL, std::min(cap, (long long)n * n * n));
-    hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((nblk + 3) / 4)), dim3(256), 0, s, w.coef, nblk, tol2, flo, ent,
-                       c32, boff, bcnt, w.cnt);
+    hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((nblk + PW - 1) / PW)), dim3(64 * PW), 0, s, w.coef, nblk, tol2,
+                       flo, ent, c32, boff, bcnt, w.cnt);
     return hipGetLastError();
 }
 

@@ -49,8 +49,8 @@ __global__ void image_kernel(Grid g, int np, int nx, int ny, int nz, const float
                              const float *__restrict__ gP, float *img)
 {
     const int iz = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ix = blockIdx.y, iy = blockIdx.z;
-    if (iz >= nz) return;
+    const int ix = blockIdx.y * blockDim.y + threadIdx.y, iy = blockIdx.z;
+    if (iz >= nz || ix >= nx) return;
     const long long c = (iz + np) + (long long)(ix + np) * g.ldz + (long long)(iy + (g.nny > 1 ? np : 0)) * g.ldxy;
     const long long o = iz + (long long)ix * nz + (long long)iy * nz * nx;
     img[o] += sP[c] * gP[c];
@@ -60,8 +60,8 @@ __global__ void illum_kernel(Grid g, int np, int nx, int ny, int nz, const float
                              const float *__restrict__ Q, float *illum, int clamp)
 {
     const int iz = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ix = blockIdx.y, iy = blockIdx.z;
-    if (iz >= nz) return;
+    const int ix = blockIdx.y * blockDim.y + threadIdx.y, iy = blockIdx.z;
+    if (iz >= nz || ix >= nx) return;
     const long long c = (iz + np) + (long long)(ix + np) * g.ldz + (long long)(iy + (g.nny > 1 ? np : 0)) * g.ldxy;
     const long long o = iz + (long long)ix * nz + (long long)iy * nz * nx;
     const float p = P[c], q = Q[c];
@@ -96,7 +96,7 @@ hipError_t launch_image(vti_ctx *c, const float *sP, const float *gP, float *img
 {
     const vti_cfg &k = c->cfg;
     const int ny = k.ndim == 3 ? k.ny : 1;
-    dim3 blk(64), grd((k.nz + 63) / 64, k.nx, ny);
+    dim3 blk(64, 4), grd((k.nz + 63) / 64, (k.nx + 3) / 4, ny);
     hipLaunchKernelGGL(image_kernel, grd, blk, 0, c->stream, c->g, k.npml, k.nx, ny, k.nz, sP, gP, img);
     return hipGetLastError();
 }
@@ -105,7 +105,7 @@ hipError_t launch_illum(vti_ctx *c, const float *P, const float *Q, float *illum
 {
     const vti_cfg &k = c->cfg;
     const int ny = k.ndim == 3 ? k.ny : 1;
-    dim3 blk(64), grd((k.nz + 63) / 64, k.nx, ny);
+    dim3 blk(64, 4), grd((k.nz + 63) / 64, (k.nx + 3) / 4, ny);
     hipLaunchKernelGGL(illum_kernel, grd, blk, 0, c->stream, c->g, k.npml, k.nx, ny, k.nz, P, Q, illum, clamp);
     return hipGetLastError();
 }

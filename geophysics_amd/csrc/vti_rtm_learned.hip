@@ -42,8 +42,8 @@ __global__ void image_dense_kernel(Grid g, int np, int nx, int ny, int nz, const
                                    const float *__restrict__ gP, float *img)
 {
     const int iz = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ix = blockIdx.y, iy = blockIdx.z;
-    if (iz >= nz) return;
+    const int ix = blockIdx.y * blockDim.y + threadIdx.y, iy = blockIdx.z;
+    if (iz >= nz || ix >= nx) return;
     const long long c = (iz + np) + (long long)(ix + np) * g.ldz + (long long)(iy + np) * g.ldxy;
     const long long o = iz + (long long)ix * nz + (long long)iy * nz * nx;
     img[o] += S[o] * gP[c];
@@ -402,7 +402,7 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         if (launch_stress(3, ra, s) != hipSuccess) return fail(VTI_ERR_HIP, "stress");
         if (it % se) continue;
         const int j = it / se;
-        const dim3 blk(64), grd((k.nz + 63) / 64, k.nx, k.ny);
+        const dim3 blk(64, 4), grd((k.nz + 63) / 64, (k.nx + 3) / 4, k.ny);
         if (exact[j] >= 0) {
             hipLaunchKernelGGL(image_dense_kernel, grd, blk, 0, s, c->g, np, k.nx, k.ny, k.nz, X + (size_t)exact[j] * nn3,
                                G.f[VTI_F_P], c->img);
