@@ -187,7 +187,8 @@ def bench_batch_rtm(args, torch, vti):
         "config": {"workload": f"{nshot} independent 2D shots, nx in [300,1200], nz in [200,800], dx 10 m, "
                                f"nt {nt}, dt 1 ms, Ricker 10-25 Hz, layered models with a dipping fault",
                    "shots": nshot, "cells_per_step": cells, "forward_steps": fwd, "checkpoint_slots": o.nsnap,
-                   "stress_update": "split" if args.split else "unsplit interior (source side)",
+                   "stress_update": "split" if args.split else "unsplit interior (both sides; the receiver "
+                                                               "injection row split)",
                    "tile": list(cfgd["tile"]), "grid_mult": cfgd["grid_mult"], "seconds": round(secs, 3),
                    "shots_per_s": round(nshot / secs, 3)},
         "roofline": None, "cpu_baseline": None,
@@ -399,7 +400,8 @@ def bench_rtm3d_learned(args, torch, vti):
         "config": {"workload": f"3D VTI RTM {n}^3, {nt} steps, dt 0.8 ms, 15 Hz, {ntrain} fitting shots then "
                                f"{len(rows)} shots imaged both ways",
                    "codec": args.codec, "block_tol": args.tol if sparse else None,
-                   "source_side_stress": "split" if args.split else "unsplit interior",
+                   "stress_update": "split" if args.split else "reduced split (both sides; the receiver "
+                                                               "injection row split)",
                    "block_floor": args.floor if sparse else None, "src_box": args.src_box,
                    "plain_s_per_shot": round(tp, 3) if rows else None,
                    "learned_s_per_shot": round(tl, 3) if rows else None,
@@ -441,8 +443,8 @@ def main():
     ap.add_argument("--fused", action="store_true",
                     help="fd3d: one fused kernel per step instead of the velocity and stress kernels")
     ap.add_argument("--split", action="store_true",
-                    help="fd3d: the reference's split stress update everywhere (bit-identical to the oracle) "
-                         "instead of the unsplit interior")
+                    help="the reference's split stress update everywhere (bit-identical to the oracle) "
+                         "instead of the unsplit interior / reduced split")
     ap.add_argument("--exact-frac", type=float, default=0.25,
                     help="learned RTM, latent codec: fraction of snapshots the exact-storage fallback may keep")
     ap.add_argument("--codec", default="sparse", choices=["sparse", "latent"],

@@ -162,6 +162,7 @@ StressArgs stress_args_for(vti_ctx *c, const VtiState &s, int it, bool inject, b
     a.rz = c->shot.rec_z;
     a.rx0 = c->shot.rec_x0; a.rdx = c->shot.rec_dx; a.rnx = c->shot.rec_nx;
     a.ry0 = c->shot.rec_y0; a.rdy = c->shot.rec_dy; a.rny = c->shot.rec_ny;
+    a.split_z = -1;
     return a;
 }
 
@@ -186,9 +187,8 @@ StressArgs with_illum(vti_ctx *c, StressArgs a, int clamp)
 StressArgs rcv_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, int inj_z)
 {
     StressArgs a = stress_args_for(c, s, it, false, sv, nullptr);
-    const size_t M = (size_t)c->nnmax;
-    const bool damped_row = c->htab[10 * M + inj_z] != 1.0f || c->htab[11 * M + inj_z] != 1.0f;
-    a.unsplit = c->unsplit && c->cfg.mode == VTI_MODE_GPU_REF && damped_row;
+    a.unsplit = c->unsplit && c->cfg.mode == VTI_MODE_GPU_REF;
+    a.split_z = inj_z;
     return a;
 }
 

@@ -41,7 +41,6 @@ struct BShot {
     const double *tap;
     float *rec;         // [it][nrec]
     int nrec, rz, rx0, rdx, inj_z;
-    int inj_damped;     // the injection row lies in the absorbing layer
     float *img, *illum; // interior nx*nz
     int tile0, ntz, ntx;
 };
@@ -105,7 +104,8 @@ __global__ __launch_bounds__(TZ *TXB) void bvel(const BShot *__restrict__ S, int
 // illumination P*P+Q*Q, clamped to 1 where <= 0 when illum == 2);
 // 2: receiver (imaging with the source P).  unsplit: P and Q updated
 // directly where the cell's damping factors are all 1, the split fields
-// only in the absorbing layer (as stress2d_kernel<true>).
+// only in the absorbing layer and, on the receiver side, on the injection
+// row (as stress2d_kernel<true>).
 template <int TZ, int TXB, int MODE>
 __global__ __launch_bounds__(TZ *TXB) void bstress(const BShot *__restrict__ S, int ns, int ntile,
                                                   const int *__restrict__ itp, int sv_on, int image_every,
@@ -143,10 +143,10 @@ __global__ __launch_bounds__(TZ *TXB) void bstress(const BShot *__restrict__ S, 
         const float ax1 = tb[6 * b.M + ix], ax2 = tb[7 * b.M + ix];
         const float az1 = tb[10 * b.M + iz], az2 = tb[11 * b.M + iz];
         float Pn, Qn;
-        // receiver side: only when the records are written on a damped row,
-        // where the split form rebuilds P and Q (the written values do not
-        // persist), as in the single-shot RTM
-        const bool us = unsplit && (MODE != 2 || b.inj_damped);
+        // receiver side: the injection row keeps the split form, which
+        // rebuilds P and Q (the written records do not persist), as in the
+        // single-shot RTM
+        const bool us = unsplit && !(MODE == 2 && iz == b.inj_z);
         if (us && ax1 == 1.0f && ax2 == 1.0f && az1 == 1.0f && az2 == 1.0f) {
             const float aa = vv * vv;
             Pn = F[6][id] - (aa * e2 * b.dtx * xx + aa * sd * b.dtz * zz);
@@ -521,7 +521,6 @@ int vti_batch_create(int nshot, const vti_cfg *cfgs, const vti_shot *shots, cons
         b.tap = B->taps + tapoff[s];
         b.nrec = sh.rec_nx; b.rz = sh.rec_z; b.rx0 = sh.rec_x0; b.rdx = sh.rec_dx;
         b.inj_z = sh.rec_z;
-        b.inj_damped = htab[10 * (size_t)M + b.inj_z] != 1.0f || htab[11 * (size_t)M + b.inj_z] != 1.0f;
         b.rec = B->recbuf + rec_off;
         rec_off += (size_t)sh.rec_nx * B->nt;
         B->cells += (long long)(b.nnx - 8) * (b.nnz - 8);

@@ -58,8 +58,9 @@ struct StressArgs {
     // point-source form, reduced split (unsplit): in the interior P and Q
     // are updated directly; in 3D a cell of the absorbing layer keeps the
     // split fields only of its damped axes (all of them in the corners) and
-    // carries the undamped rest in P, Q.
-    int unsplit;
+    // carries the undamped rest in P, Q.  Row split_z (z index, -1: none)
+    // keeps the full split form.
+    int unsplit, split_z;
     // source illumination accumulated in the epilogue (RTM first sweep):
     // illum[interior cell] += P^2 + Q^2, a sum <= 0 set to 1 when illum_clamp;
     // the interior is inx x iny x inz cells from padded index inp (y only in 3D)
@@ -182,11 +183,9 @@ StressArgs stress_args_for(vti_ctx *c, const VtiState &s, int it, bool inject, b
 // unsplit-interior setting applies (3D point-source form).
 StressArgs src_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, float *rec_row);
 // Receiver-side stress arguments of an RTM whose records are written into
-// P and Q at row inj_z: the reduced split applies only when that row is in
-// the damped layer.  There the row's own stress result is never read (the
-// next load overwrites it before the velocity update, and it lies outside
-// the image), so whether the written values persist in P, Q does not
-// matter; an interior row would carry them into the image.
+// P and Q at row inj_z: the context's unsplit setting applies everywhere
+// but on that row, which keeps the split form (it rebuilds P and Q, so the
+// written values do not persist, as in the split form everywhere).
 StressArgs rcv_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, int inj_z);
 // a with the context's source illumination accumulated in its epilogue
 StressArgs with_illum(vti_ctx *c, StressArgs a, int clamp);

@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void stress2d_kernel(StressArgs a)
     const float sd = sqrtf(1.0f + 2.0f * dd);
     const float ax1 = a.ax1[ix], ax2 = a.ax2[ix], az1 = a.az1[iz], az2 = a.az2[iz];
     float Pn, Qn;
-    if (US && ax1 == 1.0f && ax2 == 1.0f && az1 == 1.0f && az2 == 1.0f) {
+    if (US && iz != a.split_z && ax1 == 1.0f && ax2 == 1.0f && az1 == 1.0f && az2 == 1.0f) {
         const float aa = vv * vv;
         Pn = a.P[id] - (aa * e2 * a.dtx * xx + aa * sd * a.dtz * zz);
         Qn = a.Q[id] - (aa * sd * a.dtx * xx + aa * a.dtz * zz);

@@ -153,7 +153,8 @@ __global__ __launch_bounds__(256) void vel3d(VelArgs a, int nzt, int ntile, int 
 // components and carries the rest in P: P' = (px' + py' + pz') + ((P - px -
 // py - pz) - (tx + ty + tz)), each term present only for the kept (px ..) or
 // the undamped (tx ..) axes (56 or 72 bytes per cell).  Corners (three damped
-// axes) keep the full split form.  Point-source (load-time) form only.
+// axes) and row a.split_z keep the full split form.  Point-source
+// (load-time) form only.
 template <int YCHUNK, bool NT, bool US = false>
 __global__ __launch_bounds__(256) void stress3d(StressArgs a, int nzt, int ntile, int ntile_p)
 {
@@ -176,7 +177,8 @@ __global__ __launch_bounds__(256) void stress3d(StressArgs a, int nzt, int ntile
     const float ax1 = a.ax1[ix], ax2 = a.ax2[ix], az1 = a.az1[iz], az2 = a.az2[iz];
     const float dtx = a.dtx, dty = a.dty, dtz = a.dtz;
     const bool dxc = !(ax1 == 1.0f && ax2 == 1.0f), dzc = !(az1 == 1.0f && az2 == 1.0f);
-    const bool col_int = US && !dxc && !dzc;
+    const bool col_full = !US || iz == a.split_z;
+    const bool col_int = !col_full && !dxc && !dzc;
     float vq[8];   // vq[k] = v at y-4+k
 #pragma unroll
     for (int k = 0; k < 8; ++k) vq[k] = v[col + (long long)(c.y0 - 4 + k) * L];
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(256) void stress3d(StressArgs a, int nzt, int ntile
         const bool dy = !(ay1 == 1.0f && ay2 == 1.0f);
         const bool un = col_int && !dy;
         // kept split components per axis; full: P, Q rebuilt from them alone
-        const bool full = !US || (dxc && dy && dzc);
+        const bool full = col_full || (dxc && dy && dzc);
         const bool fx = full || dxc, fy = full || dy, fz = full || dzc;
         float opx = 0.0f, opy = 0.0f, opz = 0.0f, oqx = 0.0f, oqy = 0.0f, oqz = 0.0f, oP = 0.0f, oQ = 0.0f;
         if (fx) { opx = ld<NT>(a.px + id); oqx = ld<NT>(a.qx + id); }

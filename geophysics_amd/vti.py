@@ -620,7 +620,7 @@ class Batch:
         return {"tile": (v[0].value, v[1].value), "grid_mult": v[2].value, "graph_steps": v[3].value}
 
     def set_unsplit(self, on=True):
-        """Source-side unsplit interior for every shot (point-source form)."""
+        """Unsplit interior for every shot on both sides, the receiver injection row split (point-source form)."""
         check(lib().vti_batch_set_unsplit(self.h, 1 if on else 0), "batch_set_unsplit")
 
     def autotune(self, steps=4):

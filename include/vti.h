@@ -244,9 +244,9 @@ int vti_batch_autotune(vti_batch *b, int steps, double *best_ms);
 /* Updated cells per step summed over the shots. */
 long long vti_batch_cells(const vti_batch *b);
 /* Every shot's propagation with the unsplit interior, with the same rule as
- * the single-shot RTM (see vti_set_unsplit): source side always, receiver
- * side when its records are written on a row of the absorbing layer.
- * Point-source form only; default off. */
+ * the single-shot RTM (see vti_set_unsplit): both sides, the receiver
+ * side's injection row in the split form.  Point-source form only; default
+ * off. */
 int vti_batch_set_unsplit(vti_batch *b, int on);
 /* nsteps forward steps of every shot from rest; record: keep the records. */
 int vti_batch_forward(vti_batch *b, int nsteps, int record);
@@ -452,9 +452,10 @@ int vti_image_accumulate(vti_ctx *src, vti_ctx *rcv, float *img, float *illum, i
  * to an FP64 run.
  * Switch before a run (after vti_reset): the interior split fields are not
  * maintained while it is on.  Default off.  vti_rtm_shot and
- * vti_rtm_shot_learned use it for the source-side propagation, and for the
- * receiver side when the records are written on a row of the absorbing
- * layer (the 3D reference's npml-1); vti_grad keeps the split form. */
+ * vti_rtm_shot_learned use it for both propagations; on the receiver side
+ * the row the records are written into keeps the split form, which
+ * rebuilds P and Q so that the written values do not persist (as in the
+ * reference).  vti_grad keeps the split form. */
 int vti_set_unsplit(vti_ctx *ctx, int on);
 
 /* 3D only: vti_step runs each step as one fused kernel (on = 1) or as the

@@ -18,7 +18,8 @@
  * the product's optional reduced-split update (checker for that mode): in
  * 3D, a cell keeps the split fields only of its damped axes (none in the
  * interior, all in the corners of the absorbing layer) and carries the
- * undamped rest in P, Q; in 2D only
+ * undamped rest in P, Q; the receiver simulation of orc_rtm keeps its
+ * injection row in the split form; in 2D only
  * the interior is unsplit.  In double it equals the split form to 1e-12
  * (tests/test_oracle_unsplit.py).
  */

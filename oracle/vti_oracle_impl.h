@@ -29,6 +29,7 @@ typedef struct SIM {
     double gw[49];
     int tap_n;              /* SV taper factor per integer r^2, 0..tap_n-1 */
     double *tap;
+    int split_z;            /* unsplit mode: row kept in the split form (-1: none) */
 } SIM;
 
 /* Ricker / Gaussian wavelet.
@@ -89,6 +90,7 @@ void *FN(orc_create)(const orc_params *p, const float *vp, const float *eps,
     SIM *s = (SIM *)calloc(1, sizeof(SIM));
     if (!s) return NULL;
     s->p = *p;
+    s->split_z = -1;
     int ny = p->ndim == 3 ? p->ny : 1;
     int np = p->npml;
     s->nnx = p->nx + 2 * np;
@@ -323,8 +325,8 @@ static void FN(stress)(SIM *s, int it, int inject, int sv)
                 REAL vv = vp[id];
                 REAL e2 = 1 + 2 * ee;
                 REAL sd = SQRT_R(1 + 2 * dd);
-                if (s->p.unsplit && s->p.src_kind == 0 && ax1[ix] == 1 && ax2[ix] == 1 && az1[iz] == 1 &&
-                    az2[iz] == 1) {
+                if (s->p.unsplit && s->p.src_kind == 0 && iz != s->split_z && ax1[ix] == 1 && ax2[ix] == 1 &&
+                    az1[iz] == 1 && az2[iz] == 1) {
                     /* unsplit interior (product option, not the reference) */
                     REAL a = vv * vv;
                     P[id] = P[id] - (a * e2 * dtx * xx + a * sd * dtz * zz);
@@ -357,7 +359,8 @@ static void FN(stress)(SIM *s, int it, int inject, int sv)
                     REAL sd = SQRT_R(1 + 2 * dd);
                     const int dX = !(ax1[ix] == 1 && ax2[ix] == 1), dY = !(ay1[iy] == 1 && ay2[iy] == 1);
                     const int dZ = !(az1[iz] == 1 && az2[iz] == 1);
-                    if (s->p.unsplit && dX + dY + dZ == 0) {
+                    const int fs = iz == s->split_z;
+                    if (s->p.unsplit && !fs && dX + dY + dZ == 0) {
                         /* unsplit interior (product option, not the reference):
                          * P -= sum of the three split increments, Q likewise */
                         REAL a = vv * vv;
@@ -365,7 +368,7 @@ static void FN(stress)(SIM *s, int it, int inject, int sv)
                         Q[id] = Q[id] - (a * sd * dtx * xx + a * sd * dty * yy + a * dtz * zz);
                         continue;
                     }
-                    if (s->p.unsplit && dX + dY + dZ < 3) {
+                    if (s->p.unsplit && !fs && dX + dY + dZ < 3) {
                         /* one or two damped axes: their split components are
                          * kept, the undamped rest is P minus them */
                         REAL a = vv * vv, z0 = 0;
@@ -541,6 +544,10 @@ int FN(orc_rtm)(void *hs, void *hr, int sv_fwd, int sv_bwd, int inject_z, int il
     REAL *row = (REAL *)malloc((nrec > 0 ? nrec : 1) * sizeof(REAL));
     if (!snap || !row) { free(snap); free(row); return -2; }
     const int yo = p->ndim == 3 ? np : 0;
+    /* the records replace P, Q of row inject_z before each receiver step: in
+     * the unsplit mode that row keeps the split form, which rebuilds P, Q
+     * from the split fields, so the loaded values do not persist */
+    g->split_z = inject_z;
     for (int it = 0; it < nt; it++) {
         FN(step_ex)(s, it, 1, sv_fwd, row);
         for (long r = 0; r < nrec; r++) modelled[it + (long)nt * r] = row[r];

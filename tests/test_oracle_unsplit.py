@@ -94,3 +94,26 @@ def test_unsplit_interior_2d():
     assert 0 < e_us <= 1e-5
     assert e_u64 <= 2 * e_s64
     assert rel_l2(un64, f64) < 1e-12
+
+
+def test_unsplit_rtm_2d_interior_injection_row_fp64():
+    """2D RTM whose records are loaded into P, Q at an interior row (the
+    first interior row, as the reference's 2D RTM): in the unsplit mode
+    (both simulations) that row keeps the split form, so the loaded values
+    do not persist in P, and the FP64 image equals the FP64 split image to
+    1e-12 (with the row unsplit they would reach the image)."""
+    from conftest import layered_model
+    nx, nz, npd, nt = 70, 56, 12, 300
+    vp, e, d = layered_model(nx, nz, seed=6)
+    d = np.minimum(d, e)
+    p = po.params_gpu2d(nx, nz, 10, 10, npd, 0, 20, 1, 35, 0, 1, nt, 1000)
+    assert npd <= p.rec_z < npd + nz           # an interior row
+    rng = np.random.default_rng(5)
+    obs = rng.standard_normal((p.rec_nx, nt)) * 0.1
+    kw = dict(sv_fwd=0, sv_bwd=0, illum_mode=1, obs=obs, prec="f64")
+    ref, rill, _ = po.rtm(p, vp, e, d, **kw)
+    un, uill, _ = po.rtm(p, vp, e, d, unsplit_source=True, unsplit_receiver=True, **kw)
+    assert np.abs(ref).max() > 0
+    assert rel_l2(un, ref) < 1e-12
+    assert rel_l2(uill, rill) < 1e-12
+
