@@ -577,6 +577,9 @@ def main():
             "bound": "hbm", "kernel": kernel,
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            # measured HBM bytes of the kernel over its duration (achieved counts
+            # the reference structure's bytes, which the unsplit forms partly avoid)
+            "traffic_rate": round(traffic / str_avg / 1e9, 1) if traffic else None,
         },
     }
     prop.close()
