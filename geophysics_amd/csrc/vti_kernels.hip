@@ -76,7 +76,8 @@ __device__ __forceinline__ void taper(const StressArgs &a, int r2, float &ee, fl
 
 // US: the unsplit interior (point-source form): where the cell's damping
 // factors are all exactly 1, P and Q are updated directly and px .. qz are
-// neither read nor written (36 instead of 60 bytes per cell).
+// neither read nor written (36 instead of 60 bytes per cell); row a.split_z
+// (an RTM receiver side's injection row) keeps the split form.
 template <bool US>
 __global__ __launch_bounds__(256) void stress2d_kernel(StressArgs a)
 {
