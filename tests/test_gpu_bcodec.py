@@ -147,6 +147,34 @@ def test_codec_strided_box_and_capacity():
     assert len(ent2) == kept - 1             # the buffer held exactly cap entries and nothing faulted
 
 
+@pytest.mark.parametrize("n", [24, 8])
+def test_codec_block_counts_off_the_pack_workgroup(n):
+    """27 and 1 blocks: the pack's last workgroup (8 blocks) has idle
+    wavefronts.  Every block is within its bound, the entries and the block
+    table cover exactly the kept count, and the decode is within the
+    reported error."""
+    import torch
+    V = _orthonormal(9)
+    rng = np.random.default_rng(n)
+    big = rng.standard_normal((n + 5, n + 3, n + 9)).astype(np.float32)
+    tol, floor = 0.05, 1e-8
+    with vti.Autoencoder(crop=16, batch=1, seed=1) as A:
+        A.set_basis(V)
+        kept, err, out, ent, boff, bcnt = _codec(A, torch, big, n, tol, n ** 3, off_yxz=(2, 1, 5), floor=floor)
+    box = big[2:2 + n, 1:1 + n, 5:5 + n].astype(np.float64)
+    assert 0 < kept <= n ** 3 and err <= tol
+    assert int(np.asarray(bcnt, dtype=np.int64).sum()) == kept
+    starts = np.sort(np.asarray(boff, dtype=np.int64))
+    assert starts[0] == 0 and starts[-1] <= kept
+    assert rel_l2(out, box) <= err + 1e-6
+    nb = n // 8
+    for i in range(nb ** 3):
+        bz, bx, by = i % nb, (i // nb) % nb, i // (nb * nb)
+        sl = np.s_[8 * by:8 * by + 8, 8 * bx:8 * bx + 8, 8 * bz:8 * bz + 8]
+        e_b = np.linalg.norm(out[sl] - box[sl])
+        assert e_b <= tol * np.linalg.norm(box[sl]) * (1 + 1e-3) + 1e-6
+
+
 def _setup(seed=7):
     vp, e, d = smooth_model3d(N, N, N, seed=seed)
     cfg = vti.cfg_ref3d(N, N, N, 10, 10, 10, NPML, 1, 25, NT, 800)
