@@ -86,6 +86,7 @@ void state_free(VtiState &s)
 
 hipError_t state_zero(vti_ctx *c, const VtiState &s)
 {
+    if (s.f[VTI_F_P] == c->f[VTI_F_P]) c->us_dirty = false;
     for (int k = 0; k < 11; ++k)
         if (s.f[k]) {
             hipError_t e = launch_zero(s.f[k], c->nvol, c->stream);
@@ -153,7 +154,8 @@ StressArgs stress_args_for(vti_ctx *c, const VtiState &s, int it, bool inject, b
     a.svR = c->cfg.sv_R;
     a.cx = c->shot.sv_cx; a.cy = c->shot.sv_cy; a.cz = c->shot.sv_cz;
     a.tap = c->dtap;
-    a.gsrc = (inject && c->cfg.mode == VTI_MODE_CPU_REF && c->amp[it] != 0.0f) ? 1 : 0;
+    a.gsrc = (inject && c->cfg.mode != VTI_MODE_GPU_REF && c->amp[it] != 0.0f) ? 1 : 0;
+    a.dpq = c->cfg.mode == VTI_MODE_OMP_REF ? 1 : 0;
     a.gx = c->shot.sx; a.gy = c->shot.sy; a.gz = c->shot.sz;
     a.gamp = c->amp[it];
     a.gw = c->dgw;
@@ -170,6 +172,7 @@ StressArgs src_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, float
 {
     StressArgs a = stress_args_for(c, s, it, true, sv, rec_row);
     a.unsplit = c->unsplit && c->cfg.mode == VTI_MODE_GPU_REF;
+    if (a.unsplit) c->us_dirty = true;
     return a;
 }
 
@@ -188,6 +191,7 @@ StressArgs rcv_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, int i
 {
     StressArgs a = stress_args_for(c, s, it, false, sv, nullptr);
     a.unsplit = c->unsplit && c->cfg.mode == VTI_MODE_GPU_REF;
+    if (a.unsplit) c->us_dirty = true;
     a.split_z = inj_z;
     return a;
 }
@@ -224,7 +228,7 @@ const char *vti_last_error(void) { return last_error_cstr(); }
 int vti_cfg_ref2d(vti_cfg *cfg, int mode, int nx, int nz, int dx, int dz, int npd, int SV,
                   int favg, int nt, int dt_us)
 {
-    if (!cfg || (mode != VTI_MODE_GPU_REF && mode != VTI_MODE_CPU_REF))
+    if (!cfg || (mode != VTI_MODE_GPU_REF && mode != VTI_MODE_CPU_REF && mode != VTI_MODE_OMP_REF))
         return fail(VTI_ERR_ARG, "vti_cfg_ref2d: bad cfg pointer or mode");
     std::memset(cfg, 0, sizeof(*cfg));
     cfg->ndim = 2;
@@ -235,6 +239,10 @@ int vti_cfg_ref2d(vti_cfg *cfg, int mode, int nx, int nz, int dx, int dz, int np
     cfg->favg = (float)favg;
     cfg->pfac = 10.0f;                          // :349, CPU_vti2dfd_kernels.c:382
     cfg->wtype = 1;
+    if (mode == VTI_MODE_OMP_REF) {
+        cfg->pfac = 1000.0f;                    // Toafd2domp_2.c:70
+        cfg->wtype = 4;                         // its Ricker product order, :154
+    }
     cfg->mode = mode;
     cfg->sv = SV;
     cfg->rec_kind = VTI_REC_DEFAULT;
@@ -308,8 +316,9 @@ int vti_create(const vti_cfg *cfg, vti_ctx **out)
     const vti_cfg &c = *cfg;
     if ((c.ndim != 2 && c.ndim != 3) || c.nx < 1 || c.nz < 1 || (c.ndim == 3 && c.ny < 1) ||
         c.npml < 0 || c.nt < 1 || !(c.dt > 0) || !(c.dx > 0) || !(c.dz > 0) ||
-        (c.ndim == 3 && !(c.dy > 0)) || !(c.favg > 0) || c.wtype < 1 || c.wtype > 3 ||
-        (c.mode != VTI_MODE_GPU_REF && c.mode != VTI_MODE_CPU_REF) ||
+        (c.ndim == 3 && !(c.dy > 0)) || !(c.favg > 0) || c.wtype < 1 || c.wtype > 4 ||
+        (c.mode != VTI_MODE_GPU_REF && c.mode != VTI_MODE_CPU_REF && c.mode != VTI_MODE_OMP_REF) ||
+        (c.mode == VTI_MODE_OMP_REF && c.ndim != 2) ||
         c.rec_kind < VTI_REC_DEFAULT || c.rec_kind > VTI_REC_PQ)
         return fail(VTI_ERR_ARG, "vti_create: invalid configuration");
     const int nnx = c.nx + 2 * c.npml, nnz = c.nz + 2 * c.npml;
@@ -326,7 +335,8 @@ int vti_create(const vti_cfg *cfg, vti_ctx **out)
     if (x->cfg.sv_R <= 0) x->cfg.sv_R = 15;                   // R=15 / Circle_iso=15
     if (x->cfg.sv_r <= 0) x->cfg.sv_r = 4;                    // r=4 (GPU form)
     if (x->cfg.rec_kind == VTI_REC_DEFAULT)
-        x->cfg.rec_kind = c.mode == VTI_MODE_CPU_REF ? VTI_REC_2P : VTI_REC_P;
+        x->cfg.rec_kind = c.mode == VTI_MODE_CPU_REF ? VTI_REC_2P
+                        : (c.mode == VTI_MODE_OMP_REF ? VTI_REC_PQ : VTI_REC_P);
     HIP_TRY(hipGetDevice(&x->device));
     x->g.nnx = nnx; x->g.nny = nny; x->g.nnz = nnz;
     x->g.ldz = round_up(nnz, 32);                             // 128-byte rows
@@ -433,7 +443,9 @@ static int host_tables_impl(const vti_cfg &k, const float *hv, int M, float *h, 
         if (!(hv[i] > 0.0f) || !std::isfinite(hv[i]))
             return fail(VTI_ERR_ARG, "vp must be positive and finite");
     float d0 = 0.0f;
-    if (np > 0) {
+    // Toafd2domp_2.c has no absorbing layer (npd = 0, :67; its updates carry
+    // no damping factors, :166-180): d0 = 0 makes every table 1
+    if (np > 0 && k.mode != VTI_MODE_OMP_REF) {
         if (k.mode == VTI_MODE_GPU_REF) {
             if (!three) {   // get_d02d GPU_vti2dfd_kernels.cu:105
                 const float sdx = k.dx + k.dz;
@@ -536,18 +548,24 @@ int vti_set_shot(vti_ctx *c, const vti_shot *s)
     };
     // The source (and its 7x7 footprint in CPU mode) must lie in the
     // updated region, where the stress update rewrites P and Q every step.
-    const int fr = k.mode == VTI_MODE_CPU_REF ? 3 : 0;
+    const int fr = k.mode != VTI_MODE_GPU_REF ? 3 : 0;
     if (!inside(s->sx - fr, s->sy, s->sz - fr) || !inside(s->sx + fr, s->sy, s->sz + fr))
         return fail(VTI_ERR_ARG, "vti_set_shot: source footprint outside the updated region");
     const int ny_rec = is3d(c) ? s->rec_ny : 1;
     if (s->rec_nx < 0 || ny_rec < 0 || s->rec_dx < 1 || (is3d(c) && s->rec_dy < 1))
         return fail(VTI_ERR_ARG, "vti_set_shot: bad receiver grid");
+    // Receivers may lie anywhere in the padded grid: one outside the updated
+    // region records the P there, which no step changes (0), as the
+    // reference does (Toafd2domp_2.c:219 records row npd = 0).
+    auto in_grid = [&](int x, int y, int z) {
+        return x >= 0 && x < g.nnx && z >= 0 && z < g.nnz && (is3d(c) ? (y >= 0 && y < g.nny) : y == 0);
+    };
     if (s->rec_nx > 0 && ny_rec > 0) {
         const int xl = s->rec_x0 + (s->rec_nx - 1) * s->rec_dx;
         const int yl = is3d(c) ? s->rec_y0 + (s->rec_ny - 1) * s->rec_dy : 0;
         const int y0 = is3d(c) ? s->rec_y0 : 0;
-        if (!inside(s->rec_x0, y0, s->rec_z) || !inside(xl, yl, s->rec_z))
-            return fail(VTI_ERR_ARG, "vti_set_shot: receivers outside the updated region");
+        if (!in_grid(s->rec_x0, y0, s->rec_z) || !in_grid(xl, yl, s->rec_z))
+            return fail(VTI_ERR_ARG, "vti_set_shot: receivers outside the padded grid");
     }
     const int nrec = s->rec_nx * ny_rec;
     if (nrec != c->nrec || !c->rec) {
@@ -583,6 +601,7 @@ int vti_reset(vti_ctx *c)
     for (int k = 0; k < 11; ++k)
         if (c->f[k]) HIP_TRY(launch_zero(c->f[k], c->nvol, c->stream));
     c->alt_valid = false;
+    c->us_dirty = false;
     if (c->rec) HIP_TRY(launch_zero(c->rec, (size_t)c->nrec * c->cfg.nt, c->stream));
     return VTI_OK;
 }
@@ -591,6 +610,9 @@ static int check_step(vti_ctx *c, int it)
 {
     if (!c) return fail(VTI_ERR_ARG, "null ctx");
     if (!c->have_model || !c->have_shot) return fail(VTI_ERR_STATE, "model and shot must be set before stepping");
+    if (!c->unsplit && c->us_dirty)
+        return fail(VTI_ERR_STATE, "split-form step after unsplit steps: the interior split fields are stale "
+                                   "(call vti_reset first)");
     if (it < 0 || it >= c->cfg.nt) return fail(VTI_ERR_ARG, "time index out of range");
     return VTI_OK;
 }
@@ -700,6 +722,8 @@ int vti_load_state(vti_ctx *c, const void *src)
             p += c->nvol * sizeof(float);
         }
     c->alt_valid = false;
+    // a state saved with the unsplit update on is valid only with it on
+    c->us_dirty = c->unsplit;
     return VTI_OK;
 }
 

@@ -634,6 +634,14 @@ int vti_batch_rtm(vti_batch *B, const vti_rtm_opts *o, const float *const *obs, 
     if (!B || !o || o->image_every < 1 || o->nsnap < 0 || o->illum_mode != 0)
         return bfail(VTI_ERR_ARG, "vti_batch_rtm: bad arguments (the batch images with source illumination, illum_mode 0)");
     const int nt = B->nt;
+    // the taper switches and the imaging interval hold for this call only:
+    // vti_batch_forward keeps the batch's own (SV taper on in the forward
+    // propagation); illum_mode stays, vti_batch_image reads it
+    struct Restore {
+        vti_batch *B;
+        int sf, sb, ie;
+        ~Restore() { B->sv_fwd = sf; B->sv_bwd = sb; B->image_every = ie; }
+    } restore{B, B->sv_fwd, B->sv_bwd, B->image_every};
     B->image_every = o->image_every;
     B->illum_mode = o->illum_mode;
     B->sv_fwd = o->sv_forward;

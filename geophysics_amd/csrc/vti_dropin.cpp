@@ -5,6 +5,8 @@
 //   GPU_vti2dfd  GPU_vti2dfd_kernels.cu:301-484
 //   GPU_vti3dfd  GPU_vti3dfd_kernels.cu:371-573
 //   CPU_vti2dfd  CPU_vti2dfd_kernels.c:366-536 (the CPU driver's semantics)
+//   vti_omp2dfd  main of 2D-vti-fd-omp/Toafd2domp_2.c:40-242 (a program in the
+//                reference, with its constants as arguments here)
 //
 // Differences from the reference, all on error paths or defects: errors
 // set vti_dropin_status() instead of exit(0) / continuing with NULL files;
@@ -162,6 +164,71 @@ int run3d(int nx, int ny, int nz, int dx, int dy, int dz, int npml, int SV, cons
     return st;
 }
 
+// main of Toafd2domp_2.c (2D-vti-fd-omp/Toafd2domp_2.c:40-242), its
+// hard-coded values (:55-76) as arguments: models read x-major, z fastest
+// (:91-104); P of the interior every 50 steps from step 0 into frame it/50
+// (:222-227); the P+Q record at row npd, [ix][it] (:219, :231-235).
+int run_omp2d(int nx, int nz, float dx, float dz, int npd, float favg, float pfac, int sx, int sz, int nt,
+              float dt, const char *FNv, const char *FNe, const char *FNd, const char *FNshot, const char *FNsnap)
+{
+    vti_cfg cfg{};
+    cfg.ndim = 2;
+    cfg.nx = nx; cfg.ny = 1; cfg.nz = nz; cfg.npml = npd;
+    cfg.dx = dx; cfg.dy = dx; cfg.dz = dz;
+    cfg.dt = dt; cfg.nt = nt;
+    cfg.favg = favg; cfg.pfac = pfac;
+    cfg.wtype = 4;
+    cfg.mode = VTI_MODE_OMP_REF;
+    cfg.sv = 0;
+    cfg.rec_kind = VTI_REC_PQ;
+    const size_t n = (size_t)nx * nz;
+    std::vector<float> v, e, d;
+    int st;
+    if ((st = read_model(FNv, v, n)) || (st = read_model(FNe, e, n)) || (st = read_model(FNd, d, n)))
+        return st;
+    Ctx x;
+    if ((st = vti_create(&cfg, &x.c))) return st;
+    if ((st = vti_set_model(x.c, v.data(), e.data(), d.data(), 0))) return st;
+    vti_shot sh{};
+    sh.sx = (int)(sx + 0.5) + npd - 1;                           // :150
+    sh.sz = (int)(sz + 0.5) + npd - 1;                           // :151
+    sh.sv_cx = sh.sx; sh.sv_cz = sh.sz;
+    sh.rec_z = npd;                                              // :219
+    sh.rec_x0 = npd; sh.rec_dx = 1; sh.rec_nx = nx;
+    sh.rec_dy = 1; sh.rec_ny = 1;
+    if ((st = vti_set_shot(x.c, &sh)) || (st = vti_reset(x.c))) return st;
+    FILE *fshot = std::fopen(FNshot, "wb");
+    FILE *fsnap = std::fopen(FNsnap, "wb");
+    if (!fshot || !fsnap) {
+        if (fshot) std::fclose(fshot);
+        if (fsnap) std::fclose(fsnap);
+        vti::set_error("cannot open output files");
+        return VTI_ERR_IO;
+    }
+    int nnx, nny, nnz;
+    vti_padded_dims(x.c, &nnx, &nny, &nnz);
+    std::vector<float> full((size_t)nnx * nnz), win(n), shot((size_t)nx * nt);
+    for (int it = 0; it < nt && st == VTI_OK;) {
+        int last = (it + 49) / 50 * 50;                          // next snapshot step >= it
+        if (last > nt - 1) last = nt - 1;
+        if ((st = vti_step(x.c, it, last - it + 1))) break;
+        it = last + 1;
+        if (last % 50 == 0) {
+            if ((st = vti_get_field(x.c, VTI_F_P, full.data()))) break;
+            for (int ix = 0; ix < nx; ++ix)
+                for (int iz = 0; iz < nz; ++iz)
+                    win[iz + (size_t)nz * ix] = full[(iz + npd) + (size_t)nnz * (ix + npd)];
+            std::fseek(fsnap, (long)(last / 50) * (long)n * 4L, SEEK_SET);
+            std::fwrite(win.data(), 4, n, fsnap);
+        }
+    }
+    if (st == VTI_OK && (st = vti_get_record(x.c, shot.data())) == VTI_OK)
+        std::fwrite(shot.data(), sizeof(float), shot.size(), fshot);
+    std::fclose(fshot);
+    std::fclose(fsnap);
+    return st;
+}
+
 }  // namespace
 
 extern "C" {
@@ -192,6 +259,15 @@ void GPU_vti3dfd(int nx, int ny, int nz, int dx, int dy, int dz, int npml, int S
     (void)run_count;
     g_status = run3d(nx, ny, nz, dx, dy, dz, npml, SV, FNv, FNe, FNd, favg, ns, fsx, dsx, fsy, dsy,
                      fsz, dsz, FNshot, FNsnap, nt, dt);
+}
+
+int vti_omp2dfd(int nx, int nz, float dx, float dz, int npd, float favg, float pfac, int sx, int sz, int nt,
+                float dt, const char FNv[], const char FNe[], const char FNd[], const char FNshot[],
+                const char FNsnap[])
+{
+    const int st = run_omp2d(nx, nz, dx, dz, npd, favg, pfac, sx, sz, nt, dt, FNv, FNe, FNd, FNshot, FNsnap);
+    g_status = st;
+    return st;
 }
 
 int vti_dropin_status(void) { return g_status.load(); }

@@ -61,6 +61,9 @@ struct StressArgs {
     // carries the undamped rest in P, Q.  Row split_z (z index, -1: none)
     // keeps the full split form.
     int unsplit, split_z;
+    // Toafd2domp_2.c form (2D): pz and qx are (vp*vp)*pow(1+2*del, 0.5)*dt*d
+    // in double, subtracted in double and rounded once (:176-180)
+    int dpq;
     // source illumination accumulated in the epilogue (RTM first sweep):
     // illum[interior cell] += P^2 + Q^2, a sum <= 0 set to 1 when illum_clamp;
     // the interior is inx x iny x inz cells from padded index inp (y only in 3D)
@@ -157,6 +160,9 @@ struct vti_ctx {
     float *altf[11] = {};
     bool fused = false;
     bool unsplit = false;   // vti_step: unsplit interior (3D point-source form)
+    // an unsplit stress update has run since the last vti_reset: the
+    // interior split fields no longer add up to P and Q
+    bool us_dirty = false;
     bool alt_valid = false;   // altf equals f outside the updated region
 
     // imaging (RTM): interior-sized image and illumination, accumulated over shots

@@ -78,7 +78,10 @@ __device__ __forceinline__ void taper(const StressArgs &a, int r2, float &ee, fl
 // factors are all exactly 1, P and Q are updated directly and px .. qz are
 // neither read nor written (36 instead of 60 bytes per cell); row a.split_z
 // (an RTM receiver side's injection row) keeps the split form.
-template <bool US>
+// DP: the Toafd2domp_2.c form, whose pz and qx terms are double products
+// (pow(1+2*del, 0.5) returns double, :176-180), subtracted in double and
+// rounded to float once; px and qz stay float as there.
+template <bool US, bool DP>
 __global__ __launch_bounds__(256) void stress2d_kernel(StressArgs a)
 {
     const Grid g = a.g;
@@ -108,8 +111,15 @@ __global__ __launch_bounds__(256) void stress2d_kernel(StressArgs a)
         Qn = a.Q[id] - (aa * sd * a.dtx * xx + aa * a.dtz * zz);
     } else {
         const float px = ax2 * a.px[id] - ax1 * vv * vv * e2 * a.dtx * xx;
-        const float pz = az2 * a.pz[id] - az1 * vv * vv * sd * a.dtz * zz;
-        const float qx = ax2 * a.qx[id] - ax1 * vv * vv * sd * a.dtx * xx;
+        float pz, qx;
+        if (DP) {
+            const double sq = sqrt((double)(1.0f + 2.0f * dd));
+            pz = (float)((double)(az2 * a.pz[id]) - (double)(az1 * vv * vv) * sq * (double)a.dtz * (double)zz);
+            qx = (float)((double)(ax2 * a.qx[id]) - (double)(ax1 * vv * vv) * sq * (double)a.dtx * (double)xx);
+        } else {
+            pz = az2 * a.pz[id] - az1 * vv * vv * sd * a.dtz * zz;
+            qx = ax2 * a.qx[id] - ax1 * vv * vv * sd * a.dtx * xx;
+        }
         const float qz = az2 * a.qz[id] - az1 * vv * vv * a.dtz * zz;
         a.px[id] = px; a.pz[id] = pz; a.qx[id] = qx; a.qz[id] = qz;
         Pn = px + pz;
@@ -274,8 +284,9 @@ hipError_t launch_stress(int ndim, const StressArgs &a, hipStream_t s)
 {
     if (ndim == 2) {
         dim3 blk(TZ, TX), grd((a.g.nnz + TZ - 1) / TZ, (a.g.nnx + TX - 1) / TX);
-        if (a.unsplit && !a.gsrc) hipLaunchKernelGGL(stress2d_kernel<true>, grd, blk, 0, s, a);
-        else hipLaunchKernelGGL(stress2d_kernel<false>, grd, blk, 0, s, a);
+        if (a.dpq) hipLaunchKernelGGL((stress2d_kernel<false, true>), grd, blk, 0, s, a);
+        else if (a.unsplit && !a.gsrc) hipLaunchKernelGGL((stress2d_kernel<true, false>), grd, blk, 0, s, a);
+        else hipLaunchKernelGGL((stress2d_kernel<false, false>), grd, blk, 0, s, a);
     } else {
         int nzt, nt, ntp;
         const int grd = s3d::grid3d(a.g, kYChunk, nzt, nt, ntp);

@@ -70,16 +70,24 @@ float wavelet(float ts, float favg, int wtype)
         const float e = -x * ts * ts;
         return static_cast<float>(std::exp(static_cast<double>(e)));
     }
+    if (wtype == 4) {
+        // Toafd2domp_2.c:154: (favg*ts*favg*ts) is a float product in the
+        // first factor; inside exp, pi*pi*favg*ts*favg*ts is a double chain
+        const float x4 = favg * ts * favg * ts;
+        const double arg = kPi * kPi * favg * ts * favg * ts;
+        return static_cast<float>((1 - 2 * kPi * kPi * x4) * std::exp(-arg));
+    }
     return 0.0f;
 }
 
 // Per-step source gain pfac*wavelet(t - 1/favg), active while t <= 2/favg.
 // The time is a float accumulated from dt (t=dt; t+=dt:
-// GPU_vti2dfd_kernels.cu:419, CPU_vti2dfd_kernels.c:478).
+// GPU_vti2dfd_kernels.cu:419, CPU_vti2dfd_kernels.c:478), from 0 in the
+// Toafd2domp_2.c form (t=0.0; t+=dt, :144).
 void source_gains(const vti_cfg &c, std::vector<float> &amp)
 {
     amp.assign(c.nt > 0 ? c.nt : 0, 0.0f);
-    float t = c.dt;
+    float t = c.mode == VTI_MODE_OMP_REF ? 0.0f : c.dt;
     const float tdelay = static_cast<float>(1.0 / c.favg);
     for (int it = 0; it < c.nt; ++it) {
         if (t <= 2 * tdelay) {

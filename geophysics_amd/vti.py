@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(_HERE, "libvti.so")
 
 MODE_GPU_REF = 0
 MODE_CPU_REF = 1
+MODE_OMP_REF = 2      # 2D-vti-fd-omp/Toafd2domp_2.c
 REC_DEFAULT, REC_P, REC_2P, REC_PQ = -1, 0, 1, 2
 
 FIELDS = {n: i for i, n in enumerate(
@@ -169,6 +170,9 @@ def _declare(L):
         fn = getattr(L, name)
         fn.restype = None
         fn.argtypes = [C.c_int] * 6 + [C.c_char_p] * 3 + [C.c_int] * 5 + [C.c_char_p] * 2 + [C.c_int] * 3
+    L.vti_omp2dfd.restype = C.c_int
+    L.vti_omp2dfd.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_float, C.c_float,
+                              C.c_int, C.c_int, C.c_int, C.c_float] + [C.c_char_p] * 5
     L.GPU_vti3dfd.restype = None
     L.GPU_vti3dfd.argtypes = [C.c_int] * 8 + [C.c_char_p] * 3 + [C.c_int] * 8 + [C.c_char_p] * 2 + \
         [C.c_int] * 3
@@ -402,6 +406,8 @@ class Propagator:
         p = None
         if obs is not None:
             a, p = _f32(obs)
+            if a.shape != (self.nrec, self.cfg.nt):
+                raise VtiError(f"obs must be shaped ({self.nrec}, {self.cfg.nt})")
             self._obs_keep = a
         check(lib().vti_rtm_shot_learned(self.h, ae.h, C.byref(o), C.byref(lo), p, st.ctypes.data_as(DP)),
               "rtm_shot_learned")
@@ -496,6 +502,14 @@ def GPU_vti3dfd(nx, ny, nz, dx, dy, dz, npml, SV, FNv, FNe, FNd, favg, ns, fsx, 
     lib().GPU_vti3dfd(nx, ny, nz, dx, dy, dz, npml, SV, _b(FNv), _b(FNe), _b(FNd), favg, ns,
                       fsx, dsx, fsy, dsy, fsz, dsz, _b(FNshot), _b(FNsnap), nt, dt, run_count)
     check(lib().vti_dropin_status(), "GPU_vti3dfd")
+
+
+def omp2dfd(FNv, FNe, FNd, FNshot, FNsnap, nx=601, nz=301, dx=5.0, dz=5.0, npd=0, favg=30.0,
+            pfac=1000.0, sx=100, sz=100, nt=4001, dt=0.0005):
+    """main of 2D-vti-fd-omp/Toafd2domp_2.c:40-242 with its hard-coded values
+    as defaults (files in, shot / snapshot files out)."""
+    check(lib().vti_omp2dfd(nx, nz, dx, dz, npd, favg, pfac, sx, sz, nt, dt, _b(FNv), _b(FNe), _b(FNd),
+                            _b(FNshot), _b(FNsnap)), "vti_omp2dfd")
 
 
 def _f64(a):

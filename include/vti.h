@@ -40,6 +40,12 @@ extern "C" {
  * selects host-built tables and the fused source/receiver work. */
 #define VTI_MODE_GPU_REF 0  /* GPU_vti2dfd_kernels.cu / GPU_vti3dfd_kernels.cu */
 #define VTI_MODE_CPU_REF 1  /* CPU_vti2dfd_kernels.c                            */
+/* 2D-vti-fd-omp/Toafd2domp_2.c (2D only): the CPU form's in-stress 7x7
+ * Gaussian source, no absorbing layer (every damping factor 1), the first
+ * time sample at t = 0 (:144), pfac 1000 (:70), its Ricker product order
+ * (wtype 4, :154), the pz and qx updates in double precision (pow promotes
+ * them, :176-180), P+Q recorded (:219). */
+#define VTI_MODE_OMP_REF 2
 
 #define VTI_REC_DEFAULT -1  /* mode default: P (GPU), P+P (CPU)                 */
 #define VTI_REC_P        0  /* GPU_vti2dfd_kernels.cu:147                       */
@@ -55,7 +61,8 @@ typedef struct vti_cfg {
     int nt;            /* number of time steps                              */
     float favg;        /* wavelet peak frequency, Hz                        */
     float pfac;        /* source gain (reference: 10)                       */
-    int wtype;         /* 1 Ricker, 2 / 3 Gaussian forms                    */
+    int wtype;         /* 1 Ricker, 2 / 3 Gaussian forms, 4 Ricker with the
+                          product order of Toafd2domp_2.c:154               */
     int mode;          /* VTI_MODE_*                                        */
     int sv;            /* taper eps/delta around the source (SV flag)       */
     int sv_R, sv_r;    /* taper radii in cells; 0 = mode default            */
@@ -451,7 +458,10 @@ int vti_image_accumulate(vti_ctx *src, vti_ctx *rcv, float *img, float *illum, i
  * everywhere): the records agree with it to ~1e-6 rel-L2 and are as close
  * to an FP64 run.
  * Switch before a run (after vti_reset): the interior split fields are not
- * maintained while it is on.  Default off.  vti_rtm_shot and
+ * maintained while it is on, so a split-form step (vti_update_vel,
+ * vti_update_stress, vti_step with it off) after unsplit steps returns
+ * VTI_ERR_STATE until vti_reset; a state loaded with vti_load_state counts
+ * as unsplit when the setting is on at the load.  Default off.  vti_rtm_shot and
  * vti_rtm_shot_learned use it for both propagations; on the receiver side
  * the row the records are written into keeps the split form, which
  * rebuilds P and Q so that the written values do not persist (as in the
@@ -499,6 +509,19 @@ void CPU_vti2dfd(int nx, int nz, int dx, int dz, int npd, int SV,
                  int favg, int ns, int fs, int ds, int zs,
                  const char FNshot[], const char FNsnap[], int nt, int dt,
                  int run_count);
+/* main of 2D-vti-fd-omp/Toafd2domp_2.c:40-242, a program in the reference,
+ * with its hard-coded values as arguments (there: 601, 301, 5.0, 5.0, 0,
+ * 30, 1000, 100, 100, 4001, 0.0005, and the file names v_601_301.dat,
+ * e_601_301.dat, d_601_301.dat, shot.dat, snap.dat): VTI_MODE_OMP_REF
+ * semantics; models read x-major, z fastest (:91-104); the interior P every
+ * 50 steps from step 0 into frame it/50 of FNsnap (:222-227); the P+Q
+ * record of row npd, [ix][it], into FNshot (:219, :231-235; with npd = 0
+ * that row is never updated and the record is zero).  Returns the status
+ * (also kept for vti_dropin_status). */
+int vti_omp2dfd(int nx, int nz, float dx, float dz, int npd, float favg,
+                float pfac, int sx, int sz, int nt, float dt,
+                const char FNv[], const char FNe[], const char FNd[],
+                const char FNshot[], const char FNsnap[]);
 /* Status of the last drop-in call (the reference entry points return void). */
 int vti_dropin_status(void);
 

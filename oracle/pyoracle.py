@@ -10,7 +10,10 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "liboracle_vti.so")
+_DEFAULT_LIB = os.path.join(_HERE, "liboracle_vti.so")
+# ORC_LIB names another build of the same source (a build with a planted
+# mistake, for checking that the tests can fail); it is never built here.
+_LIB = os.environ.get("ORC_LIB") or _DEFAULT_LIB
 
 
 class OrcParams(C.Structure):
@@ -22,7 +25,7 @@ class OrcParams(C.Structure):
             "sx", "sy", "sz", "sv", "sv_kind", "sv_R", "sv_r",
             "sv_cx", "sv_cy", "sv_cz",
             "rec_z", "rec_x0", "rec_dx", "rec_nx", "rec_y0", "rec_dy", "rec_ny",
-            "rec_kind", "unsplit")]
+            "rec_kind", "omp_form", "unsplit")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -45,7 +48,7 @@ def build():
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB):
+        if not os.path.exists(_LIB) and _LIB == _DEFAULT_LIB:
             build()
         L = C.CDLL(_LIB)
         fp = C.POINTER(C.c_float)
@@ -64,6 +67,8 @@ def lib():
         L.orc_params_gpu2d.argtypes = [pp] + [C.c_int] * 14
         L.orc_params_cpu2d.argtypes = [pp] + [C.c_int] * 14
         L.orc_params_gpu3d.argtypes = [pp] + [C.c_int] * 20
+        L.orc_params_omp2d.argtypes = [pp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_float,
+                                       C.c_float, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int]
         L.orc_mute2d.argtypes = [C.c_int, C.c_int] + [C.c_float] * 4 + [C.c_int] * 4 + \
             [C.c_float, C.c_float, fp, C.c_int, C.c_int, C.c_int]
         L.orc_mute3d.argtypes = [C.c_int] * 3 + [C.c_float] * 5 + [C.c_int] * 6 + \
@@ -97,6 +102,14 @@ def params_cpu2d(nx, nz, dx, dz, npd, SV, favg, is_, fs, ds, zs, nt, dt_us, rec_
     p = OrcParams()
     lib().orc_params_cpu2d(C.byref(p), nx, nz, dx, dz, npd, SV, favg, is_, fs, ds, zs,
                            nt, dt_us, rec_depth)
+    return p
+
+
+def params_omp2d(nx, nz, dx, dz, npd, favg, pfac, sx, sz, nt, dt, rec_depth=1):
+    """Toafd2domp_2.c's main (reference values: 601, 301, 5.0, 5.0, 0, 30,
+    1000, 100, 100, 4001, 0.0005)."""
+    p = OrcParams()
+    lib().orc_params_omp2d(C.byref(p), nx, nz, dx, dz, npd, favg, pfac, sx, sz, nt, dt, rec_depth)
     return p
 
 
@@ -158,6 +171,11 @@ class Sim:
         out = np.zeros(n, dtype=self.dtype)
         k = getattr(lib(), "orc_get_" + self.prec)(self.h, TABLES[name], self._ptr(out))
         return out[:k]
+
+    def set_table(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert a.size == self.table(name).size
+        getattr(lib(), "orc_set_" + self.prec)(self.h, TABLES[name], self._ptr(a))
 
     def srcamp(self):
         out = np.zeros(self.p.nt, dtype=self.dtype)

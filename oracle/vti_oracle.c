@@ -11,6 +11,19 @@
 /* #define pi 3.141592653  (GPU_vti2dfd_kernels.cu:31, CPU_vti2dfd_kernels.c:27) */
 #define ORC_PI 3.141592653
 
+/* Planted mistakes, for checking that the oracle's pinning tests can fail
+ * (tests/test_oracle_pin3d.py builds mutants with -DORC_PLANT=k; the
+ * default build has none):
+ *   1  3D velocity update: y stencil offsets of the stress update
+ *   2  3D velocity update: v damped with the x tables
+ *   3  3D stress update: x stencil offsets of the velocity update
+ *   4  3D stress update: py scaled by dt/dx instead of dt/dy
+ *   5  point source injected after the velocity update
+ *   6  3D stress update: qy scaled by (1+2 eps) instead of sqrt(1+2 delta) */
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+
 #define REAL float
 #define SFX _f32
 #include "vti_oracle_impl.h"
@@ -88,6 +101,40 @@ int orc_params_cpu2d(orc_params *p, int nx, int nz, int dx, int dz, int npd,
     p->rec_x0 = npd; p->rec_dx = 1; p->rec_nx = nx;
     p->rec_y0 = 0; p->rec_dy = 1; p->rec_ny = 1;
     p->rec_kind = 1;                                      /* P+P :490 */
+    return 0;
+}
+
+/* main of Toafd2domp_2.c (:40-242): npd = 0 there (:67), so no absorbing
+ * layer (d0 none); t starts at 0 (:144); 7x7 Gaussian source of gain pfac
+ * (= 1000, :70) added in the stress update (:155-161, :182-184); its own
+ * Ricker product order (:154); P+Q recorded at row npd (:219). */
+int orc_params_omp2d(orc_params *p, int nx, int nz, float dx, float dz, int npd,
+                     float favg, float pfac, int sx, int sz, int nt, float dt,
+                     int rec_depth)
+{
+    memset(p, 0, sizeof(*p));
+    p->ndim = 2;
+    p->nx = nx; p->ny = 1; p->nz = nz; p->npml = npd;
+    p->dx = dx; p->dy = dx; p->dz = dz;
+    p->dt = dt;
+    p->nt = nt;
+    p->favg = favg;
+    p->pfac = pfac;
+    p->wtype = 4;
+    p->sv = 0;
+    p->d0_kind = 2;
+    p->acoff_kind = 1;
+    p->src_kind = 1;
+    p->t0_kind = 1;
+    p->omp_form = 1;
+    p->sx = (int)(sx + 0.5) + npd - 1;                    /* :150 */
+    p->sz = (int)(sz + 0.5) + npd - 1;                    /* :151 */
+    p->sv_kind = 1; p->sv_R = 15; p->sv_r = 0;
+    p->sv_cx = p->sx; p->sv_cz = p->sz;
+    p->rec_z = rec_depth + npd - 1;                       /* :219 (row npd) */
+    p->rec_x0 = npd; p->rec_dx = 1; p->rec_nx = nx;
+    p->rec_y0 = 0; p->rec_dy = 1; p->rec_ny = 1;
+    p->rec_kind = 2;
     return 0;
 }
 

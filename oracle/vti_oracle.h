@@ -54,6 +54,9 @@ typedef struct orc_params {
     int sv_cx, sv_cy, sv_cz; /* taper centre */
     int rec_z, rec_x0, rec_dx, rec_nx, rec_y0, rec_dy, rec_ny;
     int rec_kind;          /* 0 P, 1 P+P, 2 P+Q */
+    int omp_form;          /* Toafd2domp_2.c arithmetic (2D): the Pz and Qx
+                              updates in double (pow promotes them,
+                              :176-180), rounded to float once */
     int unsplit;           /* point-source form only: the product's unsplit
                               interior update (not the reference's arithmetic:
                               where every damping factor is exactly 1, P and Q
@@ -69,6 +72,13 @@ int orc_params_gpu2d(orc_params *p, int nx, int nz, int dx, int dz, int npd,
 int orc_params_cpu2d(orc_params *p, int nx, int nz, int dx, int dz, int npd,
                      int SV, int favg, int is, int fs, int ds, int zs,
                      int nt, int dt_us, int rec_depth);
+/* main of Toafd2domp_2.c (2D-vti-fd-omp/Toafd2domp_2.c:40-242): npd cells
+ * of padding with no absorbing layer, first time sample 0, in-stress 7x7
+ * Gaussian source of gain pfac (1000 there), P+Q recorded at 1-based depth
+ * rec_depth (the reference records row npd, i.e. 1). */
+int orc_params_omp2d(orc_params *p, int nx, int nz, float dx, float dz, int npd,
+                     float favg, float pfac, int sx, int sz, int nt, float dt,
+                     int rec_depth);
 int orc_params_gpu3d(orc_params *p, int nx, int ny, int nz, int dx, int dy,
                      int dz, int npml, int SV, int favg, int is, int fsx,
                      int dsx, int fsy, int dsy, int fsz, int dsz, int nt,

@@ -47,6 +47,12 @@ static REAL FN(wavelet)(REAL ts, REAL favg, int wtype)
     } else if (wtype == 3) {
         REAL x = (REAL)((-1) * favg * favg * ORC_PI * ORC_PI / log(0.1));
         src = (REAL)exp(-x * ts * ts);
+    } else if (wtype == 4) {
+        /* Toafd2domp_2.c:154: (favg*ts*favg*ts) in float inside the first
+         * factor, a double product chain inside exp */
+        REAL x4 = favg * ts * favg * ts;
+        src = (REAL)((1 - 2 * ORC_PI * ORC_PI * (x4)) *
+                     exp(-(ORC_PI * ORC_PI * (double)favg * (double)ts * (double)favg * (double)ts)));
     }
     return src;
 }
@@ -283,13 +289,14 @@ static void FN(vel)(SIM *s)
                 for (int iz = 4; iz < nnz - 4; iz++) {
                     long id = iz + (long)ix * nnz + (long)iy * sy;
                     REAL xx = 0, yy = 0, zz = 0;
+                    const int pyo = ORC_PLANT == 1, ty = ORC_PLANT == 2 ? ix : iy;
                     for (int im = 0; im < 4; im++) {
-                        yy += c[im] * (P[id + (im + 1) * sy] - P[id - im * sy]);
+                        yy += c[im] * (P[id + (im + 1 - pyo) * sy] - P[id - (im + pyo) * sy]);
                         xx += c[im] * (P[id + (im + 1) * sx] - P[id - im * sx]);
                         zz += c[im] * (Q[id + im + 1] - Q[id - im]);
                     }
                     u[id] = cx2[ix] * u[id] - cx1[ix] * dtx * xx;
-                    v[id] = cy2[iy] * v[id] - cy1[iy] * dty * yy;
+                    v[id] = cy2[ty] * v[id] - cy1[ty] * dty * yy;
                     w[id] = cz2[iz] * w[id] - cz1[iz] * dtz * zz;
                 }
     }
@@ -334,8 +341,18 @@ static void FN(stress)(SIM *s, int it, int inject, int sv)
                     continue;
                 }
                 px[id] = ax2[ix] * px[id] - ax1[ix] * vv * vv * e2 * dtx * xx;
-                pz[id] = az2[iz] * pz[id] - az1[iz] * vv * vv * sd * dtz * zz;
-                qx[id] = ax2[ix] * qx[id] - ax1[ix] * vv * vv * sd * dtx * xx;
+                if (s->p.omp_form) {
+                    /* Toafd2domp_2.c:176-180: (vp*vp)*pow(1+2*del,0.5)*dt*xx
+                     * is a double product, subtracted in double */
+                    const double sq = pow((double)(1 + 2 * dd), 0.5);
+                    pz[id] = (REAL)((double)(az2[iz] * pz[id]) -
+                                    (double)(az1[iz] * vv * vv) * sq * (double)dtz * (double)zz);
+                    qx[id] = (REAL)((double)(ax2[ix] * qx[id]) -
+                                    (double)(ax1[ix] * vv * vv) * sq * (double)dtx * (double)xx);
+                } else {
+                    pz[id] = az2[iz] * pz[id] - az1[iz] * vv * vv * sd * dtz * zz;
+                    qx[id] = ax2[ix] * qx[id] - ax1[ix] * vv * vv * sd * dtx * xx;
+                }
                 qz[id] = az2[iz] * qz[id] - az1[iz] * vv * vv * dtz * zz;
                 P[id] = px[id] + pz[id];
                 Q[id] = qx[id] + qz[id];
@@ -349,14 +366,17 @@ static void FN(stress)(SIM *s, int it, int inject, int sv)
                     REAL ee, dd;
                     FN(taper)(s, sv, ix, iy, iz, id, &ee, &dd);
                     REAL xx = 0, yy = 0, zz = 0;
+                    const int pxo = ORC_PLANT == 3;
                     for (int im = 0; im < 4; im++) {
                         yy += c[im] * (v[id + im * sy] - v[id - (im + 1) * sy]);
-                        xx += c[im] * (u[id + im * sx] - u[id - (im + 1) * sx]);
+                        xx += c[im] * (u[id + (im + pxo) * sx] - u[id - (im + 1 - pxo) * sx]);
                         zz += c[im] * (w[id + im] - w[id - im - 1]);
                     }
                     REAL vv = vp[id];
                     REAL e2 = 1 + 2 * ee;
                     REAL sd = SQRT_R(1 + 2 * dd);
+                    const REAL dtyp = ORC_PLANT == 4 ? dtx : dty;
+                    const REAL sdy = ORC_PLANT == 6 ? e2 : sd;
                     const int dX = !(ax1[ix] == 1 && ax2[ix] == 1), dY = !(ay1[iy] == 1 && ay2[iy] == 1);
                     const int dZ = !(az1[iz] == 1 && az2[iz] == 1);
                     const int fs = iz == s->split_z;
@@ -364,8 +384,8 @@ static void FN(stress)(SIM *s, int it, int inject, int sv)
                         /* unsplit interior (product option, not the reference):
                          * P -= sum of the three split increments, Q likewise */
                         REAL a = vv * vv;
-                        P[id] = P[id] - (a * e2 * dtx * xx + a * e2 * dty * yy + a * sd * dtz * zz);
-                        Q[id] = Q[id] - (a * sd * dtx * xx + a * sd * dty * yy + a * dtz * zz);
+                        P[id] = P[id] - (a * e2 * dtx * xx + a * e2 * dtyp * yy + a * sd * dtz * zz);
+                        Q[id] = Q[id] - (a * sd * dtx * xx + a * sdy * dtyp * yy + a * dtz * zz);
                         continue;
                     }
                     if (s->p.unsplit && !fs && dX + dY + dZ < 3) {
@@ -373,15 +393,15 @@ static void FN(stress)(SIM *s, int it, int inject, int sv)
                          * kept, the undamped rest is P minus them */
                         REAL a = vv * vv, z0 = 0;
                         REAL npx = dX ? ax2[ix] * px[id] - ax1[ix] * vv * vv * e2 * dtx * xx : z0;
-                        REAL npy = dY ? ay2[iy] * py[id] - ay1[iy] * vv * vv * e2 * dty * yy : z0;
+                        REAL npy = dY ? ay2[iy] * py[id] - ay1[iy] * vv * vv * e2 * dtyp * yy : z0;
                         REAL npz = dZ ? az2[iz] * pz[id] - az1[iz] * vv * vv * sd * dtz * zz : z0;
                         REAL nqx = dX ? ax2[ix] * qx[id] - ax1[ix] * vv * vv * sd * dtx * xx : z0;
-                        REAL nqy = dY ? ay2[iy] * qy[id] - ay1[iy] * vv * vv * sd * dty * yy : z0;
+                        REAL nqy = dY ? ay2[iy] * qy[id] - ay1[iy] * vv * vv * sdy * dtyp * yy : z0;
                         REAL nqz = dZ ? az2[iz] * qz[id] - az1[iz] * vv * vv * dtz * zz : z0;
                         REAL rp = ((P[id] - (dX ? px[id] : z0)) - (dY ? py[id] : z0)) - (dZ ? pz[id] : z0);
                         REAL rq = ((Q[id] - (dX ? qx[id] : z0)) - (dY ? qy[id] : z0)) - (dZ ? qz[id] : z0);
-                        REAL tp = ((dX ? z0 : a * e2 * dtx * xx) + (dY ? z0 : a * e2 * dty * yy)) + (dZ ? z0 : a * sd * dtz * zz);
-                        REAL tq = ((dX ? z0 : a * sd * dtx * xx) + (dY ? z0 : a * sd * dty * yy)) + (dZ ? z0 : a * dtz * zz);
+                        REAL tp = ((dX ? z0 : a * e2 * dtx * xx) + (dY ? z0 : a * e2 * dtyp * yy)) + (dZ ? z0 : a * sd * dtz * zz);
+                        REAL tq = ((dX ? z0 : a * sd * dtx * xx) + (dY ? z0 : a * sdy * dtyp * yy)) + (dZ ? z0 : a * dtz * zz);
                         if (dX) px[id] = npx, qx[id] = nqx;
                         if (dY) py[id] = npy, qy[id] = nqy;
                         if (dZ) pz[id] = npz, qz[id] = nqz;
@@ -390,10 +410,10 @@ static void FN(stress)(SIM *s, int it, int inject, int sv)
                         continue;
                     }
                     px[id] = ax2[ix] * px[id] - ax1[ix] * vv * vv * e2 * dtx * xx;
-                    py[id] = ay2[iy] * py[id] - ay1[iy] * vv * vv * e2 * dty * yy;
+                    py[id] = ay2[iy] * py[id] - ay1[iy] * vv * vv * e2 * dtyp * yy;
                     pz[id] = az2[iz] * pz[id] - az1[iz] * vv * vv * sd * dtz * zz;
                     qx[id] = ax2[ix] * qx[id] - ax1[ix] * vv * vv * sd * dtx * xx;
-                    qy[id] = ay2[iy] * qy[id] - ay1[iy] * vv * vv * sd * dty * yy;
+                    qy[id] = ay2[iy] * qy[id] - ay1[iy] * vv * vv * sdy * dtyp * yy;
                     qz[id] = az2[iz] * qz[id] - az1[iz] * vv * vv * dtz * zz;
                     P[id] = px[id] + py[id] + pz[id];
                     Q[id] = qx[id] + qy[id] + qz[id];
@@ -449,6 +469,14 @@ static int FN(step_ex)(SIM *s, int it, int inject, int sv, REAL *rec)
             FN(vel)(s);
             s->f[9][id] = p0;
             s->f[10][id] = q0;
+            FN(stress)(s, it, inject, sv);
+            if (rec) FN(record)(s, rec);
+            return 0;
+        }
+        if (a != 0 && ORC_PLANT == 5) {
+            FN(vel)(s);
+            s->f[9][id] += a;
+            s->f[10][id] += a;
             FN(stress)(s, it, inject, sv);
             if (rec) FN(record)(s, rec);
             return 0;

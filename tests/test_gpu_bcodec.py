@@ -46,14 +46,19 @@ def _field(seed=7, steps=120):
         return prop.field("P")      # padded (nny, nnx, nnz)
 
 
-def _codec(A, torch, P_pad, n, tol, cap, off_yxz=(NPML, NPML, NPML), floor=1e-8):
-    """Encode the n^3 box of the padded array at off_yxz and decode it."""
+SENTINEL = 0xDEADBEEF
+
+
+def _codec(A, torch, P_pad, n, tol, cap, off_yxz=(NPML, NPML, NPML), floor=1e-8, guard=0):
+    """Encode the n^3 box of the padded array at off_yxz and decode it.  The
+    entry buffer holds cap entries and `guard` more, all SENTINEL before the
+    call; the returned entries include the guard."""
     dev = torch.device("cuda:0")
     Pd = torch.from_numpy(np.ascontiguousarray(P_pad)).to(dev)
     ny, nx, nz = P_pad.shape
     off = off_yxz[2] + off_yxz[1] * nz + off_yxz[0] * nx * nz
     nblk = (n // 8) ** 3
-    ent = torch.full((max(cap, 1),), 0xDEADBEEF - 2 ** 32, dtype=torch.int32, device=dev)
+    ent = torch.full((max(cap, 1) + guard,), SENTINEL - 2 ** 32, dtype=torch.int32, device=dev)
     boff = torch.zeros(nblk, dtype=torch.int32, device=dev)
     bcnt = torch.zeros(nblk, dtype=torch.int16, device=dev)
     torch.cuda.synchronize()         # the codec runs on the autoencoder's own stream
@@ -142,9 +147,17 @@ def test_codec_strided_box_and_capacity():
         kept, err, out, ent, _, _ = _codec(A, torch, big, n, 0.1, n ** 3, off_yxz=(2, 1, 5))
         box = big[2:2 + n, 1:1 + n, 5:5 + n]
         assert rel_l2(out, box) <= err + 1e-6 and err <= 0.1
-        kept2, _, out2, ent2, _, _ = _codec(A, torch, big, n, 0.1, kept - 1, off_yxz=(2, 1, 5))
+        cap, guard = kept - 1, 4096
+        kept2, _, out2, ent2, boff2, bcnt2 = _codec(A, torch, big, n, 0.1, cap, off_yxz=(2, 1, 5), guard=guard)
     assert kept2 == kept and out2 is None
-    assert len(ent2) == kept - 1             # the buffer held exactly cap entries and nothing faulted
+    assert len(ent2) == cap + guard
+    assert np.all(ent2[cap:] == SENTINEL)    # nothing written past cap
+    ends = boff2.astype(np.int64) + bcnt2.astype(np.int64)
+    fit = ends <= cap
+    assert fit.any()
+    for b in np.nonzero(fit)[0]:             # the blocks that fit hold their entries
+        e = ent2[boff2[b]:ends[b]]
+        assert not np.any(e == SENTINEL)
 
 
 @pytest.mark.parametrize("n", [24, 8])

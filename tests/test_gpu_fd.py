@@ -195,6 +195,13 @@ def test_dropin_gpu_vti2dfd_and_3dfd_files(tmp_path):
     assert np.array_equal(out3, ref3)
     snap3 = np.fromfile(snap, dtype=np.float32).reshape(-1, ny3, nx3, nz3)
     assert snap3.shape[0] == 1
+    # the frame is the interior of P after step 300 (GPU_vti3dfd_kernels.cu:526-532)
+    sims = po.Sim(p3, vp3, e3, d3, "f32")
+    for it in range(301):
+        sims.step(it, rec=False)
+    win = sims.field("P")[np3:np3 + ny3, np3:np3 + nx3, np3:np3 + nz3]
+    assert np.abs(win).max() > 0
+    assert np.array_equal(snap3[0], win)
 
 
 def test_config1_full_size_parity():
@@ -425,3 +432,58 @@ def test_config2_full_size_parity():
     _say(f"512^3, {nt} steps: split bit-exact; unsplit bit-exact to the oracle's unsplit mode, "
          f"{err:.2e} from the split records")
     assert err <= 1e-5
+
+
+def test_omp_form_dropin_matches_toafd2domp_golden(tmp_path):
+    """vti_omp2dfd (main of Toafd2domp_2.c with its constants as
+    arguments) against the frames and shot that program wrote: all 81
+    snapshot frames of 601 x 301 / 4001 steps bit for bit (sha256), and the
+    shot record (row npd = 0, never updated: zeros)."""
+    import hashlib
+    g = load_golden("omp2d_601x301")
+    nx, nz, dx, dz, npd, favg, pfac, sx, sz, nt, dt = g["params"]
+    nx, nz, nt = int(nx), int(nz), int(nt)
+    vp = (g["vz"][None, :] + g["vx"][:, None]).astype(np.float32)
+    e = np.tile(g["ez"], (nx, 1))
+    d = np.tile(g["dz"], (nx, 1))
+    fv, fe, fd = _write(str(tmp_path), v=vp, e=e, d=d)
+    shot, snap = str(tmp_path / "shot.dat"), str(tmp_path / "snap.dat")
+    vti.omp2dfd(fv, fe, fd, shot, snap, nx=nx, nz=nz, dx=float(dx), dz=float(dz), npd=int(npd),
+                favg=float(favg), pfac=float(pfac), sx=int(sx), sz=int(sz), nt=nt, dt=float(np.float32(dt)))
+    frames = np.fromfile(snap, dtype=np.float32).reshape(-1, nx, nz)
+    rec = np.fromfile(shot, dtype=np.float32).reshape(nx, nt)
+
+    def sha(a):
+        return np.frombuffer(hashlib.sha256((a + np.float32(0)).tobytes()).digest(), dtype=np.uint8)
+    assert frames.shape[0] == g["frame_sha"].shape[0]
+    assert np.array_equal(frames[2, 59:140, 59:140], g["crop100"])
+    assert np.array_equal(frames[4, 19:180, 19:180], g["crop200"])
+    for k in range(frames.shape[0]):
+        assert np.array_equal(sha(frames[k]), g["frame_sha"][k]), f"frame {k} (step {50 * k})"
+    assert np.array_equal(sha(rec), g["shot_sha"])
+
+
+def test_omp_form_context_records_p_plus_q():
+    """VTI_MODE_OMP_REF through the context API with receivers inside the
+    updated region: the P+Q record and the final fields equal the oracle's
+    Toafd2domp_2.c form bit for bit; receivers outside it (columns 0..3)
+    record 0, as the reference does."""
+    nx, nz, npd, nt = 200, 120, 0, 700
+    vp, e, d = layered_model(nx, nz, seed=31, elo=0.1, ehi=0.2, dlo=0.0, dhi=0.1)
+    cfg = vti.cfg_ref2d(vti.MODE_OMP_REF, nx, nz, 5, 5, npd, 0, 30, nt, 500)
+    assert cfg.pfac == 1000.0 and cfg.wtype == 4
+    with vti.Propagator(cfg, vp, e, d) as prop:
+        sh = vti.shot_ref2d(cfg, 1, 100, 0, 60, rec_depth=40)
+        prop.set_shot(sh)
+        prop.run()
+        rec = prop.record()
+        P = prop.field("P")
+    p = po.params_omp2d(nx, nz, 5.0, 5.0, npd, 30.0, 1000.0, 100, 60, nt, float(np.float32(0.0005)), 40)
+    sim = po.Sim(p, vp, e, d, "f32")
+    ref = sim.run()
+    assert np.abs(ref).max() > 0
+    assert np.array_equal(rec, ref), rel_l2(rec, ref)
+    assert not rec[:4].any() and not rec[-4:].any()
+    assert np.array_equal(P, sim.field("P"))
+    r64 = po.Sim(p, vp, e, d, "f64").run()
+    _say(f"Toafd2domp_2.c form: rel-L2 vs FP64 oracle {rel_l2(rec, r64):.3e}")

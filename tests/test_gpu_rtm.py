@@ -105,11 +105,19 @@ def test_rtm_3d_matches_oracle_bitwise():
         prop.rtm_reset()
         prop.rtm_shot(o, obs)
         img, ill = prop.rtm_image(finalize=False)
+        fin, _ = prop.rtm_image(finalize=True)
     p = po.params_gpu3d(nx, ny, nz, 10, 10, 10, npml, 1, 25, 0, 14, 0, 13, 0, 1, 0, nt, 800, 1, 2)
     p.rec_kind = 2
     rimg, rill, _ = po.rtm(p, vp, e, d, sv_fwd=1, sv_bwd=0, inject_z=npml - 1, illum_mode=1, obs=obs)
     assert np.array_equal(ill, rill)
     assert np.array_equal(img, rimg), rel_l2(img, rimg)
+    # finalize: divide where the illumination is nonzero (3D form,
+    # Toa_gpu_3dvti_rtm_adcigs_1orderfunction.cu:550-551), then the 3D
+    # adjoint Laplacian (laplace_3d_filter :555-616)
+    q = np.divide(rimg, rill, out=rimg.copy(), where=rill != 0).astype(np.float32)
+    rfin = po.laplace_adj(q)
+    assert np.abs(rfin).max() > 0
+    assert np.array_equal(fin, rfin), rel_l2(fin, rfin)
 
 
 def test_rtm_3d_unsplit_interior():

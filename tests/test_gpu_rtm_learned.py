@@ -63,13 +63,16 @@ def test_fallback_equals_plain_rtm_bitwise(exact):
     assert np.array_equal(img_l, img_p), rel_l2(img_l, img_p)
 
 
-def test_fitted_autoencoder_image_error():
-    """Principal-component fit on two shots' snapshots, then a third shot
-    imaged from latents, without and with the exact-storage fallback,
-    against the plain RTM image of the same steps: the raw image and the
-    final image (divided by the illumination, Laplacian-filtered).  This
-    32^3 / 25 Hz case is far harder for 16x compression than the 15 Hz
-    benchmark (more energy near the grid Nyquist)."""
+def test_exact_fallback_reduces_the_latent_image_error():
+    """Property test, not the 1 % bar (the 384^3 benchmark test asserts that
+    one): principal-component fit on two shots' snapshots, then a third
+    shot imaged from latents, without and with the exact-storage fallback,
+    against the plain RTM image of the same steps (raw image, and final
+    image divided by the illumination and Laplacian-filtered).  The fit
+    samples random blocks, so its captured energy varies from run to run
+    (0.9877 to 0.995 measured); the assertion is a floor below that range.
+    This 32^3 / 25 Hz case is far harder for 16x compression than the
+    15 Hz benchmark (more energy near the grid Nyquist)."""
     cfg, vp, e, d = _setup()
     with vti.Propagator(cfg, vp, e, d) as prop, vti.Autoencoder(crop=32, batch=2, seed=2) as A:
         for cx, cy in ((10, 12), (22, 20)):
