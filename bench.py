@@ -9,9 +9,23 @@ surface centre; 256 x 256 receivers (every 2nd cell) at z = 3.
 
 A step is one velocity update plus one stress update of the whole padded
 grid (572^3) with the fused source / damping / receiver work.  Inputs are
-resident in HBM before the timed region.  Prints one JSON line.
+resident in HBM before the timed region.  Prints one JSON line, which also
+carries:
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--no-cpu]
+  * cpu_baseline: the oracle (the C restatement, built for the host with
+    -O3 -march=native -ffp-contract=off) on a bounded sample of the same
+    workload, FP32 and FP64, all host threads and one, with the records of
+    the sampled steps compared with the GPU's;
+  * config1_2d: the 2D 400 x 400 / 2000-step configuration of the
+    reference's CPU path on the GPU and on the host (oracle, and the
+    reference CPU path itself where oracle/_ref holds it), with the errors
+    between them;
+  * learned_rtm: the second half of the metric, 3D RTM per-shot time with
+    learned snapshot compression against plain checkpoint recomputation at
+    equal memory (384^3, 4000 steps, 3 fitting shots, one shot imaged both
+    ways), with the final-image error.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--no-cpu] [--no-learned]
 """
 import argparse
 import json
@@ -78,29 +92,186 @@ def make_model(n, seed, device):
     return vp, eps, dl
 
 
-def cpu_baseline(vp, eps, dl, n, npml, nt_total, budget_s=12.0):
-    """The oracle (C restatement, FP32, OpenMP) on a bounded sample of the
-    same workload: the full padded grid for as many steps as fit in budget_s."""
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo, as lscpu prints it)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+
+
+def native_oracle():
+    """The oracle (the checker's C restatement) built for this host with
+    -O3 -march=native -ffp-contract=off (bit-identical float results to the
+    default build), loaded in place of the default build."""
+    import subprocess
+    odir = os.path.join(ROOT, "oracle")
+    lib = os.path.join(odir, "_native", "liboracle_vti.so")
+    subprocess.run(["make", "-s", "-B", "-C", odir, "native"], check=True)
+    os.environ["ORC_LIB"] = lib
     from oracle import pyoracle as po
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    if po.library_path() != lib:
+        raise RuntimeError("the oracle was loaded before the native build")
+    return po
+
+
+CPU_FLAGS = "gcc -O3 -march=native -ffp-contract=off -fopenmp (oracle/Makefile native)"
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def cpu_legs_3d(po, vp, eps, dl, n, npml, nt_total, gpu_rec, budget_s=8.0):
+    """The oracle on a bounded sample of the 3D workload, split form (the
+    reference arithmetic), FP32 and FP64, all host threads and one thread:
+    the full padded grid for as many steps as fit in budget_s (all threads;
+    one step on one thread), after one untimed step that first-touches the
+    arrays.  The records of the sampled steps are compared with the GPU
+    run's."""
+    allc = host_threads()
     c = n // 2
     p = po.params_gpu3d(n, n, n, 10, 10, 10, npml, 1, 15, 0, c, 0, c, 0, 1, 0, nt_total, 800, 3, 2)
-    sim = po.Sim(p, vp, eps, dl, "f32")
     cells = (n + 2 * npml - 8) ** 3
-    sim.step(0)          # first touch of the arrays outside the timed loop
-    t0 = time.perf_counter()
-    steps = 0
-    while True:
-        sim.step(1 + steps)
-        steps += 1
-        if time.perf_counter() - t0 > budget_s or steps >= 50:
-            break
-    dt = time.perf_counter() - t0
-    sim.close()
-    return {"value": round(cells * steps / dt / 1e9, 4), "unit": "Gcell-updates/s", "cores": threads,
-            "kind": "port",
-            "sample": f"oracle/vti_oracle.c FP32 on the full {n}^3 workload ({n + 2 * npml}^3 padded), "
-                      f"{steps} steps after the first, {threads} OpenMP threads, {dt:.1f} s"}
+    legs, recs = [], {}
+    for prec in ("f32", "f64"):
+        sim = po.Sim(p, vp, eps, dl, prec)
+        po.set_threads(allc)
+        rows = [sim.step(0)]
+        t0 = time.perf_counter()
+        while True:
+            rows.append(sim.step(len(rows)))
+            if time.perf_counter() - t0 > budget_s or len(rows) > 40:
+                break
+        dt_all = time.perf_counter() - t0
+        k_all = len(rows) - 1
+        legs.append({"precision": prec, "threads": allc, "steps": k_all, "seconds": round(dt_all, 2),
+                     "gcells": round(cells * k_all / dt_all / 1e9, 4)})
+        po.set_threads(1)
+        t0 = time.perf_counter()
+        rows.append(sim.step(len(rows)))
+        dt_one = time.perf_counter() - t0
+        legs.append({"precision": prec, "threads": 1, "steps": 1, "seconds": round(dt_one, 2),
+                     "gcells": round(cells / dt_one / 1e9, 4)})
+        po.set_threads(allc)
+        recs[prec] = np.stack(rows, axis=1)
+        sim.close()
+        log(f"[bench] CPU 3D {prec}: {legs[-2]['gcells']} Gcell/s on {allc} threads, "
+            f"{legs[-1]['gcells']} on 1")
+    k = min(gpu_rec.shape[1], recs["f32"].shape[1])
+    errs = {"steps_compared": k,
+            "gpu_vs_cpu_f32": rel(gpu_rec[:, :k], recs["f32"][:, :k]),
+            "gpu_vs_cpu_f64": rel(gpu_rec[:, :k], recs["f64"][:, :k]),
+            "cpu_f32_vs_cpu_f64": rel(recs["f32"][:, :k], recs["f64"][:, :k])}
+    return legs, errs
+
+
+def layered_1d(n, nlayer, seed, lo, hi):
+    rng = np.random.default_rng(seed)
+    vals = np.sort(rng.uniform(lo, hi, nlayer))
+    return vals[np.minimum(np.arange(n) * nlayer // n, nlayer - 1)]
+
+
+def bench_config1(torch, vti, po):
+    """The 2D configuration the reference's CPU path runs: 400 x 400, dx 10 m,
+    dt 1 ms, 2000 steps, 20-cell boundary, 20 Hz Ricker at (x=200, z=2),
+    receivers at every x at z=2, 4 flat layers (vp 1500..4000 m/s increasing
+    with depth, eps in [0, 0.2], delta in [0, 0.1] kept <= eps), with the
+    CPU driver's semantics (CPU_vti2dfd_kernels.c: 7x7 Gaussian source,
+    P+P record) on the GPU, the oracle on the host (FP32, FP64; all threads
+    and one) and, where it was built, the reference CPU path itself."""
+    import subprocess
+    import tempfile
+    nx = nz = 400
+    npd, nt, favg = 20, 2000, 20
+    vz = layered_1d(nz, 4, 11, 1500.0, 4000.0)
+    ez = layered_1d(nz, 4, 12, 0.0, 0.2)
+    dz = np.minimum(layered_1d(nz, 4, 13, 0.0, 0.1), ez)
+    vp = np.tile(vz, (nx, 1)).astype(np.float32)
+    e = np.tile(ez, (nx, 1)).astype(np.float32)
+    d = np.tile(dz, (nx, 1)).astype(np.float32)
+    cfg = vti.cfg_ref2d(vti.MODE_CPU_REF, nx, nz, 10, 10, npd, 0, favg, nt, 1000)
+    cells = (nx + 2 * npd - 8) * (nz + 2 * npd - 8)
+    with vti.Propagator(cfg, vp, e, d) as prop:
+        prop.set_shot(vti.shot_ref2d(cfg, 1, 200, 0, 2, rec_depth=2))
+        prop.run()                       # warm-up
+        prop.sync()
+        prop.reset()
+        prop.sync()
+        t0 = time.perf_counter()
+        prop.step(0, nt)
+        prop.sync()
+        t_gpu = time.perf_counter() - t0
+        rec = prop.record()
+    allc = host_threads()
+    p = po.params_cpu2d(nx, nz, 10, 10, npd, 0, favg, 1, 200, 0, 2, nt, 1000, 2)
+    legs, recs = [], {}
+    for prec in ("f32", "f64"):
+        po.set_threads(allc)
+        sim = po.Sim(p, vp, e, d, prec)
+        t0 = time.perf_counter()
+        recs[prec] = sim.run()
+        t_all = time.perf_counter() - t0
+        sim.close()
+        legs.append({"precision": prec, "threads": allc, "steps": nt, "seconds": round(t_all, 3),
+                     "gcells": round(cells * nt / t_all / 1e9, 4)})
+        po.set_threads(1)
+        k = 300
+        sim = po.Sim(p, vp, e, d, prec)
+        t0 = time.perf_counter()
+        for it in range(k):
+            sim.step(it, rec=False)
+        t_one = time.perf_counter() - t0
+        sim.close()
+        legs.append({"precision": prec, "threads": 1, "steps": k, "seconds": round(t_one, 3),
+                     "gcells": round(cells * k / t_one / 1e9, 4)})
+    po.set_threads(allc)
+    out = {"workload": "2D VTI FD, 400x400, dx 10 m, dt 1 ms, 2000 steps, 20-cell boundary, 20 Hz Ricker at "
+                       "(200, 2), 400 receivers at z=2, 4 flat layers (delta clipped to <= eps), CPU driver "
+                       "semantics (CPU_vti2dfd_kernels.c)",
+           "cells_per_step": cells,
+           "gpu": {"seconds": round(t_gpu, 4), "ms_per_step": round(t_gpu / nt * 1e3, 4),
+                   "gcells": round(cells * nt / t_gpu / 1e9, 3)},
+           "cpu_oracle": legs, "cpu_flags": CPU_FLAGS,
+           "rel_l2_gpu_vs_cpu_f32": rel(rec, recs["f32"]), "gpu_equals_cpu_f32_bitwise": bool(np.array_equal(rec, recs["f32"])),
+           "rel_l2_gpu_vs_cpu_f64": rel(rec, recs["f64"]), "rel_l2_cpu_f32_vs_f64": rel(recs["f32"], recs["f64"])}
+    ref = os.path.join(ROOT, "oracle", "_ref", "cpu_vti2dfd_ref")
+    if os.path.exists(ref):
+        # the reference's own CPU path (CPU_vti2dfd_kernels.c, compiled from
+        # its source by oracle/Makefile, -O2): a bounded sample of the steps
+        k = 300
+        with tempfile.TemporaryDirectory() as td:
+            for name, a in (("v", vp), ("e", e), ("d", d)):
+                a.tofile(os.path.join(td, name))
+            cmd = [ref, str(nx), str(nz), "10", "10", str(npd), "0"] + [os.path.join(td, x) for x in "ved"] + \
+                [str(favg), "1", "200", "0", "2", os.path.join(td, "shot"), os.path.join(td, "snap"), str(k), "1000"]
+            t0 = time.perf_counter()
+            subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL)
+            t_ref = time.perf_counter() - t0
+            shot = np.fromfile(os.path.join(td, "shot"), dtype=np.float32).reshape(nx, k)
+        # the reference records row npd (z = 1); the GPU run above records
+        # z = 2, so compare a GPU run of the reference geometry
+        with vti.Propagator(cfg, vp, e, d) as prop:
+            prop.set_shot(vti.shot_ref2d(cfg, 1, 200, 0, 2, rec_depth=1))
+            prop.reset()
+            prop.step(0, k)
+            rec1 = prop.record()[:, :k]
+        out["reference_cpu_path"] = {
+            "steps": k, "seconds": round(t_ref, 3), "gcells": round(cells * k / t_ref / 1e9, 5), "threads": 1,
+            "build": "gcc -O2 -ffp-contract=off of 2D-3D-vti-fd-cuda-gtk-GUI/CPU_vti2dfd_kernels.c (oracle/Makefile ref)",
+            "gpu_equals_reference_bitwise": bool(np.array_equal(rec1, shot)), "rel_l2_gpu_vs_reference": rel(rec1, shot)}
+    log(f"[bench] config 1: GPU {out['gpu']['gcells']} Gcell/s; CPU {legs[0]['gcells']} Gcell/s on {allc} threads")
+    return out
 
 
 def copy_calibration(torch, vti, nbytes=4 << 30, reps=20):
@@ -298,20 +469,21 @@ def bench_fwi_grad(args, torch, vti):
     print(json.dumps(out), flush=True)
 
 
-def bench_rtm3d_learned(args, torch, vti):
+def learned_rtm(args, torch, vti, nshot, nt, n, rows_out=None):
     """3D RTM with learned snapshot compression against plain checkpoint
-    recomputation: 384^3 grid, dx 10 m, 4000 steps, dt 0.8 ms, smoothed
+    recomputation: an n^3 grid, dx 10 m, nt steps, dt 0.8 ms, smoothed
     random vp / eps / delta as the headline workload; snapshots every 4
     steps, compressed with the block basis fitted online on the first 3
     shots' snapshots, applied to the following shots; per-shot wall time and
-    image error against the plain path imaging the same steps, the plain
-    path given as many checkpoint slots as the compressed store's bytes.
-    codec "sparse": error-bounded sparse coefficients (every snapshot within
+    image error against the plain path imaging the same steps.  Equal
+    memory: the plain path gets as many checkpoint slots as fit in the
+    device bytes the learned call held for the stored source field (store
+    as allocated, source boxes, work buffers; stats alloc_bytes).  codec
+    "sparse": error-bounded sparse coefficients (every snapshot within
     relative L2 error --tol); codec "latent": the autoencoder's 64-channel
-    latents with the exact-storage fallback (--exact-frac)."""
-    n = args.size if args.size != 512 else 384
-    nt = args.nt or 4000
-    nshot = args.shots or 19
+    latents with the exact-storage fallback (--exact-frac).  Returns the
+    result record; each imaged shot is logged (and appended to rows_out)
+    as it finishes."""
     npml = args.npml
     sparse = args.codec == "sparse"
     log(f"[bench] learned RTM ({args.codec} codec): {n}^3, nt {nt}, {nshot} shots")
@@ -328,10 +500,8 @@ def bench_rtm3d_learned(args, torch, vti):
     A = vti.Autoencoder(crop=64, batch=4, seed=args.seed)
     nsave = (nt + 3) // 4
     max_exact = max(1, int(nsave * args.exact_frac))   # exact-storage fallback budget (latent codec)
-    lat_bytes = nsave * vti.Autoencoder.latent_elems(n) * 2
-    exact_bytes = max_exact * n ** 3 * 4
-    # one checkpoint slot of the plain path: the 11 state fields of the padded grid
-    state_bytes = 11 * 4 * (n + 2 * npml) ** 2 * ((n + 2 * npml + 31) // 32 * 32)
+    state_bytes = prop.state_bytes()                   # one checkpoint slot of the plain path
+    fp32_bytes = nsave * n ** 3 * 4
     rows = []
     t_fit = 0.0
     ntrain = min(3, nshot)
@@ -350,7 +520,7 @@ def bench_rtm3d_learned(args, torch, vti):
         lo.src_box = args.src_box
         if sparse:
             lo.codec, lo.tol, lo.floor = 1, args.tol, args.floor
-            lo.budget = int(nsave * n ** 3 * 4 * args.store_frac)
+            lo.budget = int(fp32_bytes * args.store_frac)
         else:
             lo.check_every, lo.tol, lo.ck_every, lo.max_exact = 1, 0.02, 0, max_exact
         ro = prop.rtm_defaults()
@@ -362,8 +532,8 @@ def bench_rtm3d_learned(args, torch, vti):
         t_learn = time.perf_counter() - t0
         img_l, _ = prop.rtm_image(finalize=False)
         fin_l, _ = prop.rtm_image(finalize=True)
-        store = st["store_bytes"]
-        nsnap_equal = max(1, int(store // state_bytes))
+        store, held = st["store_bytes"], st["alloc_bytes"]
+        nsnap_equal = max(1, int(held // state_bytes))
         o = prop.rtm_defaults()
         o.image_every = 4
         o.nsnap = nsnap_equal
@@ -376,46 +546,62 @@ def bench_rtm3d_learned(args, torch, vti):
         t_plain = time.perf_counter() - t0
         img_p, _ = prop.rtm_image(finalize=False)
         fin_p, _ = prop.rtm_image(finalize=True)
-        err_raw = float(np.linalg.norm(img_l - img_p) / max(np.linalg.norm(img_p), 1e-30))
-        err = float(np.linalg.norm(fin_l - fin_p) / max(np.linalg.norm(fin_p), 1e-30))
-        rows.append(dict(t_plain=t_plain, t_learn=t_learn, err=err, err_raw=err_raw, fsteps=fsteps, st=st,
-                         slots=nsnap_equal, store=store))
-        log(f"[bench] shot {k}: learned {t_learn:.2f} s (store {store / 1e9:.2f} GB = "
-            f"{nsave * n ** 3 * 4 / store:.2f}x below fp32, snapshot err {st['mean_checked_err']:.3e}), "
-            f"plain {t_plain:.2f} s ({fsteps} fwd steps, {nsnap_equal} slots), "
-            f"final image rel-L2 {err:.3e} (raw {err_raw:.3e})")
+        err_raw = rel(img_l, img_p)
+        err = rel(fin_l, fin_p)
+        row = dict(shot=k, t_plain=round(t_plain, 3), t_learn=round(t_learn, 3), speedup=round(t_plain / t_learn, 3),
+                   err=err, err_raw=err_raw, fsteps=fsteps, slots=nsnap_equal, store=store, held=held,
+                   enc_ms=st["encode_ms"], dec_ms=st["decode_ms"], snap_err=st["mean_checked_err"],
+                   exact=st["exact"], recomputed=st["recomputed"])
+        rows.append(row)
+        log("[bench] learned RTM row " + json.dumps(row))
+        if rows_out:
+            with open(rows_out, "a") as f:
+                f.write(json.dumps(row) + "\n")
     prop.close()
     A.close()
 
-    def mean(key, sub=None):
-        v = [(r[key][sub] if sub else r[key]) for r in rows]
+    def mean(key):
+        v = [r[key] for r in rows]
         return float(np.mean(v)) if v else None
 
     tp, tl = mean("t_plain"), mean("t_learn")
-    out = {
+    return {
         "metric": "3D RTM shot time, learned snapshot compression vs plain checkpoint recomputation at equal memory",
-        "value": round(tp / tl, 3) if rows else None, "unit": "speedup (plain s / learned s)", "n_gpus": 1,
-        "steps": nt, "warmup": 0, "ms_per_step": round(tl / nt * 1e3, 4) if rows else None,
+        "speedup": round(tp / tl, 3) if rows else None, "unit": "plain s / learned s",
+        "workload": f"3D VTI RTM {n}^3, {nt} steps, dt 0.8 ms, 15 Hz, {ntrain} fitting shots then "
+                    f"{len(rows)} shot(s) imaged both ways",
+        "codec": args.codec, "block_tol": args.tol if sparse else None, "block_floor": args.floor if sparse else None,
+        "stress_update": "split" if args.split else "reduced split (both sides; the receiver injection row split)",
+        "src_box": args.src_box,
+        "plain_s_per_shot": round(tp, 3) if rows else None,
+        "learned_s_per_shot": round(tl, 3) if rows else None,
+        "final_image_rel_l2_max": max((r["err"] for r in rows), default=None),
+        "final_image_rel_l2_mean": mean("err"), "raw_image_rel_l2_mean": mean("err_raw"),
+        "snapshot_rel_err_mean": mean("snap_err"),
+        "plain_checkpoint_slots": [r["slots"] for r in rows],
+        "plain_forward_steps": [r["fsteps"] for r in rows],
+        "store_GB_mean": round(mean("store") / 1e9, 3) if rows else None,
+        "held_GB_mean": round(mean("held") / 1e9, 3) if rows else None,
+        "compression_vs_fp32_snapshots": round(fp32_bytes / mean("store"), 3) if rows else None,
+        "fit_s": round(t_fit, 2), "fit_energy_captured_64": explained, "snapshots": nsave,
+        "exact_snapshots_mean": mean("exact"), "recomputed_snapshots_mean": mean("recomputed"),
+        "encode_gpu_ms_per_shot": round(mean("enc_ms"), 1) if rows else None,
+        "decode_gpu_ms_per_shot": round(mean("dec_ms"), 1) if rows else None,
+        "rows": rows,
+    }
+
+
+def bench_rtm3d_learned(args, torch, vti):
+    """The learned-vs-plain RTM workload on its own line (--workload
+    rtm3d-learned): 384^3, 4000 steps, 19 shots (3 fitting, 16 imaged)."""
+    n = args.size if args.size != 512 else 384
+    r = learned_rtm(args, torch, vti, args.shots or 19, args.nt or 4000, n, rows_out=args.rows_out)
+    out = {
+        "metric": r["metric"], "value": r["speedup"], "unit": "speedup (plain s / learned s)", "n_gpus": 1,
+        "steps": args.nt or 4000, "warmup": 0,
+        "ms_per_step": round(r["learned_s_per_shot"] / (args.nt or 4000) * 1e3, 4) if r["rows"] else None,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16", "data": "synthetic",
-        "config": {"workload": f"3D VTI RTM {n}^3, {nt} steps, dt 0.8 ms, 15 Hz, {ntrain} fitting shots then "
-                               f"{len(rows)} shots imaged both ways",
-                   "codec": args.codec, "block_tol": args.tol if sparse else None,
-                   "stress_update": "split" if args.split else "reduced split (both sides; the receiver "
-                                                               "injection row split)",
-                   "block_floor": args.floor if sparse else None, "src_box": args.src_box,
-                   "plain_s_per_shot": round(tp, 3) if rows else None,
-                   "learned_s_per_shot": round(tl, 3) if rows else None,
-                   "final_image_rel_l2_max": max((r["err"] for r in rows), default=None),
-                   "final_image_rel_l2_mean": mean("err"), "raw_image_rel_l2_mean": mean("err_raw"),
-                   "snapshot_rel_err_mean": mean("st", "mean_checked_err"),
-                   "plain_checkpoint_slots": [r["slots"] for r in rows],
-                   "plain_forward_steps": [r["fsteps"] for r in rows],
-                   "store_GB_mean": round(mean("store") / 1e9, 3) if rows else None,
-                   "compression_vs_fp32_snapshots": round(nsave * n ** 3 * 4 / mean("store"), 2) if rows else None,
-                   "fit_s": round(t_fit, 2), "fit_energy_captured_64": explained, "snapshots": nsave,
-                   "exact_snapshots_mean": mean("st", "exact"), "recomputed_snapshots_mean": mean("st", "recomputed"),
-                   "encode_gpu_ms_per_shot": round(mean("st", "encode_ms"), 1) if rows else None,
-                   "decode_gpu_ms_per_shot": round(mean("st", "decode_ms"), 1) if rows else None},
+        "config": {k: v for k, v in r.items() if k not in ("metric", "speedup", "unit")},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
@@ -454,11 +640,17 @@ def main():
     ap.add_argument("--src-box", type=int, default=32,
                     help="learned RTM: half-width of the exactly kept box around the source")
     ap.add_argument("--store-frac", type=float, default=0.9,
-                    help="learned RTM, sparse codec: store allocated as this fraction of the fp32 snapshots "
-                         "(only the bytes used count)")
+                    help="learned RTM, sparse codec: most the entry store may take, as a fraction of the fp32 "
+                         "snapshots (it is allocated in 4 GB chunks as needed)")
     ap.add_argument("--floor", type=float, default=1e-6,
                     help="learned RTM, sparse codec: blocks below this fraction of the strongest block's "
                          "energy are held to tol of that fraction")
+    ap.add_argument("--rows-out", default=None,
+                    help="learned RTM: append each imaged shot's result to this file as it finishes")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="fd3d: leave out the learned-vs-plain 3D RTM record (384^3, 4000 steps, one imaged shot)")
+    ap.add_argument("--learned-shots", type=int, default=4,
+                    help="fd3d: shots of the learned-vs-plain record (3 fitting, the rest imaged both ways)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the library has no multi-GPU path "
@@ -582,15 +774,35 @@ def main():
             "traffic_rate": round(traffic / str_avg / 1e9, 1) if traffic else None,
         },
     }
+    gpu_rec = prop.record()[:, : args.steps].copy()
     prop.close()
+    h = [t.cpu().numpy() for t in (vp, eps, dl)]
+    del vp, eps, dl
+    torch.cuda.empty_cache()
+    out["config"]["workload"] += ("; delta clipped to <= eps (pseudo-acoustic VTI grows without bound "
+                                  "where delta > eps)")
+    out["cpu_baseline"] = None
     if not args.no_cpu:
-        log("[bench] CPU baseline (oracle) ...")
-        h = [t.cpu().numpy() for t in (vp, eps, dl)]
-        del vp, eps, dl
-        torch.cuda.empty_cache()
-        out["cpu_baseline"] = cpu_baseline(h[0], h[1], h[2], n, npml, nt)
-    else:
-        out["cpu_baseline"] = None
+        log("[bench] CPU baseline (oracle built for this host) ...")
+        po = native_oracle()
+        legs, errs = cpu_legs_3d(po, h[0], h[1], h[2], n, npml, nt, gpu_rec)
+        main_leg = legs[0]                 # FP32, all host threads
+        out["cpu_baseline"] = {
+            "value": main_leg["gcells"], "unit": "Gcell-updates/s", "cores": main_leg["threads"], "kind": "port",
+            "sample": f"oracle/vti_oracle.c (split form, FP32) on the full {n}^3 workload "
+                      f"({n + 2 * npml}^3 padded), {main_leg['steps']} steps after the first, "
+                      f"{main_leg['threads']} OpenMP threads, {main_leg['seconds']} s; {cpu_model()}; {CPU_FLAGS}",
+            "legs": legs, "cpu_model": cpu_model(), "nproc": len(os.sched_getaffinity(0)),
+            "flags": CPU_FLAGS, "records_rel_l2": errs}
+        del h
+        out["config1_2d"] = bench_config1(torch, vti, po)
+    if not args.no_learned:
+        try:
+            out["learned_rtm"] = learned_rtm(args, torch, vti, max(4, args.learned_shots), 4000, 384,
+                                             rows_out=args.rows_out)
+        except Exception as exc:          # the headline line is still printed
+            log(f"[bench] learned RTM record failed: {exc}")
+            out["learned_rtm"] = {"error": str(exc)}
     print(json.dumps(out), flush=True)
 
 

@@ -25,6 +25,9 @@ namespace vti {
 int fail(int code, const std::string &msg)
 {
     set_error(msg);
+    // a failed hipMalloc leaves its error as the thread's last HIP error;
+    // read it here so that the next launch's check does not report it
+    if (code == VTI_ERR_ALLOC) (void)hipGetLastError();
     return code;
 }
 }  // namespace vti
@@ -391,6 +394,8 @@ int vti_destroy(vti_ctx *c)
     hipFree(c->alt);
     hipFree(c->img);
     hipFree(c->illum);
+    hipFree(c->adcig);
+    hipFree(c->adcig_tab);
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
     delete c;
     return VTI_OK;

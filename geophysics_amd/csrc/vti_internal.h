@@ -169,6 +169,11 @@ struct vti_ctx {
     float *img = nullptr, *illum = nullptr;
     size_t nint = 0;            // nx*ny*nz
     int rtm_illum_mode = 0;
+    // Poynting-vector angle gathers (vti_adcig_*): [iya][ixa][ia][iz] over
+    // the cdps, host-built bin thresholds and cos^3 weights (nangle each)
+    float *adcig = nullptr, *adcig_tab = nullptr;
+    int adcig_na = 0, adcig_da = 1, adcig_dcdp = 1;
+    size_t adcig_n = 0;
 };
 
 namespace vti {
@@ -195,4 +200,7 @@ StressArgs src_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, float
 StressArgs rcv_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, int inj_z);
 // a with the context's source illumination accumulated in its epilogue
 StressArgs with_illum(vti_ctx *c, StressArgs a, int clamp);
+// one angle-gather accumulation (vti_adcig.hip) from a source and a receiver
+// state of the context's grid, when the context has gathers configured
+hipError_t launch_adcig(vti_ctx *c, const VtiState &src, const VtiState &rcv);
 }  // namespace vti

@@ -179,7 +179,8 @@ int vti_rtm_reset(vti_ctx *c)
             return fail(VTI_ERR_ALLOC, "vti_rtm_reset: image buffers");
         c->nint = n;
     }
-    if (launch_zero(c->img, n, c->stream) != hipSuccess || launch_zero(c->illum, n, c->stream) != hipSuccess)
+    if (launch_zero(c->img, n, c->stream) != hipSuccess || launch_zero(c->illum, n, c->stream) != hipSuccess ||
+        (c->adcig && launch_zero(c->adcig, c->adcig_n, c->stream) != hipSuccess))
         return fail(VTI_ERR_HIP, "vti_rtm_reset: zero");
     return VTI_OK;
 }
@@ -252,8 +253,12 @@ int vti_rtm_shot(vti_ctx *c, const vti_rtm_opts *o, const float *obs, long long 
         StressArgs ra = rcv_stress_args(c, rc.g, it, svb, inj_z);
         if (o->illum_mode == 1) ra = with_illum(c, ra, 0);
         if (launch_stress(nd, ra, c->stream) != hipSuccess) return fail(VTI_ERR_HIP, "stress");
-        if (it % o->image_every == 0)
-            return hip(launch_image(c, W.f[VTI_F_P], rc.g.f[VTI_F_P], c->img), "image");
+        if (it % o->image_every == 0) {
+            const int r = hip(launch_image(c, W.f[VTI_F_P], rc.g.f[VTI_F_P], c->img), "image");
+            if (r) return r;
+            // Poynting_Adcigs (:1475-1477) on the same source and receiver states
+            return hip(launch_adcig(c, W, rc.g), "adcig");
+        }
         return VTI_OK;
     };
     c->rtm_illum_mode = o->illum_mode;

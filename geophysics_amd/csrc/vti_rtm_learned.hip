@@ -226,6 +226,9 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         lo->codec < 0 || lo->codec > 1 || lo->budget < 0 ||
         (lo->codec == 1 && (!(lo->tol >= 0.0f) || !(lo->floor >= 0.0f))))
         return fail(VTI_ERR_ARG, "vti_rtm_shot_learned: bad options");
+    if (c->adcig_na > 0)
+        return fail(VTI_ERR_STATE, "vti_rtm_shot_learned: angle gathers need the full source state "
+                                   "(vti_rtm_shot); switch them off with vti_adcig_config(ctx, 0, 0, 0)");
     const bool sparse = lo->codec == 1;
     int st;
     if (!c->img && (st = vti_rtm_reset(c))) return st;
@@ -236,24 +239,34 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
     const int nsave = (nt + se - 1) / se;
     const long long off = np + (long long)np * c->g.ldz + (long long)np * c->g.ldxy;
     hipStream_t s = c->stream;
-    Buf lat, scl, dense, dense2, acc, exa, boxes, ents, boffs, bcnts;
+    Buf lat, scl, dense, dense2, acc, exa, boxes, boffs, bcnts;
     if ((!sparse && (hipMalloc(&lat.p, (size_t)nsave * lat_n * sizeof(uint16_t)) != hipSuccess ||
                      hipMalloc(&scl.p, (size_t)nsave * 64) != hipSuccess)) ||
         hipMalloc(&dense.p, (size_t)n * n * n * sizeof(float)) != hipSuccess ||
         hipMalloc(&acc.p, 2 * sizeof(double)) != hipSuccess)
         return fail(VTI_ERR_ALLOC, "vti_rtm_shot_learned: latent store");
-    // sparse codec store: entries (4 bytes each) within the budget, block tables per snapshot
+    // sparse codec store: entries (4 bytes each) in chunks allocated as the
+    // snapshots need them, at most `budget` bytes in all; block tables per
+    // snapshot.  A chunk is taken when the current one has room for less than
+    // one snapshot's largest possible entry count (n^3), so an encode never
+    // runs out of room inside a chunk and the store holds what the entries
+    // use plus less than one chunk.
     const long long nblk = (long long)(n / 8) * (n / 8) * (n / 8);
     const long long budget = lo->budget > 0 ? lo->budget : (long long)nsave * n * n * n;
-    const long long cap = sparse ? budget / 4 : 0;
-    if (sparse && (hipMalloc(&ents.p, (size_t)std::max(cap, 1LL) * 4) != hipSuccess ||
-                   hipMalloc(&boffs.p, (size_t)nsave * nblk * 4) != hipSuccess ||
+    const long long snap_max = (long long)n * n * n;                    // entries of one snapshot at most
+    const long long chunk_ents = std::max(snap_max, (4LL << 30) / 4);   // 4 GB chunks
+    if (sparse && (hipMalloc(&boffs.p, (size_t)nsave * nblk * 4) != hipSuccess ||
                    hipMalloc(&bcnts.p, (size_t)nsave * nblk * 2) != hipSuccess))
         return fail(VTI_ERR_ALLOC, "vti_rtm_shot_learned: sparse store");
-    uint32_t *ENT = static_cast<uint32_t *>(ents.p);
+    struct Chunks {
+        std::vector<void *> p;
+        ~Chunks() { for (void *q : p) (void)hipFree(q); }
+    } chunks;
+    long long chunk_room = 0, alloc_ents = 0;   // room left in the current chunk, entries allocated
+    uint32_t *cur = nullptr;                    // next free entry of the current chunk
     uint32_t *BOFF = static_cast<uint32_t *>(boffs.p);
     uint16_t *BCNT = static_cast<uint16_t *>(bcnts.p);
-    std::vector<long long> ebase(nsave, 0);
+    std::vector<uint32_t *> ebase(nsave, nullptr);
     long long used = 0;
     uint16_t *L = static_cast<uint16_t *>(lat.p);
     float *Sc = static_cast<float *>(scl.p);
@@ -328,18 +341,34 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
             if (box_n)
                 hipLaunchKernelGGL(box_save_kernel, dim3(64), dim3(256), 0, s, c->g, np, P, bx, BX + (size_t)j * box_n);
             const bc::Mask m{bx.y0, bx.x0, bx.z0, bx.by, bx.bx, bx.bz};
+            if (chunk_room < snap_max) {   // a new chunk: 4 GB, or what the budget has left
+                const long long take = std::min(chunk_ents, budget / 4 - alloc_ents);
+                void *q = nullptr;
+                if (take > chunk_room) {
+                    if (hipMalloc(&q, (size_t)take * 4) == hipSuccess) {
+                        chunks.p.push_back(q);
+                        cur = static_cast<uint32_t *>(q);
+                        chunk_room = take;
+                        alloc_ents += take;
+                    } else {
+                        (void)hipGetLastError();   // no room on the device: the snapshots past it fall back
+                    }
+                }
+            }
             long long need = 0;
             double e = 0.0;
-            if ((st = ae_bc_encode(A, P, off, c->g.ldxy, c->g.ldz, n, m, lo->tol, lo->floor, ENT + used, cap - used,
+            if ((st = ae_bc_encode(A, P, off, c->g.ldxy, c->g.ldz, n, m, lo->tol, lo->floor, cur, chunk_room,
                                    BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, &need, &e, nullptr, s)))
                 return st;
             if ((st = hip(t_enc.end(s), "event"))) return st;
-            if (need > cap - used) {   // over budget: recompute this snapshot from a checkpoint
+            if (need > chunk_room) {   // over budget: recompute this snapshot from a checkpoint
                 if (!nck) return fail(VTI_ERR_ALLOC, "vti_rtm_shot_learned: sparse store budget exhausted (set ck_every)");
                 bad[j] = 1;
                 ++nbad;
             } else {
-                ebase[j] = used;
+                ebase[j] = cur;
+                cur += need;
+                chunk_room -= need;
                 used += need;
                 err_sum += e;
                 ++nchecked;
@@ -418,7 +447,7 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
             if ((st = hip(launch_image(c, W.f[VTI_F_P], G.f[VTI_F_P], c->img), "image"))) return st;
         } else {
             if ((st = hip(t_dec.begin(s), "event"))) return st;
-            if (sparse) st = ae_bc_decode(A, ENT + ebase[j], BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, n, D, s);
+            if (sparse) st = ae_bc_decode(A, ebase[j], BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, n, D, s);
             else st = ae_decode(A, L + (size_t)j * lat_n, Sc + (size_t)j * 16, n, D, s);
             if (st) return st;
             if (box_n)
@@ -442,6 +471,15 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         stats[6] = sparse ? (double)used * 4 + (double)nsave * nblk * 6 + (double)nsave * box_n * 4
                           : (double)nsave * lat_n * 2 + (double)nexact * nn3 * 4 + (double)nsave * box_n * 4;
         stats[7] = nrec_snap;
+        // device memory held for the stored source field, as allocated: the
+        // store (entry chunks and block tables, or latents, scales and exact
+        // slots), the source boxes, the two dense n^3 work buffers and the
+        // fallback checkpoints
+        const double ckb = (double)nck * (double)vti_state_bytes(c);
+        stats[8] = sparse ? (double)alloc_ents * 4 + (double)nsave * nblk * 6 + (double)nsave * box_n * 4 +
+                                2.0 * nn3 * 4 + ckb
+                          : (double)nsave * lat_n * 2 + (double)nsave * 64 + (double)lo->max_exact * nn3 * 4 +
+                                (double)nsave * box_n * 4 + 2.0 * nn3 * 4 + ckb;
     }
     return VTI_OK;
 }

@@ -149,6 +149,14 @@ def _declare(L):
         "vti_bc_encode": (C.c_int, [VP, VP, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_float,
                                     C.c_float, VP, C.c_longlong, VP, VP, C.POINTER(C.c_longlong), DP]),
         "vti_bc_decode": (C.c_int, [VP, VP, VP, VP, C.c_int, VP]),
+        "vti_adcig_config": (C.c_int, [VP, C.c_int, C.c_int, C.c_int]),
+        "vti_adcig_elems": (C.c_longlong, [VP]),
+        "vti_adcig_tables": (C.c_int, [C.c_int, C.c_int, FP, FP]),
+        "vti_adcig_reset": (C.c_int, [VP]),
+        "vti_adcig_accumulate": (C.c_int, [VP, VP]),
+        "vti_adcig_get": (C.c_int, [VP, C.c_int, FP]),
+        "vti_adcig_stack": (C.c_int, [C.c_int, C.c_int, C.c_int, FP, FP]),
+        "vti_adcig_thin": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, FP, FP]),
         "vti_sym_eigen": (C.c_int, [C.c_int, DP, DP, DP]),
         "vti_ae_fit_shot": (C.c_int, [VP, VP, C.c_int, C.c_int, C.c_int]),
         "vti_learned_defaults": (C.c_int, [C.POINTER(LearnedOpts)]),
@@ -402,7 +410,7 @@ class Propagator:
         lopts.codec = 1); returns the stats dict."""
         o = opts if opts is not None else self.rtm_defaults()
         lo = lopts if lopts is not None else learned_defaults()
-        st = np.zeros(8, np.float64)
+        st = np.zeros(9, np.float64)
         p = None
         if obs is not None:
             a, p = _f32(obs)
@@ -412,7 +420,8 @@ class Propagator:
         check(lib().vti_rtm_shot_learned(self.h, ae.h, C.byref(o), C.byref(lo), p, st.ctypes.data_as(DP)),
               "rtm_shot_learned")
         return {"encode_ms": st[0], "decode_ms": st[1], "mean_checked_err": st[2], "over_tol": int(st[3]),
-                "snapshots": int(st[4]), "exact": int(st[5]), "store_bytes": st[6], "recomputed": int(st[7])}
+                "snapshots": int(st[4]), "exact": int(st[5]), "store_bytes": st[6], "recomputed": int(st[7]),
+                "alloc_bytes": st[8]}
 
     def ae_fit_shot(self, ae, save_every=4, nblocks=4096, src_box=8):
         check(lib().vti_ae_fit_shot(self.h, ae.h, save_every, nblocks, src_box), "ae_fit_shot")
@@ -438,6 +447,27 @@ class Propagator:
         """img += P(src) * P(self), illum += P(src)^2 + Q(src)^2 on device
         arrays of the interior (waits)."""
         check(lib().vti_image_accumulate(src.h, self.h, img_ptr, illum_ptr, 1 if clamp else 0), "image_accumulate")
+        self.sync()
+
+    # ---- Poynting-vector angle gathers (vti_adcig_*) ----
+    def adcig_config(self, nangle, dangle=1, dcdp=1):
+        check(lib().vti_adcig_config(self.h, nangle, dangle, dcdp), "adcig_config")
+        self._adcig = (nangle, dangle, dcdp)
+
+    def adcig_shape(self):
+        na, _, d = self._adcig
+        if self.cfg.ndim == 3:
+            return (self.cfg.ny // d, self.cfg.nx // d, na, self.cfg.nz)
+        return (self.cfg.nx, na, self.cfg.nz)
+
+    def adcig(self, illum_divide=False):
+        out = np.zeros(self.adcig_shape(), np.float32)
+        check(lib().vti_adcig_get(self.h, 1 if illum_divide else 0, out.ctypes.data_as(FP)), "adcig_get")
+        return out
+
+    def adcig_accumulate(self, src):
+        """One gather accumulation: source fields of src, receiver fields of self (waits)."""
+        check(lib().vti_adcig_accumulate(src.h, self.h), "adcig_accumulate")
         self.sync()
 
     def set_unsplit(self, on=True):
@@ -502,6 +532,32 @@ def GPU_vti3dfd(nx, ny, nz, dx, dy, dz, npml, SV, FNv, FNe, FNd, favg, ns, fsx, 
     lib().GPU_vti3dfd(nx, ny, nz, dx, dy, dz, npml, SV, _b(FNv), _b(FNe), _b(FNd), favg, ns,
                       fsx, dsx, fsy, dsy, fsz, dsz, _b(FNshot), _b(FNsnap), nt, dt, run_count)
     check(lib().vti_dropin_status(), "GPU_vti3dfd")
+
+
+def adcig_tables(nangle, dangle):
+    """Bin thresholds and cos^3 weight factors of the angle gathers (host)."""
+    t = np.zeros(nangle, np.float32)
+    w = np.zeros(nangle, np.float32)
+    check(lib().vti_adcig_tables(nangle, dangle, t.ctypes.data_as(FP), w.ctypes.data_as(FP)), "adcig_tables")
+    return t, w
+
+
+def adcig_stack(adcigs):
+    """stk_adcigs: (nx, nangle, nz) gathers -> Laplacian-filtered stack (nx, nz)."""
+    a, p = _f32(adcigs)
+    nx, na, nz = a.shape
+    out = np.zeros((nx, nz), np.float32)
+    check(lib().vti_adcig_stack(nx, nz, na, p, out.ctypes.data_as(FP)), "adcig_stack")
+    return out
+
+
+def adcig_thin(adcigs, d):
+    """adcigs_smiled: the gathers of every d-th cdp."""
+    a, p = _f32(adcigs)
+    nx, na, nz = a.shape
+    out = np.zeros(((nx + d - 1) // d, na, nz), np.float32)
+    check(lib().vti_adcig_thin(nx, nz, na, d, p, out.ctypes.data_as(FP)), "adcig_thin")
+    return out
 
 
 def omp2dfd(FNv, FNe, FNd, FNshot, FNsnap, nx=601, nz=301, dx=5.0, dz=5.0, npd=0, favg=30.0,

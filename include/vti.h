@@ -216,6 +216,45 @@ int vti_rtm_image(vti_ctx *ctx, int finalize, float *img, float *illum);
  * (Toa_gpu_3dvti_rtm_adcigs_1orderfunction.cu:353-381). */
 int vti_mute3d_window(vti_ctx *ctx, int fsx, int dsx, int fsy, int dsy,
                       int fsz, int is, int tt);
+/* Poynting-vector angle-domain common-image gathers (ADCIGs) of the RTM:
+ * Poynting_Adcigs Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:718-766 (2D, every
+ * cdp) and Toa_gpu_3dvti_rtm_adcigs_1orderfunction.cu:618-660 (3D, every
+ * dcdp-th cdp in x and y).  With nangle > 0, every imaging step of
+ * vti_rtm_shot (and vti_adcig_accumulate) adds, per cdp cell, P_s P_g
+ * cos^3(ia) to angle bin ia = (int)(half the angle between the source
+ * Poynting vector (P u, [P v,] Q w) and the receiver one (P u, [P v,] Q w),
+ * in degrees / dangle) of that cell's gather (the reference negates the
+ * source vector of its backward-rebuilt source field, :746-747; the source
+ * field here is the exact forward one), the cosine scaled by
+ * 1/(1 - 0.1) as there (cells whose scaled cosine leaves [-1, 1] add
+ * nothing).  Gathers [iya][ixa][ia][iz] (2D: [ix][ia][iz]), nx/dcdp x
+ * ny/dcdp cdps in 3D; vti_rtm_reset zeroes them.  nangle = 0 switches
+ * them off.  The learned RTM (vti_rtm_shot_learned) keeps only P of the
+ * source field and refuses to run while gathers are configured. */
+int vti_adcig_config(vti_ctx *ctx, int nangle, int dangle, int dcdp);
+/* Number of gather samples (0 when off). */
+long long vti_adcig_elems(const vti_ctx *ctx);
+/* Host-only: the bin thresholds the kernel compares the scaled cosine a
+ * with (thresholds[k-1]: the largest float a whose bin, by the reference's
+ * arithmetic with the host's acosf, is at least k; -2 when no a reaches k)
+ * and the cos^3 weight factors cosf(ia pi / 180), nangle each. */
+int vti_adcig_tables(int nangle, int dangle, float *thresholds, float *weights);
+int vti_adcig_reset(vti_ctx *ctx);
+/* One accumulation from the current fields of a source-side and a
+ * receiver-side context of the same grid into the receiver context's
+ * gathers (ordered like vti_image_accumulate). */
+int vti_adcig_accumulate(vti_ctx *src, vti_ctx *rcv);
+/* The gathers into host memory; illum_divide: divided by the RTM's
+ * illumination of each gather's cdp (adcigs_illum :768-786, 3D where
+ * nonzero :663-684). */
+int vti_adcig_get(vti_ctx *ctx, int illum_divide, float *adcigs);
+/* Host post-processing (2D layout [ix][ia][iz]): the stack over angles then
+ * the adjoint Laplacian (stk_adcigs :788-816), and the gathers of every
+ * dAdcigs-th cdp, packed ((nx + dAdcigs - 1) / dAdcigs of them,
+ * adcigs_smiled :818-847). */
+int vti_adcig_stack(int nx, int nz, int nangle, const float *adcigs, float *migration);
+int vti_adcig_thin(int nx, int nz, int nangle, int dAdcigs, const float *adcigs, float *out);
+
 /* The checkpoint schedule of vti_rtm_shot / vti_batch_rtm / vti_grad_run
  * run on positions instead of wavefields (host only, no device needed):
  * VTI_OK when the reverse sweep receives X_nt .. X_1 in order with at most
@@ -378,14 +417,18 @@ typedef struct {
                           checkpoints (or the call fails without them) */
 } vti_learned_opts;
 int vti_learned_defaults(vti_learned_opts *o);
-/* stats (optional, 8 doubles): encode ms, decode ms (GPU time between
+/* stats (optional, 9 doubles): encode ms, decode ms (GPU time between
  * HIP events around the codec's launches; the encode's up to the host
  * synchronisation it needs to place the entries), mean
  * checked relative error (codec 1: mean error bound of the stored
  * snapshots), snapshots above tol (codec 1: over budget), snapshots,
  * snapshots kept exactly, bytes of the compressed store (latents or
  * entries + block tables, exactly kept snapshots, source boxes), snapshots
- * recomputed from checkpoints. */
+ * recomputed from checkpoints, and the device bytes the call held for the
+ * stored source field as allocated (the store, its sparse entries in 4 GB
+ * chunks taken as needed up to the budget, source boxes, two dense n^3 work
+ * buffers, fallback checkpoints): the memory a plain checkpointed RTM would
+ * be given for a comparison at equal memory. */
 int vti_rtm_shot_learned(vti_ctx *ctx, vti_ae *ae, const vti_rtm_opts *opts,
                          const vti_learned_opts *lo, const float *obs, double *stats);
 /* Forward modelling of the current shot; every save_every-th snapshot, the

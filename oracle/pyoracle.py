@@ -62,6 +62,8 @@ def lib():
             getattr(L, "orc_get" + sfx).argtypes = [C.c_void_p, C.c_int, rp]
             getattr(L, "orc_set" + sfx).argtypes = [C.c_void_p, C.c_int, rp]
             getattr(L, "orc_destroy" + sfx).argtypes = [C.c_void_p]
+        L.orc_set_threads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
         L.orc_ncell.restype = C.c_long
         L.orc_ncell.argtypes = [C.c_void_p]
         L.orc_params_gpu2d.argtypes = [pp] + [C.c_int] * 14
@@ -77,10 +79,25 @@ def lib():
             getattr(L, "orc_rtm" + sfx).argtypes = [C.c_void_p, C.c_void_p] + [C.c_int] * 5 + [rp] * 4
         L.orc_laplace_adj.argtypes = [C.c_int] * 3 + [fp, fp]
         for sfx, rp in (("_f32", fp), ("_f64", dp)):
+            getattr(L, "orc_rtm_ex" + sfx).argtypes = [C.c_void_p, C.c_void_p] + [C.c_int] * 5 + [rp] * 4 + \
+                [C.c_int] * 3 + [rp]
+            getattr(L, "orc_adcig" + sfx).argtypes = [C.c_void_p, C.c_void_p] + [C.c_int] * 3 + [rp]
+        L.orc_adcig_stack.argtypes = [C.c_int] * 3 + [fp, fp]
+        L.orc_adcig_thin.argtypes = [C.c_int] * 4 + [fp, fp]
+        for sfx, rp in (("_f32", fp), ("_f64", dp)):
             getattr(L, "orc_fwi_grad" + sfx).argtypes = [C.c_void_p] + [rp] * 6
             getattr(L, "orc_set_source" + sfx).argtypes = [C.c_void_p, rp]
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's loops; returns the number in effect."""
+    return int(lib().orc_set_threads(int(n)))
+
+
+def library_path():
+    return _LIB
 
 
 def _fptr(a):
@@ -227,8 +244,42 @@ def laplace_adj(img):
     return out
 
 
+def adcig_shape(params, nangle, dcdp=1):
+    """(nya, nxa, nangle, nz) of the gathers: every cdp in 2D, every dcdp-th in 3D."""
+    if params.ndim == 3:
+        return (params.ny // dcdp, params.nx // dcdp, nangle, params.nz)
+    return (1, params.nx, nangle, params.nz)
+
+
+def adcig_accumulate(src, rcv, adcigs, nangle, dangle, dcdp=1):
+    """One Poynting_Adcigs accumulation from the current fields of two Sims
+    (adcigs: array of adcig_shape, updated in place)."""
+    assert adcigs.dtype == src.dtype and adcigs.flags.c_contiguous
+    getattr(lib(), "orc_adcig_" + src.prec)(src.h, rcv.h, nangle, dangle, dcdp, src._ptr(adcigs))
+    return adcigs
+
+
+def adcig_stack(adcigs):
+    """stk_adcigs: (nx, nangle, nz) gathers -> Laplacian-filtered stack (nx, nz)."""
+    a = np.ascontiguousarray(adcigs, dtype=np.float32)
+    nx, na, nz = a.shape
+    out = np.zeros((nx, nz), np.float32)
+    lib().orc_adcig_stack(nx, nz, na, _fptr(a), _fptr(out))
+    return out
+
+
+def adcig_thin(adcigs, d):
+    """adcigs_smiled: the gathers of every d-th cdp."""
+    a = np.ascontiguousarray(adcigs, dtype=np.float32)
+    nx, na, nz = a.shape
+    out = np.zeros(((nx + d - 1) // d, na, nz), np.float32)
+    lib().orc_adcig_thin(nx, nz, na, d, _fptr(a), _fptr(out))
+    return out
+
+
 def rtm(params, vp, eps, dl, sv_fwd=1, sv_bwd=0, inject_z=None, illum_mode=0, image_every=1,
-        obs=None, mute=None, prec="f32", unsplit_source=False, unsplit_receiver=False):
+        obs=None, mute=None, prec="f32", unsplit_source=False, unsplit_receiver=False,
+        nangle=0, dangle=1, dcdp=1):
     """One-shot RTM with every forward snapshot kept (no checkpointing).
 
     mute: callable(modelled_record) applied in place before it is used as
@@ -257,11 +308,15 @@ def rtm(params, vp, eps, dl, sv_fwd=1, sv_bwd=0, inject_z=None, illum_mode=0, im
         mute(rec)
         obs = rec
     o = None if obs is None else np.ascontiguousarray(obs, dtype=dt)
-    fn = getattr(lib(), "orc_rtm_" + prec)
+    fn = getattr(lib(), "orc_rtm_ex_" + prec)
+    ad = np.zeros(adcig_shape(params, nangle, dcdp), dtype=dt) if nangle > 0 else None
     fn(s.h, g.h, sv_fwd, sv_bwd, iz, illum_mode, image_every,
-       s._ptr(o) if o is not None else None, s._ptr(modelled), s._ptr(img), s._ptr(illum))
+       s._ptr(o) if o is not None else None, s._ptr(modelled), s._ptr(img), s._ptr(illum),
+       nangle, dangle, dcdp, s._ptr(ad) if ad is not None else None)
     s.close()
     g.close()
+    if nangle > 0:
+        return img, illum, modelled, ad
     return img, illum, modelled
 
 

@@ -6,6 +6,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "vti_oracle.h"
 
 /* #define pi 3.141592653  (GPU_vti2dfd_kernels.cu:31, CPU_vti2dfd_kernels.c:27) */
@@ -39,6 +42,19 @@
 long orc_ncell(void *h)
 {
     return h ? ((orc_sim_f32 *)h)->n : 0;
+}
+
+/* OpenMP threads of the loops above (timing of the CPU baseline: 1 = the
+ * reference's serial CPU path); returns the number in effect. */
+int orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
 }
 
 static void base2d(orc_params *p, int nx, int nz, int dx, int dz, int npd,
@@ -232,4 +248,28 @@ void orc_laplace_adj(int nz, int nx, int ny, const float *in, float *out)
                     if (iy < ny - 1) { out[j + sy] -= in[j]; out[j] += in[j]; }
                 }
             }
+}
+
+/* stk_adcigs Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:788-816: float sum over the
+ * angles in order, then the adjoint Laplacian. */
+void orc_adcig_stack(int nx, int nz, int nangle, const float *adcigs, float *migration)
+{
+    float *temp = (float *)malloc((size_t)nz * nx * sizeof(float));
+    for (int ix = 0; ix < nx; ix++)
+        for (int iz = 0; iz < nz; iz++) {
+            float stk = 0.0f;
+            for (int ia = 0; ia < nangle; ia++) stk += adcigs[(long)ix * nangle * nz + (long)ia * nz + iz];
+            temp[(long)ix * nz + iz] = stk;
+        }
+    orc_laplace_adj(nz, nx, 1, temp, migration);
+    free(temp);
+}
+
+/* adcigs_smiled :818-847: the gathers of every dAdcigs-th cdp, packed. */
+void orc_adcig_thin(int nx, int nz, int nangle, int dAdcigs, const float *adcigs, float *out)
+{
+    for (int ix = 0; ix < nx; ix += dAdcigs)
+        for (int ia = 0; ia < nangle; ia++)
+            for (int iz = 0; iz < nz; iz++)
+                out[(long)(ix / dAdcigs) * nangle * nz + (long)ia * nz + iz] = adcigs[(long)ix * nangle * nz + (long)ia * nz + iz];
 }

@@ -108,12 +108,33 @@ int orc_set_f64(void *h, int which, const double *in);
 void orc_destroy_f32(void *h);
 void orc_destroy_f64(void *h);
 long orc_ncell(void *h32);
+/* OpenMP threads of the stencil loops (n > 0 sets them); returns the number
+ * in effect (1 without OpenMP). */
+int orc_set_threads(int n);
 
 /* RTM of one shot (see vti_oracle_impl.h). */
 int orc_rtm_f32(void *hs, void *hr, int sv_fwd, int sv_bwd, int inject_z, int illum_mode,
                 int image_every, const float *obs, float *modelled, float *img, float *illum);
 int orc_rtm_f64(void *hs, void *hr, int sv_fwd, int sv_bwd, int inject_z, int illum_mode,
                 int image_every, const double *obs, double *modelled, double *img, double *illum);
+/* The same with Poynting-vector angle gathers (nangle > 0: adcigs
+ * [iy][ix][ia][iz] over the cdps, every dcdp-th in 3D), and one gather
+ * accumulation from the current fields of two simulations. */
+int orc_rtm_ex_f32(void *hs, void *hr, int sv_fwd, int sv_bwd, int inject_z, int illum_mode,
+                   int image_every, const float *obs, float *modelled, float *img, float *illum,
+                   int nangle, int dangle, int dcdp, float *adcigs);
+int orc_rtm_ex_f64(void *hs, void *hr, int sv_fwd, int sv_bwd, int inject_z, int illum_mode,
+                   int image_every, const double *obs, double *modelled, double *img, double *illum,
+                   int nangle, int dangle, int dcdp, double *adcigs);
+int orc_adcig_f32(void *hs, void *hr, int nangle, int dangle, int dcdp, float *adcigs);
+int orc_adcig_f64(void *hs, void *hr, int nangle, int dangle, int dcdp, double *adcigs);
+/* Post-processing of the gathers (host functions of the reference):
+ * stk_adcigs Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:788-816 (sum over angles,
+ * then laplace_filter adj=1) and adcigs_smiled :818-847 (every dAdcigs-th
+ * gather); 2D layout [ix][ia][iz]. */
+void orc_adcig_stack(int nx, int nz, int nangle, const float *adcigs, float *migration);
+void orc_adcig_thin(int nx, int nz, int nangle, int dAdcigs, const float *adcigs, float *out);
+
 /* Misfit gradient with respect to vp (see vti_oracle_impl.h). */
 int orc_fwi_grad_f32(void *h, const float *dobs, const float *resid, float *misfit,
                      float *grad_int, float *grad_src, float *d_out);

@@ -551,6 +551,71 @@ int FN(orc_set)(void *h, int which, const REAL *in)
     return 0;
 }
 
+/* Poynting-vector angle gathers, one accumulation: Poynting_Adcigs
+ * Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:718-766 (2D, every cdp) and
+ * Toa_gpu_3dvti_rtm_adcigs_1orderfunction.cu:618-660 (3D, every dcdp-th cdp
+ * in x and y).  Source Poynting vector (P u, [P v,] Q w), receiver
+ * (P u, [P v,] Q w).  The reference negates the source vector (:746-747)
+ * because its source field is rebuilt backward in time, where u changes
+ * sign; the source field here is the exact forward field, whose own P u
+ * already points along the incident wave (with the negation the bins would
+ * hold 90 degrees minus the reflection angle).
+ * a = S_s.S_g / (|S_s||S_g| (1 - 0.1)); cells with
+ * |a| <= 1 add P_s P_g cos^3(ia degrees) to angle bin ia = (int)(0.5 acos(a)
+ * in degrees / dangle) of their own gather when ia < nangle.  Fields are
+ * padded volumes; adcigs [iy][ix][ia][iz] over the cdps.  Float arithmetic
+ * as the reference's: the quotient in double (the 1 - 0.1 literal), acosf,
+ * the angle in double rounded to float, cosf of a double argument. */
+static void FN(adcig_core)(const SIM *s, int nangle, int dangle, int dcdp,
+                           const REAL *sP, const REAL *sQ, const REAL *su, const REAL *sv, const REAL *sw,
+                           const REAL *gP, const REAL *gQ, const REAL *gu, const REAL *gv, const REAL *gw,
+                           REAL *adcigs)
+{
+    const orc_params *p = &s->p;
+    const int three = p->ndim == 3, np = p->npml, nz = p->nz;
+    const int d = three ? dcdp : 1;
+    const int nxa = p->nx / d, nya = three ? p->ny / d : 1;
+    for (int iya = 0; iya < nya; iya++)
+        for (int ixa = 0; ixa < nxa; ixa++)
+            for (int iz = 0; iz < nz; iz++) {
+                long c = (iz + np) + (long)(ixa * d + np) * s->nnz + (three ? (long)(iya * d + np) * s->nnz * s->nnx : 0);
+                REAL Ssx = sP[c] * su[c], Ssz = sQ[c] * sw[c];
+                REAL Sgx = gP[c] * gu[c], Sgz = gQ[c] * gw[c];
+                REAL b1, b2, num;
+                if (three) {
+                    REAL Ssy = sP[c] * sv[c], Sgy = gP[c] * gv[c];
+                    b1 = Ssx * Ssx + Ssy * Ssy + Ssz * Ssz;
+                    b2 = Sgx * Sgx + Sgy * Sgy + Sgz * Sgz;
+                    num = Ssx * Sgx + Ssy * Sgy + Ssz * Sgz;
+                } else {
+                    b1 = Ssx * Ssx + Ssz * Ssz;
+                    b2 = Sgx * Sgx + Sgz * Sgz;
+                    num = Ssx * Sgx + Ssz * Sgz;
+                }
+                REAL a = (REAL)(num / (SQRT_R(b1 * b2) * (1 - 0.1)));
+                if (!(a >= -1 && a <= 1)) continue;
+                REAL ang = (REAL)(0.5 * (sizeof(REAL) == sizeof(float) ? (double)acosf((float)a) : acos((double)a)) *
+                                  180.0 / ORC_PI);
+                int ia = (int)(ang / (dangle * 1.0));
+                if (ia >= nangle) continue;
+                REAL cw = sizeof(REAL) == sizeof(float) ? (REAL)cosf((float)(ia * ORC_PI / 180.0))
+                                                        : (REAL)cos(ia * ORC_PI / 180.0);
+                long id = iz + (long)nz * ia + (long)nz * nangle * ixa + (long)nz * nangle * nxa * iya;
+                adcigs[id] += sP[c] * gP[c] * cw * cw * cw;
+            }
+}
+
+/* One accumulation from the current fields of a source and a receiver
+ * simulation of the same grid. */
+int FN(orc_adcig)(void *hs, void *hr, int nangle, int dangle, int dcdp, REAL *adcigs)
+{
+    SIM *s = (SIM *)hs, *g = (SIM *)hr;
+    if (!s || !g || nangle < 1 || dangle < 1 || dcdp < 1) return -1;
+    FN(adcig_core)(s, nangle, dangle, dcdp, s->f[9], s->f[10], s->f[0], s->f[1], s->f[2],
+                   g->f[9], g->f[10], g->f[0], g->f[1], g->f[2], adcigs);
+    return 0;
+}
+
 /* RTM of one shot, restating RTM() Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:1378-1484
  * with the exact forward field of every step (all P snapshots kept in
  * memory).  hs: the shot's simulation (source side), fresh; hr: a second
@@ -561,6 +626,16 @@ int FN(orc_set)(void *h, int which, const REAL *in)
 int FN(orc_rtm)(void *hs, void *hr, int sv_fwd, int sv_bwd, int inject_z, int illum_mode,
                 int image_every, const REAL *obs, REAL *modelled, REAL *img, REAL *illum)
 {
+    return FN(orc_rtm_ex)(hs, hr, sv_fwd, sv_bwd, inject_z, illum_mode, image_every, obs, modelled, img, illum,
+                          0, 1, 1, NULL);
+}
+
+/* orc_rtm with the angle gathers of every imaged step (nangle > 0): the
+ * source fields P, Q, u, v, w of every step are kept (padded). */
+int FN(orc_rtm_ex)(void *hs, void *hr, int sv_fwd, int sv_bwd, int inject_z, int illum_mode,
+                   int image_every, const REAL *obs, REAL *modelled, REAL *img, REAL *illum,
+                   int nangle, int dangle, int dcdp, REAL *adcigs)
+{
     SIM *s = (SIM *)hs, *g = (SIM *)hr;
     if (!s || !g || image_every < 1) return -1;
     const orc_params *p = &s->p;
@@ -568,9 +643,11 @@ int FN(orc_rtm)(void *hs, void *hr, int sv_fwd, int sv_bwd, int inject_z, int il
     const int ny = p->ndim == 3 ? p->ny : 1;
     const long nint = (long)p->nx * ny * p->nz;
     const long nrec = (long)p->rec_nx * (p->ndim == 3 ? p->rec_ny : 1);
+    const int gat = nangle > 0 && adcigs != NULL;
     REAL *snap = (REAL *)malloc((size_t)nt * nint * sizeof(REAL));
     REAL *row = (REAL *)malloc((nrec > 0 ? nrec : 1) * sizeof(REAL));
-    if (!snap || !row) { free(snap); free(row); return -2; }
+    REAL *full = gat ? (REAL *)malloc((size_t)nt * 5 * s->n * sizeof(REAL)) : NULL;
+    if (!snap || !row || (gat && !full)) { free(snap); free(row); free(full); return -2; }
     const int yo = p->ndim == 3 ? np : 0;
     /* the records replace P, Q of row inject_z before each receiver step: in
      * the unsplit mode that row keeps the split form, which rebuilds P, Q
@@ -579,6 +656,11 @@ int FN(orc_rtm)(void *hs, void *hr, int sv_fwd, int sv_bwd, int inject_z, int il
     for (int it = 0; it < nt; it++) {
         FN(step_ex)(s, it, 1, sv_fwd, row);
         for (long r = 0; r < nrec; r++) modelled[it + (long)nt * r] = row[r];
+        if (gat) {
+            const int sel[5] = {9, 10, 0, 1, 2};   /* P Q u v w */
+            for (int k = 0; k < 5; k++)
+                memcpy(full + ((size_t)it * 5 + k) * s->n, s->f[sel[k]], s->n * sizeof(REAL));
+        }
         REAL *P = s->f[9], *Q = s->f[10];
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < p->nx; ix++)
@@ -613,9 +695,15 @@ int FN(orc_rtm)(void *hs, void *hr, int sv_fwd, int sv_bwd, int inject_z, int il
                     if (illum_mode == 1) illum[o] = illum[o] + (gP[c] * gP[c] + gQ[c] * gQ[c]);
                     if (it % image_every == 0) img[o] += snap[(size_t)it * nint + o] * gP[c];
                 }
+        if (gat && it % image_every == 0) {
+            const REAL *f = full + (size_t)it * 5 * s->n;
+            FN(adcig_core)(s, nangle, dangle, dcdp, f, f + s->n, f + 2 * s->n, f + 3 * s->n, f + 4 * s->n,
+                           g->f[9], g->f[10], g->f[0], g->f[1], g->f[2], adcigs);
+        }
     }
     free(snap);
     free(row);
+    free(full);
     return 0;
 }
 
