@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <random>
 #include <string>
@@ -668,6 +669,73 @@ int vti_bc_encode(vti_ae *A, const float *P, long long off, long long sy, long l
     const bc::Mask m{0, 0, 0, 0, 0, 0};
     const int st = ae_bc_encode(A, P, off, sy, sx, n, m, tol, floor, ent, cap, boff, bcnt, kept, err, nullptr, A->stream);
     if (st) return st;
+    AE_HIP(hipStreamSynchronize(A->stream));
+    return VTI_OK;
+}
+
+}  // extern "C"
+
+// ---- the codec's encode without allocation -------------------------------
+// Scratch of vti_bc_encode_noalloc: the counters (64 bytes, first) and the
+// dense coefficients of at most kNoAllocBlocks 8^3 blocks (boxes up to
+// kNoAllocSide^3), 8 MiB + 256 bytes in all, held by every GPU the library
+// runs on until the process ends.  In host code `scratch` names a host-side
+// stand-in; the device address comes from hipGetSymbolAddress, per device.
+constexpr int kNoAllocSide = 128;
+constexpr long long kNoAllocBlocks = (long long)(kNoAllocSide / 8) * (kNoAllocSide / 8) * (kNoAllocSide / 8);
+constexpr long long kScratchHead = 64;   // floats before the coefficients (256 bytes: counters, padding)
+static_assert(sizeof(bc::Counters) <= kScratchHead * sizeof(float), "counters fit the scratch head");
+static __device__ __align__(256) float scratch[kScratchHead + kNoAllocBlocks * 512];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_dev[kMaxDevices] = {};
+
+// The device address of this GPU's scratch (looked up once per device, with
+// that device current).
+int scratch_address(float **out)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+        return fail(VTI_ERR_HIP, "vti_bc_encode_noalloc: current device");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(VTI_ERR_HIP, std::string("vti_bc_encode_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<float *>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return VTI_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int vti_bc_encode_noalloc(vti_ae *A, const float *P, long long off, long long sy, long long sx, int n, float tol,
+                          float floor, unsigned *ent, long long cap, unsigned *boff, unsigned short *bcnt,
+                          long long *kept, double *err)
+{
+    if (!A || !P || !ent || !boff || !bcnt || n < 8 || n % 8 || !(tol >= 0.0f) || !(floor >= 0.0f) || cap < 0)
+        return aefail(VTI_ERR_ARG, "vti_bc_encode_noalloc: bad arguments");
+    if (n > kNoAllocSide)
+        return aefail(VTI_ERR_ARG, "vti_bc_encode_noalloc: box side " + std::to_string(n) + " above the maximum " +
+                                       std::to_string(kNoAllocSide) + " (encode larger volumes as boxes of at most " +
+                                       std::to_string(kNoAllocSide) + "^3 through off / sy / sx)");
+    if (!A->basis) return aefail(VTI_ERR_STATE, "sparse codec: no basis (fit or vti_ae_set_basis first)");
+    float *sp = nullptr;
+    int st = scratch_address(&sp);
+    if (st) return st;
+    bc::Work w;
+    w.cnt = reinterpret_cast<bc::Counters *>(sp);
+    w.coef = sp + kScratchHead;
+    w.cap_blocks = kNoAllocBlocks;
+    const bc::Mask m{0, 0, 0, 0, 0, 0};
+    st = bc::encode_in(bc::basis_view(A->basis), P, off, sy, sx, n, m, tol, floor, w, ent, cap, boff, bcnt, kept, err,
+                       nullptr, A->stream);
+    if (st) return aefail(st, "vti_bc_encode_noalloc: encode");
     AE_HIP(hipStreamSynchronize(A->stream));
     return VTI_OK;
 }

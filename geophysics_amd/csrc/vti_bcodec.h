@@ -68,7 +68,7 @@ Basis basis_view(const uint16_t *dev);
 // Coefficients of the n^3 box at P[off + y*sy + x*sx + z] (mask zeroed) into
 // w.coef; counters cleared, the strongest block's energy recorded.
 hipError_t launch_encode(const Basis &V, const float *P, long long off, long long sy, long long sx, int n,
-                         const Mask &m, Work &w, hipStream_t s);
+                         const Mask &m, const Work &w, hipStream_t s);
 // Per block: drop the smallest coefficients while their energy stays within
 // tol2 x max(block energy, flo x strongest block energy); entries of the
 // rest at ent + the block's range (blocks whose range would pass cap write
@@ -89,6 +89,11 @@ hipError_t launch_decode(const Basis &V, const uint32_t *ent, const uint32_t *bo
 int encode(const Basis &V, const float *P, long long off, long long sy, long long sx, int n, const Mask &m,
            float tol, float flo, Work &w, uint32_t *ent, long long cap, uint32_t *boff, uint16_t *bcnt,
            long long *kept, double *err, double *energy, hipStream_t s);
+// The same in a Work the caller has sized (no allocation): VTI_ERR_ARG when
+// w holds fewer than (n/8)^3 blocks or no counters.
+int encode_in(const Basis &V, const float *P, long long off, long long sy, long long sx, int n, const Mask &m,
+              float tol, float flo, const Work &w, uint32_t *ent, long long cap, uint32_t *boff, uint16_t *bcnt,
+              long long *kept, double *err, double *energy, hipStream_t s);
 
 }  // namespace bc
 }  // namespace vti

@@ -377,7 +377,7 @@ Basis basis_view(const uint16_t *dev)
 }
 
 hipError_t launch_encode(const Basis &V, const float *P, long long off, long long sy, long long sx, int n,
-                         const Mask &m, Work &w, hipStream_t s)
+                         const Mask &m, const Work &w, hipStream_t s)
 {
     const long long nb = n / 8, nblk = nb * nb * nb;
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t *>(w.cnt),
@@ -412,8 +412,18 @@ int encode(const Basis &V, const float *P, long long off, long long sy, long lon
            long long *kept, double *err, double *energy, hipStream_t s)
 {
     if (n < 8 || n % 8 || !(tol >= 0.0f) || !(flo >= 0.0f)) return VTI_ERR_ARG;
-    int st = work_reserve(w, n);
+    const int st = work_reserve(w, n);
     if (st) return st;
+    return encode_in(V, P, off, sy, sx, n, m, tol, flo, w, ent, cap, boff, bcnt, kept, err, energy, s);
+}
+
+int encode_in(const Basis &V, const float *P, long long off, long long sy, long long sx, int n, const Mask &m,
+              float tol, float flo, const Work &w, uint32_t *ent, long long cap, uint32_t *boff, uint16_t *bcnt,
+              long long *kept, double *err, double *energy, hipStream_t s)
+{
+    const long long nb = n / 8;
+    if (n < 8 || n % 8 || !(tol >= 0.0f) || !(flo >= 0.0f) || !w.cnt || !w.coef || nb * nb * nb > w.cap_blocks)
+        return VTI_ERR_ARG;
     // a kept value is stored to 2^-15 of itself and the two transforms add
     // ~2^-16 of the block norm each: 2^-26 of every block's budget is set
     // aside for them

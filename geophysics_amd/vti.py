@@ -148,6 +148,8 @@ def _declare(L):
         "vti_ae_get_basis": (C.c_int, [VP, FP]),
         "vti_bc_encode": (C.c_int, [VP, VP, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_float,
                                     C.c_float, VP, C.c_longlong, VP, VP, C.POINTER(C.c_longlong), DP]),
+        "vti_bc_encode_noalloc": (C.c_int, [VP, VP, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_float,
+                                            C.c_float, VP, C.c_longlong, VP, VP, C.POINTER(C.c_longlong), DP]),
         "vti_bc_decode": (C.c_int, [VP, VP, VP, VP, C.c_int, VP]),
         "vti_adcig_config": (C.c_int, [VP, C.c_int, C.c_int, C.c_int]),
         "vti_adcig_elems": (C.c_longlong, [VP]),
@@ -827,6 +829,15 @@ class Autoencoder:
         err = C.c_double()
         check(lib().vti_bc_encode(self.h, ptr, off, sy, sx, n, tol, floor, ent, cap, boff, bcnt, C.byref(kept),
                                   C.byref(err)), "bc_encode")
+        return kept.value, err.value
+
+    def bc_encode_noalloc(self, ptr, off, sy, sx, n, tol, ent, cap, boff, bcnt, floor=1e-8):
+        """bc_encode with no device allocation (n <= 128; scratch in a static
+        device array)."""
+        kept = C.c_longlong()
+        err = C.c_double()
+        check(lib().vti_bc_encode_noalloc(self.h, ptr, off, sy, sx, n, tol, floor, ent, cap, boff, bcnt,
+                                          C.byref(kept), C.byref(err)), "bc_encode_noalloc")
         return kept.value, err.value
 
     def bc_decode(self, ent, boff, bcnt, n, out):

@@ -378,6 +378,19 @@ int vti_ae_get_basis(vti_ae *ae, float *V);
 int vti_bc_encode(vti_ae *ae, const float *P, long long off, long long sy, long long sx, int n, float tol,
                   float floor, unsigned *ent, long long cap, unsigned *boff, unsigned short *bcnt, long long *kept,
                   double *err);
+/* vti_bc_encode without any allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and entries as vti_bc_encode (the entry order across
+ * blocks may differ; every block's entries are the same); its scratch (the
+ * dense coefficients and counters, 8 MiB + 256 bytes) is a static device
+ * array that every GPU the library runs on holds until the process ends,
+ * so n <= 128 (larger volumes: boxes of at most 128^3 through off / sy / sx;
+ * VTI_ERR_ARG beyond).  Calls that overlap on one GPU share that scratch
+ * and race; calls in sequence (this call returns after its work) do not. */
+int vti_bc_encode_noalloc(vti_ae *ae, const float *P, long long off, long long sy, long long sx, int n,
+                          float tol, float floor, unsigned *ent, long long cap, unsigned *boff,
+                          unsigned short *bcnt, long long *kept, double *err);
 /* Dense n^3 box (z fastest) from the entries of vti_bc_encode. */
 int vti_bc_decode(vti_ae *ae, const unsigned *ent, const unsigned *boff, const unsigned short *bcnt, int n,
                   float *out);
