@@ -480,12 +480,13 @@ def learned_rtm(args, torch, vti, nshot, nt, n, rows_out=None):
     device bytes the learned call held for the stored source field (store
     as allocated, source boxes, work buffers; stats alloc_bytes).  codec
     "sparse": error-bounded sparse coefficients (every snapshot within
-    relative L2 error --tol); codec "latent": the autoencoder's 64-channel
+    relative L2 error --tol); codec "quant": the same bound with every
+    coefficient quantized per block; codec "latent": the autoencoder's 64-channel
     latents with the exact-storage fallback (--exact-frac).  Returns the
     result record; each imaged shot is logged (and appended to rows_out)
     as it finishes."""
     npml = args.npml
-    sparse = args.codec == "sparse"
+    sparse = args.codec in ("sparse", "quant")
     log(f"[bench] learned RTM ({args.codec} codec): {n}^3, nt {nt}, {nshot} shots")
     vp, eps, dl = make_model(n, args.seed, torch.device("cuda:0"))
     torch.cuda.synchronize()
@@ -519,7 +520,7 @@ def learned_rtm(args, torch, vti, nshot, nt, n, rows_out=None):
         lo = vti.learned_defaults()
         lo.src_box = args.src_box
         if sparse:
-            lo.codec, lo.tol, lo.floor = 1, args.tol, args.floor
+            lo.codec, lo.tol, lo.floor = (2 if args.codec == "quant" else 1), args.tol, args.floor
             lo.budget = int(fp32_bytes * args.store_frac)
         else:
             lo.check_every, lo.tol, lo.ck_every, lo.max_exact = 1, 0.02, 0, max_exact
@@ -633,8 +634,9 @@ def main():
                          "instead of the unsplit interior / reduced split")
     ap.add_argument("--exact-frac", type=float, default=0.25,
                     help="learned RTM, latent codec: fraction of snapshots the exact-storage fallback may keep")
-    ap.add_argument("--codec", default="sparse", choices=["sparse", "latent"],
-                    help="learned RTM: error-bounded sparse block coefficients or autoencoder latents")
+    ap.add_argument("--codec", default="sparse", choices=["sparse", "quant", "latent"],
+                    help="learned RTM: error-bounded block coefficients (sparse: the largest at 15 significant "
+                         "bits; quant: all quantized with a per-block step) or autoencoder latents")
     ap.add_argument("--tol", type=float, default=0.0007,
                     help="learned RTM, sparse codec: relative L2 error bound of every stored block")
     ap.add_argument("--src-box", type=int, default=32,

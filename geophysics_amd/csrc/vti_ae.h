@@ -8,6 +8,7 @@
 
 #include "../../include/vti.h"
 #include "vti_bcodec.h"
+#include "vti_bqcodec.h"
 
 namespace vti {
 
@@ -32,6 +33,13 @@ int ae_bc_encode(vti_ae *A, const float *P, long long off, long long sy, long lo
                  float tol, float flo, uint32_t *ent, long long cap, uint32_t *boff, uint16_t *bcnt, long long *kept, double *err,
                  double *energy, hipStream_t s);
 int ae_bc_decode(vti_ae *A, const uint32_t *ent, const uint32_t *boff, const uint16_t *bcnt, int n, float *out,
+                 hipStream_t s);
+// Quantized codec (vti_bqcodec.h) with the same basis: block streams in
+// 32-bit words, at most bq::kMaxBlockWords per block.
+int ae_bq_encode(vti_ae *A, const float *P, long long off, long long sy, long long sx, int n, const bc::Mask &m,
+                 float tol, float flo, uint32_t *ent, long long cap, uint32_t *boff, uint16_t *bcnt, long long *kept,
+                 double *err, hipStream_t s);
+int ae_bq_decode(vti_ae *A, const uint32_t *ent, const uint32_t *boff, const uint16_t *bcnt, int n, float *out,
                  hipStream_t s);
 
 }  // namespace vti

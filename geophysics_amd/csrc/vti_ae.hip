@@ -499,6 +499,25 @@ int ae_bc_decode(vti_ae *A, const uint32_t *ent, const uint32_t *boff, const uin
     return VTI_OK;
 }
 
+int ae_bq_encode(vti_ae *A, const float *P, long long off, long long sy, long long sx, int n, const bc::Mask &m,
+                 float tol, float flo, uint32_t *ent, long long cap, uint32_t *boff, uint16_t *bcnt, long long *kept,
+                 double *err, hipStream_t s)
+{
+    if (!A->basis) return aefail(VTI_ERR_STATE, "quantized codec: no basis (fit or vti_ae_set_basis first)");
+    const int st = bq::encode(bc::basis_view(A->basis), P, off, sy, sx, n, m, tol, flo, A->bcw, ent, cap, boff, bcnt,
+                              kept, err, s);
+    if (st) return aefail(st, "quantized codec: encode");
+    return VTI_OK;
+}
+
+int ae_bq_decode(vti_ae *A, const uint32_t *ent, const uint32_t *boff, const uint16_t *bcnt, int n, float *out,
+                 hipStream_t s)
+{
+    if (!A->basis) return aefail(VTI_ERR_STATE, "quantized codec: no basis (fit or vti_ae_set_basis first)");
+    AE_HIP(bq::launch_decode(bc::basis_view(A->basis), ent, boff, bcnt, n, out, s));
+    return VTI_OK;
+}
+
 // Weights of the linear autoencoder whose encoder projects every 8^3 block
 // on the 64 leading principal components of the fitted blocks and whose
 // decoder reconstructs from them; the latent convolutions start as identity.
@@ -747,6 +766,35 @@ int vti_bc_decode(vti_ae *A, const unsigned *ent, const unsigned *boff, const un
     if (st) return st;
     AE_HIP(hipStreamSynchronize(A->stream));
     return VTI_OK;
+}
+
+int vti_bq_encode(vti_ae *A, const float *P, long long off, long long sy, long long sx, int n, float tol,
+                  float floor, unsigned *ent, long long cap, unsigned *boff, unsigned short *bcnt, long long *kept,
+                  double *err)
+{
+    if (!A || !P || !ent || !boff || !bcnt || n < 8 || n % 8 || !(tol >= 0.0f) || !(floor >= 0.0f) || cap < 0)
+        return aefail(VTI_ERR_ARG, "vti_bq_encode: bad arguments");
+    const bc::Mask m{0, 0, 0, 0, 0, 0};
+    const int st = ae_bq_encode(A, P, off, sy, sx, n, m, tol, floor, ent, cap, boff, bcnt, kept, err, A->stream);
+    if (st) return st;
+    AE_HIP(hipStreamSynchronize(A->stream));
+    return VTI_OK;
+}
+
+int vti_bq_decode(vti_ae *A, const unsigned *ent, const unsigned *boff, const unsigned short *bcnt, int n, float *out)
+{
+    if (!A || !ent || !boff || !bcnt || !out || n < 8 || n % 8) return aefail(VTI_ERR_ARG, "vti_bq_decode: bad arguments");
+    const int st = ae_bq_decode(A, ent, boff, bcnt, n, out, A->stream);
+    if (st) return st;
+    AE_HIP(hipStreamSynchronize(A->stream));
+    return VTI_OK;
+}
+
+long long vti_bq_max_words(int n)
+{
+    if (n < 8 || n % 8) return 0;
+    const long long nb = n / 8;
+    return nb * nb * nb * bq::kMaxBlockWords;
 }
 
 int vti_ae_destroy(vti_ae *A)

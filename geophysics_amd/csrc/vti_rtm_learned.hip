@@ -1,9 +1,11 @@
 // vti_rtm_learned.hip — 3D RTM with the source wavefield kept compressed
 // instead of being recomputed from checkpoints: as autoencoder latents
-// (codec 0), or as error-bounded sparse coefficients of the fitted block
-// basis (codec 1, vti_bcodec.h: every stored snapshot within relative L2
-// error tol, the source box exact, snapshots past the byte budget
-// recomputed from checkpoints).  The rest of this comment is codec 0.
+// (codec 0), or as error-bounded coefficients of the fitted block basis
+// (codec 1, vti_bcodec.h: the largest coefficients at 15 significant bits;
+// codec 2, vti_bqcodec.h: every coefficient quantized with a per-block
+// step; both keep every stored snapshot within relative L2 error tol, the
+// source box exact, and recompute snapshots past the byte budget from
+// checkpoints).  The rest of this comment is codec 0.
 //
 // First sweep (the forward modelling of the shot, with record and source
 // illumination exactly as vti_rtm_shot): after every step it with
@@ -223,13 +225,14 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         return fail(VTI_ERR_ARG, "vti_rtm_shot_learned: 3D cubic interiors with a side divisible by 8 only");
     if (c->nrec <= 0) return fail(VTI_ERR_STATE, "vti_rtm_shot_learned: the shot has no receivers");
     if (lo->save_every < 1 || lo->check_every < 0 || lo->ck_every < 0 || lo->max_exact < 0 || lo->src_box < 0 ||
-        lo->codec < 0 || lo->codec > 1 || lo->budget < 0 ||
-        (lo->codec == 1 && (!(lo->tol >= 0.0f) || !(lo->floor >= 0.0f))))
+        lo->codec < 0 || lo->codec > 2 || lo->budget < 0 ||
+        (lo->codec >= 1 && (!(lo->tol >= 0.0f) || !(lo->floor >= 0.0f))))
         return fail(VTI_ERR_ARG, "vti_rtm_shot_learned: bad options");
     if (c->adcig_na > 0)
         return fail(VTI_ERR_STATE, "vti_rtm_shot_learned: angle gathers need the full source state "
                                    "(vti_rtm_shot); switch them off with vti_adcig_config(ctx, 0, 0, 0)");
-    const bool sparse = lo->codec == 1;
+    const bool sparse = lo->codec >= 1;   // codec 1 or 2: block streams in 32-bit words
+    const bool quant = lo->codec == 2;
     int st;
     if (!c->img && (st = vti_rtm_reset(c))) return st;
     const int nt = k.nt, nrec = c->nrec, np = k.npml, se = lo->save_every;
@@ -245,15 +248,16 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         hipMalloc(&dense.p, (size_t)n * n * n * sizeof(float)) != hipSuccess ||
         hipMalloc(&acc.p, 2 * sizeof(double)) != hipSuccess)
         return fail(VTI_ERR_ALLOC, "vti_rtm_shot_learned: latent store");
-    // sparse codec store: entries (4 bytes each) in chunks allocated as the
-    // snapshots need them, at most `budget` bytes in all; block tables per
-    // snapshot.  A chunk is taken when the current one has room for less than
-    // one snapshot's largest possible entry count (n^3), so an encode never
-    // runs out of room inside a chunk and the store holds what the entries
-    // use plus less than one chunk.
+    // codec 1 / 2 store: words (entries, or quantized block streams; 4 bytes
+    // each) in chunks allocated as the snapshots need them, at most `budget`
+    // bytes in all; block tables per snapshot.  A chunk is taken when the
+    // current one has room for less than one snapshot's largest possible word
+    // count (n^3 entries; (n/8)^3 x bq::kMaxBlockWords stream words), so an
+    // encode never runs out of room inside a chunk and the store holds what
+    // the words use plus less than one chunk.
     const long long nblk = (long long)(n / 8) * (n / 8) * (n / 8);
     const long long budget = lo->budget > 0 ? lo->budget : (long long)nsave * n * n * n;
-    const long long snap_max = (long long)n * n * n;                    // entries of one snapshot at most
+    const long long snap_max = quant ? nblk * bq::kMaxBlockWords : (long long)n * n * n;   // words of one snapshot at most
     const long long chunk_ents = std::max(snap_max, (4LL << 30) / 4);   // 4 GB chunks
     if (sparse && (hipMalloc(&boffs.p, (size_t)nsave * nblk * 4) != hipSuccess ||
                    hipMalloc(&bcnts.p, (size_t)nsave * nblk * 2) != hipSuccess))
@@ -357,9 +361,11 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
             }
             long long need = 0;
             double e = 0.0;
-            if ((st = ae_bc_encode(A, P, off, c->g.ldxy, c->g.ldz, n, m, lo->tol, lo->floor, cur, chunk_room,
-                                   BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, &need, &e, nullptr, s)))
-                return st;
+            st = quant ? ae_bq_encode(A, P, off, c->g.ldxy, c->g.ldz, n, m, lo->tol, lo->floor, cur, chunk_room,
+                                      BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, &need, &e, s)
+                       : ae_bc_encode(A, P, off, c->g.ldxy, c->g.ldz, n, m, lo->tol, lo->floor, cur, chunk_room,
+                                      BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, &need, &e, nullptr, s);
+            if (st) return st;
             if ((st = hip(t_enc.end(s), "event"))) return st;
             if (need > chunk_room) {   // over budget: recompute this snapshot from a checkpoint
                 if (!nck) return fail(VTI_ERR_ALLOC, "vti_rtm_shot_learned: sparse store budget exhausted (set ck_every)");
@@ -447,7 +453,8 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
             if ((st = hip(launch_image(c, W.f[VTI_F_P], G.f[VTI_F_P], c->img), "image"))) return st;
         } else {
             if ((st = hip(t_dec.begin(s), "event"))) return st;
-            if (sparse) st = ae_bc_decode(A, ebase[j], BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, n, D, s);
+            if (quant) st = ae_bq_decode(A, ebase[j], BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, n, D, s);
+            else if (sparse) st = ae_bc_decode(A, ebase[j], BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, n, D, s);
             else st = ae_decode(A, L + (size_t)j * lat_n, Sc + (size_t)j * 16, n, D, s);
             if (st) return st;
             if (box_n)

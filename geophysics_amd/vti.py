@@ -65,6 +65,12 @@ class LearnedOpts(C.Structure):
                 ("budget", C.c_longlong)]
 
 
+class SuGeometry(C.Structure):
+    _fields_ = [(k, C.c_int) for k in ("nshot", "ntr", "ns", "dt_us", "fldr0", "scalco")] + \
+               [(k, C.c_double) for k in ("sx0", "dsx", "sy0", "dsy", "gx0", "dgx", "dgx_shot", "gy0", "dgy",
+                                          "dgy_shot", "sz", "gz")]
+
+
 FP = C.POINTER(C.c_float)
 DP = C.POINTER(C.c_double)
 VP = C.c_void_p
@@ -151,6 +157,10 @@ def _declare(L):
         "vti_bc_encode_noalloc": (C.c_int, [VP, VP, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_float,
                                             C.c_float, VP, C.c_longlong, VP, VP, C.POINTER(C.c_longlong), DP]),
         "vti_bc_decode": (C.c_int, [VP, VP, VP, VP, C.c_int, VP]),
+        "vti_bq_encode": (C.c_int, [VP, VP, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_float,
+                                    C.c_float, VP, C.c_longlong, VP, VP, C.POINTER(C.c_longlong), DP]),
+        "vti_bq_decode": (C.c_int, [VP, VP, VP, VP, C.c_int, VP]),
+        "vti_bq_max_words": (C.c_longlong, [C.c_int]),
         "vti_adcig_config": (C.c_int, [VP, C.c_int, C.c_int, C.c_int]),
         "vti_adcig_elems": (C.c_longlong, [VP]),
         "vti_adcig_tables": (C.c_int, [C.c_int, C.c_int, FP, FP]),
@@ -183,6 +193,12 @@ def _declare(L):
     L.vti_omp2dfd.restype = C.c_int
     L.vti_omp2dfd.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_float, C.c_float,
                               C.c_int, C.c_int, C.c_int, C.c_float] + [C.c_char_p] * 5
+    L.vti_bin2su.restype = C.c_int
+    L.vti_bin2su.argtypes = [C.c_char_p] * 3 + [C.c_longlong, C.c_int, C.c_int]
+    L.vti_su_write.restype = C.c_int
+    L.vti_su_write.argtypes = [C.c_char_p, FP, C.POINTER(SuGeometry), C.c_int]
+    L.vti_segy_write.restype = C.c_int
+    L.vti_segy_write.argtypes = [C.c_char_p, FP, C.POINTER(SuGeometry), C.c_char_p]
     L.GPU_vti3dfd.restype = None
     L.GPU_vti3dfd.argtypes = [C.c_int] * 8 + [C.c_char_p] * 3 + [C.c_int] * 8 + [C.c_char_p] * 2 + \
         [C.c_int] * 3
@@ -408,8 +424,9 @@ class Propagator:
 
     def rtm_shot_learned(self, ae, opts=None, lopts=None, obs=None):
         """RTM of the current shot with the source field stored compressed
-        (autoencoder latents, or sparse block-transform coefficients with
-        lopts.codec = 1); returns the stats dict."""
+        (autoencoder latents, sparse block-transform coefficients with
+        lopts.codec = 1, quantized block streams with lopts.codec = 2);
+        returns the stats dict."""
         o = opts if opts is not None else self.rtm_defaults()
         lo = lopts if lopts is not None else learned_defaults()
         st = np.zeros(9, np.float64)
@@ -568,6 +585,41 @@ def omp2dfd(FNv, FNe, FNd, FNshot, FNsnap, nx=601, nz=301, dx=5.0, dz=5.0, npd=0
     as defaults (files in, shot / snapshot files out)."""
     check(lib().vti_omp2dfd(nx, nz, dx, dz, npd, favg, pfac, sx, sz, nt, dt, _b(FNv), _b(FNe), _b(FNd),
                             _b(FNshot), _b(FNsnap)), "vti_omp2dfd")
+
+
+def bin2su(su_template, raw, out, ntrace, ns, swap=True):
+    """utils-binary2segy/binary2segy.c: template SU headers + raw float
+    traces (bytes of every sample reversed when swap) -> out."""
+    check(lib().vti_bin2su(_b(su_template), _b(raw), _b(out), ntrace, ns, 1 if swap else 0), "vti_bin2su")
+
+
+def su_geometry(nshot, ntr, ns, dt_us, **kw):
+    """vti_su_geometry with fldr0 = 1, scalco = 1 and zero coordinates unless
+    given (metres: sx0, dsx, sy0, dsy, gx0, dgx, dgx_shot, gy0, dgy,
+    dgy_shot, sz, gz)."""
+    g = SuGeometry(nshot=nshot, ntr=ntr, ns=ns, dt_us=dt_us, fldr0=kw.pop("fldr0", 1), scalco=kw.pop("scalco", 1))
+    for k, v in kw.items():
+        setattr(g, k, v)
+    return g
+
+
+def _records(rec, g):
+    a, p = _f32(rec)
+    if a.size != g.nshot * g.ntr * g.ns:
+        raise VtiError(f"records must hold nshot x ntr x ns = {g.nshot} x {g.ntr} x {g.ns} samples")
+    return a, p
+
+
+def su_write(path, rec, g, big_endian=False):
+    """Records (nshot, ntr, ns) or (ntr, ns), trace-major, to an SU file."""
+    a, p = _records(rec, g)
+    check(lib().vti_su_write(_b(path), p, C.byref(g), 1 if big_endian else 0), "vti_su_write")
+
+
+def segy_write(path, rec, g, text=None):
+    """Records to a big-endian SEG-Y rev 1 file (IEEE floats)."""
+    a, p = _records(rec, g)
+    check(lib().vti_segy_write(_b(path), p, C.byref(g), text.encode() if text else None), "vti_segy_write")
 
 
 def _f64(a):
@@ -842,6 +894,23 @@ class Autoencoder:
 
     def bc_decode(self, ent, boff, bcnt, n, out):
         check(lib().vti_bc_decode(self.h, ent, boff, bcnt, n, out), "bc_decode")
+
+    def bq_encode(self, ptr, off, sy, sx, n, tol, ent, cap, boff, bcnt, floor=1e-8):
+        """Quantized codec encode (same arguments and error bound as
+        bc_encode; ent holds 32-bit stream words, at most bq_max_words(n));
+        returns (words needed, error bound)."""
+        kept = C.c_longlong()
+        err = C.c_double()
+        check(lib().vti_bq_encode(self.h, ptr, off, sy, sx, n, tol, floor, ent, cap, boff, bcnt, C.byref(kept),
+                                  C.byref(err)), "bq_encode")
+        return kept.value, err.value
+
+    def bq_decode(self, ent, boff, bcnt, n, out):
+        check(lib().vti_bq_decode(self.h, ent, boff, bcnt, n, out), "bq_decode")
+
+    @staticmethod
+    def bq_max_words(n):
+        return int(lib().vti_bq_max_words(n))
 
     @staticmethod
     def latent_elems(n):

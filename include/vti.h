@@ -394,6 +394,33 @@ int vti_bc_encode_noalloc(vti_ae *ae, const float *P, long long off, long long s
 /* Dense n^3 box (z fastest) from the entries of vti_bc_encode. */
 int vti_bc_decode(vti_ae *ae, const unsigned *ent, const unsigned *boff, const unsigned short *bcnt, int n,
                   float *out);
+/* Quantized block-transform codec (device pointers, the same box, basis
+ * and error bound as vti_bc_encode: every 8^3 block's L2 error <=
+ * tol x max(block norm, sqrt(floor) x the strongest block's norm)).  Every
+ * coefficient of a block is quantized with the block's own uniform step d,
+ * q = rint(c / d) (the largest d whose exact quantization error meets the
+ * budget, found by bisection on d's float bits); a block is a stream of
+ * 32-bit words at ent + boff[b], bcnt[b] words (0: the block is within its
+ * budget at q = 0):
+ *   word 0        d (fp32 bits; 0 in raw mode)
+ *   word 1        bit widths of coefficient groups 0..5 (5 bits each,
+ *                 group g = components 64g..64g+63) | mode << 30
+ *   word 2        widths of groups 6, 7 | number of nonzero q << 10
+ *   words 3..18   bitmap of the nonzero q (component j: bit j%32 of word 3+j/32)
+ *   words 19..    the nonzero q in component order, |q| << 1 | sign, each in
+ *                 its group's width, packed from bit 0 of word 19 upward
+ *                 (raw mode 1, when even d = 2^-22 x the block's largest
+ *                 |c| misses the budget: the nonzero c as fp32 words)
+ * A block takes at most 531 words; vti_bq_max_words(n) = (n/8)^3 x 531.
+ * *kept = words needed (> cap: incomplete, nothing written past cap);
+ * *err = the bound on the box's relative L2 error.  Both calls return after
+ * the autoencoder's stream has finished.  No counterpart in the reference. */
+int vti_bq_encode(vti_ae *ae, const float *P, long long off, long long sy, long long sx, int n, float tol,
+                  float floor, unsigned *ent, long long cap, unsigned *boff, unsigned short *bcnt, long long *kept,
+                  double *err);
+int vti_bq_decode(vti_ae *ae, const unsigned *ent, const unsigned *boff, const unsigned short *bcnt, int n,
+                  float *out);
+long long vti_bq_max_words(int n);
 /* Symmetric eigen-decomposition used by the fit (cyclic Jacobi, host):
  * a n x n row-major, w eigenvalues, v eigenvectors as columns, unsorted. */
 int vti_sym_eigen(int n, const double *a, double *w, double *v);
@@ -423,18 +450,19 @@ typedef struct {
     int codec;         /* 0: autoencoder latents; 1: sparse block transform, every 8^3 block
                           of every stored snapshot within relative L2 error tol of
                           max(its energy, floor x the strongest block's energy)
-                          (check_every, max_exact unused) */
-    float floor;       /* codec 1: 1e-8 */
-    long long budget;  /* codec 1: bytes for kept coefficients (0: nsnap * n^3, a quarter of the
-                          fp32 snapshots); past it, snapshots are recomputed from the ck_every
-                          checkpoints (or the call fails without them) */
+                          (check_every, max_exact unused); 2: the same bound with the
+                          quantized codec (vti_bq_encode) */
+    float floor;       /* codecs 1, 2: 1e-8 */
+    long long budget;  /* codecs 1, 2: bytes for the coded snapshots (0: nsnap * n^3, a quarter
+                          of the fp32 snapshots); past it, snapshots are recomputed from the
+                          ck_every checkpoints (or the call fails without them) */
 } vti_learned_opts;
 int vti_learned_defaults(vti_learned_opts *o);
 /* stats (optional, 9 doubles): encode ms, decode ms (GPU time between
  * HIP events around the codec's launches; the encode's up to the host
  * synchronisation it needs to place the entries), mean
- * checked relative error (codec 1: mean error bound of the stored
- * snapshots), snapshots above tol (codec 1: over budget), snapshots,
+ * checked relative error (codecs 1, 2: mean error bound of the stored
+ * snapshots), snapshots above tol (codecs 1, 2: over budget), snapshots,
  * snapshots kept exactly, bytes of the compressed store (latents or
  * entries + block tables, exactly kept snapshots, source boxes), snapshots
  * recomputed from checkpoints, and the device bytes the call held for the
@@ -544,6 +572,42 @@ int vti_calib_copy(void *dst, const void *src, size_t nbytes, void *hip_stream);
 int vti_sync(vti_ctx *ctx);
 const char *vti_strerror(int code);
 const char *vti_last_error(void);
+
+/* ---- shot records to SU / SEG-Y files (host) ----------------------------- */
+/* vti_bin2su: utils-binary2segy/binary2segy.c:43-85 — each of ntrace output
+ * traces is the 240-byte header of the template SU file's trace (kept as it
+ * is) and ns samples from the raw float file, each sample's bytes reversed
+ * when swap is non-zero (the reference always swaps, swap_float_4
+ * :136-144).  Unlike the reference, a short template or raw file is an error
+ * (VTI_ERR_IO), not garbage output. */
+int vti_bin2su(const char *su_template, const char *raw, const char *out, long long ntrace, int ns, int swap);
+/* Acquisition geometry of nshot shots of ntr traces of ns samples (the
+ * trace-major records the library returns and the drop-ins' shot files
+ * hold, shots one after another).  Metres; trace it of shot is has its
+ * receiver at gx0 + is*dgx_shot + it*dgx (y alike) and its source at
+ * sx0 + is*dsx; coordinates are stored as integers with scalco (SEG-Y:
+ * negative divides, so -100 keeps centimetres); fldr = fldr0 + is. */
+typedef struct {
+    int nshot, ntr, ns, dt_us;
+    int fldr0;
+    int scalco;
+    double sx0, dsx, sy0, dsy;
+    double gx0, dgx, dgx_shot;
+    double gy0, dgy, dgy_shot;
+    double sz, gz;   /* source depth, receiver depth (gelev = -gz) */
+} vti_su_geometry;
+/* SU file: 240-byte trace headers (tracl, tracr, fldr, tracf, ep, trid 1,
+ * offset, gelev, sdepth, scalel 1, scalco, sx, sy, gx, gy, counit 1, ns, dt)
+ * and float samples, in the host's byte order (big_endian = 0) or
+ * big-endian.  No counterpart in the reference (it converts existing SU
+ * files, vti_bin2su). */
+int vti_su_write(const char *path, const float *rec, const vti_su_geometry *g, int big_endian);
+/* SEG-Y revision 1 file, big-endian: 3200-byte EBCDIC textual header (40
+ * cards "C nn " + the lines of text, may be NULL), 400-byte binary header
+ * (traces per record = ntr <= 32767, dt, ns, format 5 IEEE float, sorting 1,
+ * metres, revision 0x0100, fixed length), then the SU trace headers and
+ * samples big-endian.  No counterpart in the reference. */
+int vti_segy_write(const char *path, const float *rec, const vti_su_geometry *g, const char *text);
 
 /* ---- drop-in entry points with the reference signatures ------------------ */
 /* GPU_vti2dfd_kernels.cu:301-304 (caller GTK_gpuvti2dfd.c:88). */

@@ -1,0 +1,285 @@
+"""Quantized block-transform codec (vti_bq_encode / vti_bq_decode) and the 3D
+RTM that stores its source wavefield with it: the word streams parsed here
+in numpy from the format the header documents, the reconstruction and every
+block's error against float64, the encode's limits (capacity, raw blocks,
+empty blocks, partial workgroups), and the RTM image against the plain
+checkpointed RTM imaging the same steps."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2, smooth_model3d
+from geophysics_amd import vti
+
+pytestmark = pytest.mark.gpu
+
+N, NPML, NT = 32, 6, 200
+MAXW = 19 + 512
+SENTINEL = 0xDEADBEEF
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _blocks(x):
+    """(n^3) box -> [(n/8)^3 blocks][512], blocks z fastest, voxels (ty*8+tx)*8+tz."""
+    nb = x.shape[0] // 8
+    return x.reshape(nb, 8, nb, 8, nb, 8).transpose(0, 2, 4, 1, 3, 5).reshape(nb ** 3, 512)
+
+
+def _unblocks(b, n):
+    nb = n // 8
+    return b.reshape(nb, nb, nb, 8, 8, 8).transpose(0, 3, 1, 4, 2, 5).reshape(n, n, n)
+
+
+def _orthonormal(seed):
+    q, r = np.linalg.qr(np.random.default_rng(seed).standard_normal((512, 512)))
+    return (q * np.sign(np.diag(r))).astype(np.float32)
+
+
+def _field(seed=7, steps=120):
+    vp, e, d = smooth_model3d(N, N, N, seed=seed)
+    cfg = vti.cfg_ref3d(N, N, N, 10, 10, 10, NPML, 1, 25, NT, 800)
+    with vti.Propagator(cfg, vp, e, d) as prop:
+        prop.set_shot(vti.shot_ref3d(cfg, 0, 12, 0, 20, 0, 1, 0, rec_depth=1, rec_step=2))
+        prop.step(0, steps)
+        return prop.field("P")
+
+
+def parse_block(words):
+    """One block's stream -> (coefficients as the decoder forms them (fp32),
+    q (int, None in raw mode), d, mode, widths), read bit by bit from the
+    layout: d | widths 0..5, mode | widths 6, 7, count | bitmap | values."""
+    words = np.asarray(words, np.uint32)
+    assert len(words) >= 19
+    d = words[0:1].view(np.float32)[0]
+    h1, h2 = int(words[1]), int(words[2])
+    mode = h1 >> 30
+    widths = [(h1 >> (5 * g)) & 31 for g in range(6)] + [(h2 >> (5 * g)) & 31 for g in range(2)]
+    assert mode in (0, 1)
+    nz = h2 >> 10
+    bitmap = [(int(words[3 + j // 32]) >> (j % 32)) & 1 for j in range(512)]
+    assert sum(bitmap) == nz
+    stream = 0
+    for i, w in enumerate(words):
+        stream |= int(w) << (32 * i)
+    pos = 19 * 32
+    c = np.zeros(512, np.float32)
+    q = None if mode else np.zeros(512, np.int64)
+    for j in range(512):
+        if not bitmap[j]:
+            continue
+        if mode:
+            c[j] = np.array([(stream >> pos) & 0xFFFFFFFF], np.uint32).view(np.float32)[0]
+            pos += 32
+            continue
+        w = widths[j // 64]
+        assert w > 0
+        code = (stream >> pos) & ((1 << w) - 1)
+        mag = code >> 1
+        q[j] = -mag if code & 1 else mag
+        assert q[j] != 0
+        c[j] = np.float32(q[j]) * d
+        pos += w
+    assert (pos + 31) // 32 == len(words), (pos, len(words))
+    if not mode:   # every group's width is its largest |q|'s bits + a sign bit
+        for g in range(8):
+            m = int(np.abs(q[64 * g:64 * g + 64]).max())
+            assert widths[g] == (m.bit_length() + 1 if m else 0), (g, widths[g], m)
+    return c, q, d, mode, widths
+
+
+def _codec(A, torch, P_pad, n, tol, cap, off_yxz=(NPML, NPML, NPML), floor=1e-8, guard=0):
+    """Encode the n^3 box of the padded array at off_yxz and decode it.  The
+    word buffer holds cap words and `guard` more, all SENTINEL before the
+    call; the returned words include the guard."""
+    dev = torch.device("cuda:0")
+    Pd = torch.from_numpy(np.ascontiguousarray(P_pad)).to(dev)
+    ny, nx, nz = P_pad.shape
+    off = off_yxz[2] + off_yxz[1] * nz + off_yxz[0] * nx * nz
+    nblk = (n // 8) ** 3
+    ent = torch.full((max(cap, 1) + guard,), SENTINEL - 2 ** 32, dtype=torch.int32, device=dev)
+    boff = torch.zeros(nblk, dtype=torch.int32, device=dev)
+    bcnt = torch.zeros(nblk, dtype=torch.int16, device=dev)
+    torch.cuda.synchronize()
+    kept, err = A.bq_encode(Pd.data_ptr(), off, nx * nz, nz, n, tol, ent.data_ptr(), cap, boff.data_ptr(),
+                            bcnt.data_ptr(), floor=floor)
+    out = None
+    if kept <= cap:
+        o = torch.full((n ** 3,), np.nan, dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+        A.bq_decode(ent.data_ptr(), boff.data_ptr(), bcnt.data_ptr(), n, o.data_ptr())
+        out = o.cpu().numpy().reshape(n, n, n)
+    torch.cuda.synchronize()
+    return kept, err, out, ent.cpu().numpy().view(np.uint32), boff.cpu().numpy().view(np.uint32), \
+        bcnt.cpu().numpy().view(np.uint16)
+
+
+def test_max_words():
+    assert vti.Autoencoder.bq_max_words(32) == 64 * MAXW
+    assert vti.Autoencoder.bq_max_words(8) == MAXW
+    assert vti.Autoencoder.bq_max_words(12) == 0
+
+
+@pytest.mark.parametrize("tol,floor", [(0.0, 1e-8), (0.01, 1e-8), (0.05, 1e-8), (0.2, 1e-8), (0.05, 1e-2)])
+def test_streams_error_bound_and_decode(tol, floor):
+    """Random orthonormal basis, an FD snapshot: the streams parse (format,
+    counts, group widths), the parsed values are the float64 projection
+    quantized with the block's step (|c - q d| <= d/2 up to the transform's
+    error), every block's decoded error is within tol of max(its norm,
+    sqrt(floor) x the strongest block's), the box within the reported
+    bound, and the decode equals numpy's reconstruction from the parsed
+    coefficients."""
+    import torch
+    V = _orthonormal(11)
+    P = _field()
+    box = P[NPML:NPML + N, NPML:NPML + N, NPML:NPML + N].astype(np.float64)
+    with vti.Autoencoder(crop=16, batch=1, seed=1) as A:
+        A.set_basis(V)
+        kept, err, out, ent, boff, bcnt = _codec(A, torch, P, N, tol, 64 * MAXW, floor=floor)
+    e = rel_l2(out, box)
+    raw = sum(1 for b in range(64) if bcnt[b] and (int(ent[boff[b] + 1]) >> 30) == 1)
+    _say(f"tol {tol} floor {floor}: {kept} words ({kept * 4 / (4 * N ** 3):.3f} of fp32), {raw} raw blocks, "
+         f"bound {err:.4e}, error {e:.4e}")
+    assert e <= err * (1 + 1e-3) + 1e-5
+    assert int(bcnt.astype(np.int64).sum()) == kept
+    xb = _blocks(box)
+    eb = (xb ** 2).sum(axis=1)
+    ref = np.maximum(eb, floor * eb.max())
+    errb = ((_blocks(out.astype(np.float64)) - xb) ** 2).sum(axis=1)
+    assert np.all(errb <= max(tol * tol, 2.0 ** -26) * ref * 1.01 + 1e-9 * eb + 1e-30)
+    c64 = xb @ V.astype(np.float64)
+    dense = np.zeros_like(c64)
+    ranges = []
+    for b in range(64):
+        s0, k = int(boff[b]), int(bcnt[b])
+        ranges.append((s0, s0 + k))
+        if not k:
+            assert eb[b] <= tol * tol * ref[b] * 1.01 + 1e-30   # within budget at q = 0
+            continue
+        cb, q, d, mode, _ = parse_block(ent[s0:s0 + k])
+        dense[b] = cb
+        slack = 4e-5 * np.sqrt(eb[b])
+        if mode:
+            assert np.all(np.abs(cb - c64[b]) <= slack)
+        else:
+            assert d > 0
+            assert np.all(np.abs(c64[b] - q * np.float64(d)) <= 0.5 * d + slack)
+    ranges.sort()
+    for (a0, a1), (b0, _) in zip(ranges, ranges[1:]):
+        assert a1 <= b0
+    rec = _unblocks(dense @ V.astype(np.float64).T, N)
+    assert rel_l2(out, rec) < 1e-4
+    if tol == 0.0:
+        assert raw == sum(1 for b in range(64) if bcnt[b])   # no step meets a zero budget
+        assert e < 1e-4
+
+
+def test_zero_box_and_capacity():
+    """An all-zero box needs no words (every block empty, the decode zero);
+    a capacity below the need writes nothing past it, and the blocks that
+    fit hold streams that parse."""
+    import torch
+    V = _orthonormal(5)
+    n = 16
+    rng = np.random.default_rng(3)
+    big = rng.standard_normal((n + 5, n + 3, n + 9)).astype(np.float32)
+    with vti.Autoencoder(crop=16, batch=1, seed=1) as A:
+        A.set_basis(V)
+        kz, errz, outz, _, _, bcz = _codec(A, torch, np.zeros_like(big), n, 0.05, 8 * MAXW, off_yxz=(2, 1, 5))
+        assert kz == 0 and errz == 0.0 and not bcz.any() and not outz.any()
+        kept, err, out, ent, boff, bcnt = _codec(A, torch, big, n, 0.1, 8 * MAXW, off_yxz=(2, 1, 5))
+        box = big[2:2 + n, 1:1 + n, 5:5 + n]
+        assert rel_l2(out, box) <= err + 1e-6 and err <= 0.1
+        cap, guard = kept - 1, 4096
+        kept2, _, out2, ent2, boff2, bcnt2 = _codec(A, torch, big, n, 0.1, cap, off_yxz=(2, 1, 5), guard=guard)
+    assert kept2 == kept and out2 is None
+    assert np.all(ent2[cap:] == SENTINEL)
+    ends = boff2.astype(np.int64) + bcnt2.astype(np.int64)
+    fit = (ends <= cap) & (bcnt2 > 0)
+    assert fit.any()
+    for b in np.nonzero(fit)[0]:
+        parse_block(ent2[boff2[b]:ends[b]])
+
+
+@pytest.mark.parametrize("n", [24, 8])
+def test_block_counts_off_the_pack_workgroup(n):
+    """27 and 1 blocks: the pack's last workgroup has idle wavefronts.  The
+    block table covers exactly the words kept and every block is within its
+    bound."""
+    import torch
+    V = _orthonormal(9)
+    rng = np.random.default_rng(n)
+    big = rng.standard_normal((n + 5, n + 3, n + 9)).astype(np.float32)
+    tol = 0.05
+    with vti.Autoencoder(crop=16, batch=1, seed=1) as A:
+        A.set_basis(V)
+        kept, err, out, ent, boff, bcnt = _codec(A, torch, big, n, tol, (n // 8) ** 3 * MAXW, off_yxz=(2, 1, 5))
+    box = big[2:2 + n, 1:1 + n, 5:5 + n].astype(np.float64)
+    assert 0 < kept and err <= tol
+    assert int(bcnt.astype(np.int64).sum()) == kept
+    assert rel_l2(out, box) <= err + 1e-6
+    nb = n // 8
+    for i in range(nb ** 3):
+        bz, bx, by = i % nb, (i // nb) % nb, i // (nb * nb)
+        sl = np.s_[8 * by:8 * by + 8, 8 * bx:8 * bx + 8, 8 * bz:8 * bz + 8]
+        assert np.linalg.norm(out[sl] - box[sl]) <= tol * np.linalg.norm(box[sl]) * (1 + 1e-3) + 1e-6
+        parse_block(ent[boff[i]:boff[i] + bcnt[i]])
+
+
+def _setup(seed=7):
+    vp, e, d = smooth_model3d(N, N, N, seed=seed)
+    cfg = vti.cfg_ref3d(N, N, N, 10, 10, 10, NPML, 1, 25, NT, 800)
+    return cfg, vp, e, d
+
+
+def _shot(cfg, cx, cy):
+    return vti.shot_ref3d(cfg, 0, cx, 0, cy, 0, 1, 0, rec_depth=1, rec_step=2)
+
+
+def test_quantized_rtm_image_and_store_against_sparse():
+    """Basis fitted on two shots, a third shot imaged with codec 2 at tol 1 %
+    and 0.2 %: the image against the plain RTM image of the same steps within
+    the same bounds as codec 1, in fewer stored bytes than codec 1 at the same
+    tol; a budget below the need with checkpoints recomputes every snapshot
+    (bit-identical image)."""
+    cfg, vp, e, d = _setup()
+    with vti.Propagator(cfg, vp, e, d) as prop, vti.Autoencoder(crop=32, batch=1, seed=2) as A:
+        for cx, cy in ((10, 12), (22, 20)):
+            prop.set_shot(_shot(cfg, cx, cy))
+            prop.ae_fit_shot(A, save_every=4, nblocks=64)
+        A.fit_end()
+        prop.set_shot(_shot(cfg, 16, 16))
+        o = prop.rtm_defaults()
+        o.image_every, o.nsnap = 4, 6
+        prop.rtm_reset()
+        prop.rtm_shot(o, None)
+        img_p, ill_p = prop.rtm_image(finalize=False)
+        fin_p, _ = prop.rtm_image(finalize=True)
+        res = {}
+        for tol in (0.01, 0.002):
+            for codec in (1, 2):
+                lo = vti.learned_defaults()
+                lo.codec, lo.tol = codec, tol
+                lo.budget = (NT // 4) * vti.Autoencoder.bq_max_words(N) * 4
+                prop.rtm_reset()
+                st = prop.rtm_shot_learned(A, prop.rtm_defaults(), lo, None)
+                img_l, ill_l = prop.rtm_image(finalize=False)
+                fin_l, _ = prop.rtm_image(finalize=True)
+                assert np.array_equal(ill_l, ill_p)
+                res[tol, codec] = (rel_l2(img_l, img_p), rel_l2(fin_l, fin_p), st)
+                _say(f"codec {codec} tol {tol}: raw image {res[tol, codec][0]:.3e}, "
+                     f"final image {res[tol, codec][1]:.3e}, store {st['store_bytes'] / 1e6:.3f} MB, {st}")
+                assert st["recomputed"] == 0 and st["snapshots"] == NT // 4
+                assert st["mean_checked_err"] <= tol
+        lo = vti.learned_defaults()
+        lo.codec, lo.tol, lo.budget, lo.ck_every = 2, 0.0, 4, 16
+        prop.rtm_reset()
+        st = prop.rtm_shot_learned(A, prop.rtm_defaults(), lo, None)
+        img_r, _ = prop.rtm_image(finalize=False)
+    assert res[0.01, 2][0] < 5e-3 and res[0.01, 2][1] < 3e-2
+    assert res[0.002, 2][0] < 1e-3 and res[0.002, 2][1] < 1e-2
+    for tol in (0.01, 0.002):
+        assert res[tol, 2][2]["store_bytes"] < 0.8 * res[tol, 1][2]["store_bytes"]
+    assert st["recomputed"] > 0
+    assert np.array_equal(img_r, img_p), rel_l2(img_r, img_p)

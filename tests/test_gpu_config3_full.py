@@ -67,6 +67,9 @@ def test_config3_256_uneven_shots_full_size():
                            unsplit_source=True, unsplit_receiver=True)
     assert np.array_equal(got[small][1], rill)
     assert np.array_equal(got[small][0], rimg), rel_l2(got[small][0], rimg)
-    i64, _, _ = po.rtm(p, *models[small], sv_fwd=1, sv_bwd=0, illum_mode=0, mute=mute, prec="f64")
-    _say(f"smallest shot {nx}x{nz}: bit for bit the oracle's RTM; rel-L2 vs the FP64 split-form RTM "
-         f"{rel_l2(got[small][0], i64):.2e}")
+    # the mute only zeroes samples: the FP64 record takes the FP32 mute's mask
+    mute64 = lambda r: np.multiply(r, mute(np.ones(r.shape, np.float32)), out=r)
+    i64, _, _ = po.rtm(p, *models[small], sv_fwd=1, sv_bwd=0, illum_mode=0, mute=mute64, prec="f64")
+    e64 = rel_l2(got[small][0], i64)
+    _say(f"smallest shot {nx}x{nz}: bit for bit the oracle's RTM; rel-L2 vs the FP64 split-form RTM {e64:.2e}")
+    assert e64 < 5e-3          # FP32 rounding over 3000 steps and the unsplit reformulation
