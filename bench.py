@@ -168,7 +168,7 @@ def cpu_legs_3d(po, vp, eps, dl, n, npml, nt_total, gpu_rec, budget_s=8.0):
         sim.close()
         log(f"[bench] CPU 3D {prec}: {legs[-2]['gcells']} Gcell/s on {allc} threads, "
             f"{legs[-1]['gcells']} on 1")
-    k = min(gpu_rec.shape[1], recs["f32"].shape[1])
+    k = min(gpu_rec.shape[1], recs["f32"].shape[1], recs["f64"].shape[1])
     errs = {"steps_compared": k,
             "gpu_vs_cpu_f32": rel(gpu_rec[:, :k], recs["f32"][:, :k]),
             "gpu_vs_cpu_f64": rel(gpu_rec[:, :k], recs["f64"][:, :k]),
@@ -634,7 +634,7 @@ def main():
                          "instead of the unsplit interior / reduced split")
     ap.add_argument("--exact-frac", type=float, default=0.25,
                     help="learned RTM, latent codec: fraction of snapshots the exact-storage fallback may keep")
-    ap.add_argument("--codec", default="sparse", choices=["sparse", "quant", "latent"],
+    ap.add_argument("--codec", default="quant", choices=["sparse", "quant", "latent"],
                     help="learned RTM: error-bounded block coefficients (sparse: the largest at 15 significant "
                          "bits; quant: all quantized with a per-block step) or autoencoder latents")
     ap.add_argument("--tol", type=float, default=0.0007,

@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvti.so")
-SOURCES = ["vti_kernels.hip", "vti_api.hip", "vti_tables.cpp", "vti_dropin.cpp", "vti_rtm.hip", "vti_grad.hip", "vti_batch.hip", "vti_sweep.cpp", "vti_nn.hip", "vti_ae.hip", "vti_bcodec.hip", "vti_bqcodec.hip", "vti_rtm_learned.hip", "vti_adcig.hip", "vti_segy.cpp"]
+SOURCES = ["vti_kernels.hip", "vti_api.hip", "vti_tables.cpp", "vti_dropin.cpp", "vti_rtm.hip", "vti_grad.hip", "vti_batch.hip", "vti_sweep.cpp", "vti_nn.hip", "vti_ae.hip", "vti_bcodec.hip", "vti_bqcodec.hip", "vti_rtm_learned.hip", "vti_rtm_eb.hip", "vti_adcig.hip", "vti_segy.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -ffp-contract=off: no multiply-add fusion, so every step is the reference's

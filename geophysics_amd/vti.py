@@ -120,6 +120,8 @@ def _declare(L):
         "vti_rtm_defaults": (C.c_int, [C.POINTER(RtmOpts), C.c_int]),
         "vti_rtm_reset": (C.c_int, [VP]),
         "vti_rtm_shot": (C.c_int, [VP, C.POINTER(RtmOpts), FP, C.POINTER(C.c_longlong)]),
+        "vti_rtm_shot_eb": (C.c_int, [VP, C.POINTER(RtmOpts), FP, DP]),
+        "vti_rtm_eb_bytes": (C.c_longlong, [VP]),
         "vti_rtm_image": (C.c_int, [VP, C.c_int, FP, FP]),
         "vti_mute3d_window": (C.c_int, [VP] + [C.c_int] * 7),
         "vti_batch_create": (C.c_int, [C.c_int, C.POINTER(Cfg), C.POINTER(Shot), C.POINTER(FP),
@@ -413,6 +415,24 @@ class Propagator:
             p = None
         check(lib().vti_rtm_shot(self.h, C.byref(o), p, C.byref(steps)), "rtm_shot")
         return steps.value
+
+    def rtm_shot_eb(self, opts=None, obs=None):
+        """Migrate the current shot with the source field rebuilt backward in
+        time from saved 4-cell boundary bands (vti_rtm_shot_eb); returns
+        {"band_bytes", "source_steps"}."""
+        o = opts if opts is not None else self.rtm_defaults()
+        p = None
+        if obs is not None:
+            a, p = _f32(obs)
+            if a.shape != (self.nrec, self.cfg.nt):
+                raise VtiError(f"obs must be shaped ({self.nrec}, {self.cfg.nt})")
+            self._obs_keep = a
+        st = np.zeros(2, np.float64)
+        check(lib().vti_rtm_shot_eb(self.h, C.byref(o), p, st.ctypes.data_as(DP)), "rtm_shot_eb")
+        return {"band_bytes": st[0], "source_steps": int(st[1])}
+
+    def rtm_eb_bytes(self):
+        return int(lib().vti_rtm_eb_bytes(self.h))
 
     def rtm_image(self, finalize=True):
         shape = (self.cfg.ny, self.cfg.nx, self.cfg.nz) if self.cfg.ndim == 3 else (self.cfg.nx, self.cfg.nz)

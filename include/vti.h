@@ -573,6 +573,26 @@ int vti_sync(vti_ctx *ctx);
 const char *vti_strerror(int code);
 const char *vti_last_error(void);
 
+/* RTM of the current shot with the source wavefield rebuilt backward in
+ * time instead of recomputed from checkpoints (effective-boundary
+ * reconstruction; the reference's counterpart, wavefield_bndr
+ * Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:532-612 with :1394-1437, 3D
+ * Toa_gpu_3dvti_rtm_adcigs_1orderfunction.cu:405-506, saves a one-cell ring
+ * and re-runs the forward operator, which is not a reversal).  The forward
+ * sweep is vti_rtm_shot's (record, illumination, mute / obs) and saves after
+ * every step the interior's outer 4-cell bands (x faces: u, P; y faces: v,
+ * P; z faces: w, Q, P; vti_rtm_eb_bytes(ctx) bytes for the shot); the
+ * reverse sweep steps the receiver side as vti_rtm_shot does and the
+ * source side backward on the interior less the bands, with the saved bands
+ * in place: two source-side propagations instead of the checkpointed
+ * recomputation.  The rebuilt field differs from the forward one by
+ * accumulated rounding.  Point-source (GPU) form only; opts.nsnap unused;
+ * no angle gathers.  stats (optional, 2 doubles): band bytes, source-side
+ * steps.  Leaves the main state's split fields stale (vti_reset before
+ * split-form steps).  No counterpart in the reference beyond the above. */
+int vti_rtm_shot_eb(vti_ctx *ctx, const vti_rtm_opts *opts, const float *obs, double *stats);
+long long vti_rtm_eb_bytes(const vti_ctx *ctx);
+
 /* ---- shot records to SU / SEG-Y files (host) ----------------------------- */
 /* vti_bin2su: utils-binary2segy/binary2segy.c:43-85 — each of ntrace output
  * traces is the 240-byte header of the template SU file's trace (kept as it
