@@ -22,8 +22,9 @@ carries:
     between them;
   * learned_rtm: the second half of the metric, 3D RTM per-shot time with
     learned snapshot compression against plain checkpoint recomputation at
-    equal memory (384^3, 4000 steps, 3 fitting shots, one shot imaged both
-    ways), with the final-image error.
+    equal memory (384^3, 4000 steps, 3 fitting shots, two shots imaged both
+    ways), with the final-image error, and the effective-boundary RTM (band
+    reversal) of the same shots.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--no-cpu] [--no-learned]
 """
@@ -482,7 +483,10 @@ def learned_rtm(args, torch, vti, nshot, nt, n, rows_out=None):
     "sparse": error-bounded sparse coefficients (every snapshot within
     relative L2 error --tol); codec "quant": the same bound with every
     coefficient quantized per block; codec "latent": the autoencoder's 64-channel
-    latents with the exact-storage fallback (--exact-frac).  Returns the
+    latents with the exact-storage fallback (--exact-frac).  Beside them
+    (unless --no-eb) the effective-boundary RTM of the same shot: the source
+    field rebuilt backward in time from 4-cell interior bands saved every
+    step, timed and compared with the plain image.  Returns the
     result record; each imaged shot is logged (and appended to rows_out)
     as it finishes."""
     npml = args.npml
@@ -553,6 +557,15 @@ def learned_rtm(args, torch, vti, nshot, nt, n, rows_out=None):
                    err=err, err_raw=err_raw, fsteps=fsteps, slots=nsnap_equal, store=store, held=held,
                    enc_ms=st["encode_ms"], dec_ms=st["decode_ms"], snap_err=st["mean_checked_err"],
                    exact=st["exact"], recomputed=st["recomputed"])
+        if not args.no_eb:   # the third source-state option: reversal from saved boundary bands
+            prop.rtm_reset()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            se = prop.rtm_shot_eb(o)
+            prop.sync()
+            t_eb = time.perf_counter() - t0
+            fin_e, _ = prop.rtm_image(finalize=True)
+            row.update(t_eb=round(t_eb, 3), err_eb=rel(fin_e, fin_p), band=se["band_bytes"])
         rows.append(row)
         log("[bench] learned RTM row " + json.dumps(row))
         if rows_out:
@@ -586,6 +599,12 @@ def learned_rtm(args, torch, vti, nshot, nt, n, rows_out=None):
         "compression_vs_fp32_snapshots": round(fp32_bytes / mean("store"), 3) if rows else None,
         "fit_s": round(t_fit, 2), "fit_energy_captured_64": explained, "snapshots": nsave,
         "exact_snapshots_mean": mean("exact"), "recomputed_snapshots_mean": mean("recomputed"),
+        "effective_boundary": {
+            "s_per_shot": round(mean("t_eb"), 3), "speedup_vs_plain": round(tp / mean("t_eb"), 3),
+            "final_image_rel_l2_max": max(r["err_eb"] for r in rows), "band_GB": round(mean("band") / 1e9, 3),
+            "note": "source field rebuilt backward in time from 4-cell interior bands saved every step "
+                    "(vti_rtm_shot_eb): no recomputation, the bands' memory instead of equal memory"}
+        if rows and "t_eb" in rows[0] else None,
         "encode_gpu_ms_per_shot": round(mean("enc_ms"), 1) if rows else None,
         "decode_gpu_ms_per_shot": round(mean("dec_ms"), 1) if rows else None,
         "rows": rows,
@@ -651,8 +670,10 @@ def main():
                     help="learned RTM: append each imaged shot's result to this file as it finishes")
     ap.add_argument("--no-learned", action="store_true",
                     help="fd3d: leave out the learned-vs-plain 3D RTM record (384^3, 4000 steps, one imaged shot)")
-    ap.add_argument("--learned-shots", type=int, default=4,
+    ap.add_argument("--learned-shots", type=int, default=5,
                     help="fd3d: shots of the learned-vs-plain record (3 fitting, the rest imaged both ways)")
+    ap.add_argument("--no-eb", action="store_true",
+                    help="learned RTM: leave out the effective-boundary (band reversal) leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the library has no multi-GPU path "

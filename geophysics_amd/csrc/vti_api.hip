@@ -131,6 +131,8 @@ VelArgs vel_args_for(vti_ctx *c, const VtiState &s, int it, bool inject)
     a.dtx = dt / c->cfg.dx; a.dty = dt / c->cfg.dy; a.dtz = dt / c->cfg.dz;
     a.src = -1;
     a.amp = 0.0f;
+    a.cth = c->tti;
+    a.sth = c->tti ? c->tti + c->nvol : nullptr;
     if (inject && c->cfg.mode == VTI_MODE_GPU_REF && c->amp[it] != 0.0f) {
         a.src = c->shot.sz + (long long)c->shot.sx * c->g.ldz + (long long)c->shot.sy * c->g.ldxy;
         a.amp = c->amp[it];
@@ -153,6 +155,8 @@ StressArgs stress_args_for(vti_ctx *c, const VtiState &s, int it, bool inject, b
     a.az1 = c->tab + 10 * M; a.az2 = c->tab + 11 * M;
     const float dt = c->cfg.dt;
     a.dtx = dt / c->cfg.dx; a.dty = dt / c->cfg.dy; a.dtz = dt / c->cfg.dz;
+    a.cth = c->tti;
+    a.sth = c->tti ? c->tti + c->nvol : nullptr;
     a.sv = sv ? 1 : 0;
     a.svR = c->cfg.sv_R;
     a.cx = c->shot.sv_cx; a.cy = c->shot.sv_cy; a.cz = c->shot.sv_cz;
@@ -396,6 +400,7 @@ int vti_destroy(vti_ctx *c)
     hipFree(c->illum);
     hipFree(c->adcig);
     hipFree(c->adcig_tab);
+    hipFree(c->tti);
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
     delete c;
     return VTI_OK;
@@ -737,7 +742,46 @@ int vti_set_unsplit(vti_ctx *c, int on)
     if (!c) return fail(VTI_ERR_ARG, "vti_set_unsplit: null ctx");
     if (on && c->cfg.mode != VTI_MODE_GPU_REF)
         return fail(VTI_ERR_ARG, "vti_set_unsplit: point-source (GPU) form only");
+    if (on && c->tti) return fail(VTI_ERR_STATE, "vti_set_unsplit: the TTI update is split only");
     c->unsplit = on != 0;
+    return VTI_OK;
+}
+
+// 2D TTI (2D-tti-fd+rtm-cuda/Toa_gpu_2dtti_fd_1orderfunction.cu): the tilt
+// of the interior (x-major, z fastest), in radians or, with degrees, as the
+// reference reads it (theta *= pi/180.0 in double, read_file :235); padded
+// by edge replication as the model (pad_vv :190-215), cosf / sinf of the
+// float angle per cell into two pitched device volumes.  theta NULL: back
+// to the VTI update.
+int vti_set_theta(vti_ctx *c, const float *theta, int degrees)
+{
+    if (!c) return fail(VTI_ERR_ARG, "vti_set_theta: null ctx");
+    if (!theta) {
+        (void)hipFree(c->tti);
+        c->tti = nullptr;
+        return VTI_OK;
+    }
+    const vti_cfg &k = c->cfg;
+    if (k.ndim != 2 || k.mode != VTI_MODE_GPU_REF)
+        return fail(VTI_ERR_ARG, "vti_set_theta: 2D point-source (GPU) form only");
+    if (c->unsplit) return fail(VTI_ERR_STATE, "vti_set_theta: the TTI update is split only (vti_set_unsplit(0))");
+    const int np = k.npml, nnx = c->g.nnx, nnz = c->g.nnz, ldz = c->g.ldz;
+    std::vector<float> h(2 * c->nvol, 0.0f);
+    for (int ix = 0; ix < nnx; ++ix)
+        for (int iz = 0; iz < nnz; ++iz) {
+            const int jx = ix < np ? 0 : (ix >= np + k.nx ? k.nx - 1 : ix - np);
+            const int jz = iz < np ? 0 : (iz >= np + k.nz ? k.nz - 1 : iz - np);
+            float th = theta[(size_t)jz + (size_t)jx * k.nz];
+            if (degrees) th = (float)((double)th * (kPi / 180.0));
+            const size_t id = (size_t)iz + (size_t)ix * ldz;
+            h[id] = std::cos(th);
+            h[c->nvol + id] = std::sin(th);
+        }
+    if (!c->tti && hipMalloc(&c->tti, 2 * c->nvol * sizeof(float)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(VTI_ERR_ALLOC, "vti_set_theta: tilt tables");
+    }
+    HIP_TRY(hipMemcpy(c->tti, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
     return VTI_OK;
 }
 

@@ -229,6 +229,71 @@ int run_omp2d(int nx, int nz, float dx, float dz, int npd, float favg, float pfa
     return st;
 }
 
+// main of 2D-tti-fd+rtm-cuda/Toa_gpu_2dtti_fd_1orderfunction.cu:313-499, its
+// hard-coded values (:329-359) as arguments: the four models read x-major,
+// z fastest, the tilt in degrees (read_file :217-242); one shot; the whole
+// padded P every 100 steps after step 0, frames appended (:448-454); the P
+// record of row npml, [ix][it] (shot_record :283, :457-459).
+int run_tti2d(int nx, int nz, float dx, float dz, int npml, float favg, float pfac, int fs, int zs, int nt,
+              float dt, const char *FNv, const char *FNe, const char *FNd, const char *FNth, const char *FNshot,
+              const char *FNsnap)
+{
+    vti_cfg cfg{};
+    cfg.ndim = 2;
+    cfg.nx = nx; cfg.ny = 1; cfg.nz = nz; cfg.npml = npml;
+    cfg.dx = dx; cfg.dy = dx; cfg.dz = dz;
+    cfg.dt = dt; cfg.nt = nt;
+    cfg.favg = favg; cfg.pfac = pfac;
+    cfg.wtype = 1;                                               // :329
+    cfg.mode = VTI_MODE_GPU_REF;
+    cfg.sv = 0;                                                  // update_stress(..., false) :444
+    cfg.rec_kind = VTI_REC_P;
+    const size_t n = (size_t)nx * nz;
+    std::vector<float> v, e, d, th;
+    int st;
+    if ((st = read_model(FNv, v, n)) || (st = read_model(FNe, e, n)) || (st = read_model(FNd, d, n)) ||
+        (st = read_model(FNth, th, n)))
+        return st;
+    Ctx x;
+    if ((st = vti_create(&cfg, &x.c))) return st;
+    if ((st = vti_set_model(x.c, v.data(), e.data(), d.data(), 0)) || (st = vti_set_theta(x.c, th.data(), 1)))
+        return st;
+    vti_shot sh{};
+    sh.sx = (int)((float)fs + 0.5) + npml - 1;                   // add_source :74
+    sh.sz = (int)((float)zs + 0.5) + npml - 1;                   // :75
+    sh.sv_cx = fs + npml; sh.sv_cz = zs + npml;                  // :136-137
+    sh.rec_z = npml;                                             // :283
+    sh.rec_x0 = npml; sh.rec_dx = 1; sh.rec_nx = nx;
+    sh.rec_dy = 1; sh.rec_ny = 1;
+    if ((st = vti_set_shot(x.c, &sh)) || (st = vti_reset(x.c))) return st;
+    FILE *fshot = std::fopen(FNshot, "wb");
+    FILE *fsnap = std::fopen(FNsnap, "wb");
+    if (!fshot || !fsnap) {
+        if (fshot) std::fclose(fshot);
+        if (fsnap) std::fclose(fsnap);
+        vti::set_error("cannot open output files");
+        return VTI_ERR_IO;
+    }
+    int nnx, nny, nnz;
+    vti_padded_dims(x.c, &nnx, &nny, &nnz);
+    std::vector<float> full((size_t)nnx * nnz), shot((size_t)nx * nt);
+    for (int it = 0; it < nt && st == VTI_OK;) {
+        int last = std::max(100, (it + 99) / 100 * 100);         // next frame step >= it (not step 0)
+        if (last > nt - 1) last = nt - 1;
+        if ((st = vti_step(x.c, it, last - it + 1))) break;
+        it = last + 1;
+        if (last % 100 == 0) {
+            if ((st = vti_get_field(x.c, VTI_F_P, full.data()))) break;
+            std::fwrite(full.data(), 4, full.size(), fsnap);
+        }
+    }
+    if (st == VTI_OK && (st = vti_get_record(x.c, shot.data())) == VTI_OK)
+        std::fwrite(shot.data(), sizeof(float), shot.size(), fshot);
+    std::fclose(fshot);
+    std::fclose(fsnap);
+    return st;
+}
+
 }  // namespace
 
 extern "C" {
@@ -266,6 +331,16 @@ int vti_omp2dfd(int nx, int nz, float dx, float dz, int npd, float favg, float p
                 const char FNsnap[])
 {
     const int st = run_omp2d(nx, nz, dx, dz, npd, favg, pfac, sx, sz, nt, dt, FNv, FNe, FNd, FNshot, FNsnap);
+    g_status = st;
+    return st;
+}
+
+int vti_tti2dfd(int nx, int nz, float dx, float dz, int npml, float favg, float pfac, int fs, int zs, int nt,
+                float dt, const char FNv[], const char FNe[], const char FNd[], const char FNtheta[],
+                const char FNshot[], const char FNsnap[])
+{
+    const int st = run_tti2d(nx, nz, dx, dz, npml, favg, pfac, fs, zs, nt, dt, FNv, FNe, FNd, FNtheta, FNshot,
+                             FNsnap);
     g_status = st;
     return st;
 }

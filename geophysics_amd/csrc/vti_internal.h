@@ -32,6 +32,9 @@ struct VelArgs {
     float dtx, dty, dtz;
     long long src;   // cell that receives the point source before the update (-1: none)
     float amp;       // pfac * wavelet(t) of this step
+    // 2D TTI (Toa_gpu_2dtti_fd_1orderfunction.cu:81-114): cos and sin of the
+    // per-cell tilt (padded, pitched); nullptr: the VTI update
+    const float *cth, *sth;
 };
 
 // Stress update: update_stress2d GPU_vti2dfd_kernels.cu:240-299,
@@ -69,6 +72,8 @@ struct StressArgs {
     // the interior is inx x iny x inz cells from padded index inp (y only in 3D)
     float *illum;
     int illum_clamp, inp, inx, iny, inz;
+    // 2D TTI (Toa_gpu_2dtti_fd_1orderfunction.cu:116-183): as VelArgs
+    const float *cth, *sth;
 };
 
 // One fused 3D step (vti_fused3d.h): velocity update into new u, v, w from
@@ -174,6 +179,9 @@ struct vti_ctx {
     float *adcig = nullptr, *adcig_tab = nullptr;
     int adcig_na = 0, adcig_da = 1, adcig_dcdp = 1;
     size_t adcig_n = 0;
+    // 2D TTI (vti_set_theta): cos and sin of the padded tilt, two pitched
+    // volumes; nullptr: VTI
+    float *tti = nullptr;
 };
 
 namespace vti {

@@ -145,6 +145,97 @@ __global__ __launch_bounds__(256) void stress2d_kernel(StressArgs a)
     }
 }
 
+// ------------------------------------------------------------ 2D TTI ----
+// Toa_gpu_2dtti_fd_1orderfunction.cu: the x and z derivatives of P and Q
+// (velocity) and of u and w (stress) rotated by the tilt theta, every
+// field damped by the product of its x and z factors; cos/sin of theta come
+// from host-built tables (a.cth, a.sth).  Expressions in the reference's
+// left-to-right order:
+//   u = cx2*cz2*u - cx1*cz1*(cos*dtx*Px - sin*dtz*Pz)            (:110)
+//   w = cx2*cz2*w - cx1*cz1*(sin*dtx*Qx + cos*dtz*Qz)            (:111)
+//   px = ax2*az2*px - ax1*az1*vp*vp*(1+2e)*(cos*dtx*ux - sin*dtz*uz)  (:170-171)
+//   pz = ax2*az2*pz - ax1*az1*vp*vp*sqrtf(1+2d)*(sin*dtx*wx + cos*dtz*wz)
+//   qx = ax2*az2*qx - ax1*az1*vp*vp*sqrtf(1+2d)*(cos*dtx*ux - sin*dtz*uz)
+//   qz = ax2*az2*qz - ax1*az1*vp*vp*(sin*dtx*wx + cos*dtz*wz)    (:176-177)
+// The point source (add_source before update_vel, :441) is seen by all four
+// stencils of the velocity update as P + amp, Q + amp at its cell.
+template <bool INJ>
+__global__ __launch_bounds__(256) void tti_vel2d_kernel(VelArgs a)
+{
+    const Grid g = a.g;
+    const int iz = blockIdx.x * TZ + threadIdx.x;
+    const int ix = blockIdx.y * TX + threadIdx.y;
+    if (iz < 4 || iz >= g.nnz - 4 || ix < 4 || ix >= g.nnx - 4) return;
+    const long long id = iz + (long long)ix * g.ldz;
+    float Px = 0.0f, Pz = 0.0f, Qz = 0.0f, Qx = 0.0f;
+#pragma unroll
+    for (int im = 0; im < 4; ++im) {
+        const long long i1 = id + (long long)(im + 1) * g.ldz, i0 = id - (long long)im * g.ldz;
+        const long long j1 = id + im + 1, j0 = id - im;
+        float pi1 = a.P[i1], pi0 = a.P[i0], pj1 = a.P[j1], pj0 = a.P[j0];
+        float qi1 = a.Q[i1], qi0 = a.Q[i0], qj1 = a.Q[j1], qj0 = a.Q[j0];
+        if (INJ) {
+            if (i1 == a.src) { pi1 = pi1 + a.amp; qi1 = qi1 + a.amp; }
+            if (i0 == a.src) { pi0 = pi0 + a.amp; qi0 = qi0 + a.amp; }
+            if (j1 == a.src) { pj1 = pj1 + a.amp; qj1 = qj1 + a.amp; }
+            if (j0 == a.src) { pj0 = pj0 + a.amp; qj0 = qj0 + a.amp; }
+        }
+        Px += cst(im) * (pi1 - pi0);
+        Pz += cst(im) * (pj1 - pj0);
+        Qz += cst(im) * (qj1 - qj0);
+        Qx += cst(im) * (qi1 - qi0);
+    }
+    const float cs = a.cth[id], sn = a.sth[id];
+    const float g2 = a.cx2[ix] * a.cz2[iz], g1 = a.cx1[ix] * a.cz1[iz];
+    a.u[id] = g2 * a.u[id] - g1 * (cs * a.dtx * Px - sn * a.dtz * Pz);
+    a.w[id] = g2 * a.w[id] - g1 * (sn * a.dtx * Qx + cs * a.dtz * Qz);
+}
+
+__global__ __launch_bounds__(256) void tti_stress2d_kernel(StressArgs a)
+{
+    const Grid g = a.g;
+    const int iz = blockIdx.x * TZ + threadIdx.x;
+    const int ix = blockIdx.y * TX + threadIdx.y;
+    if (iz < 4 || iz >= g.nnz - 4 || ix < 4 || ix >= g.nnx - 4) return;
+    const long long id = iz + (long long)ix * g.ldz;
+    float ux = 0.0f, uz = 0.0f, wz = 0.0f, wx = 0.0f;
+#pragma unroll
+    for (int im = 0; im < 4; ++im) {
+        ux += cst(im) * (a.u[id + (long long)im * g.ldz] - a.u[id - (long long)(im + 1) * g.ldz]);
+        uz += cst(im) * (a.u[id + im] - a.u[id - im - 1]);
+        wz += cst(im) * (a.w[id + im] - a.w[id - im - 1]);
+        wx += cst(im) * (a.w[id + (long long)im * g.ldz] - a.w[id - (long long)(im + 1) * g.ldz]);
+    }
+    const float vv = a.vp[id];
+    float ee = a.ep[id], dd = a.de[id];
+    if (a.sv) {
+        const int rx = ix - a.cx, rz = iz - a.cz;
+        taper(a, rx * rx + rz * rz, ee, dd);
+    }
+    const float cs = a.cth[id], sn = a.sth[id];
+    const float g2 = a.ax2[ix] * a.az2[iz], g1 = a.ax1[ix] * a.az1[iz];
+    const float e2 = 1.0f + 2.0f * ee;
+    const float sd = sqrtf(1.0f + 2.0f * dd);
+    const float du = cs * a.dtx * ux - sn * a.dtz * uz;   // the rotated x derivative of u
+    const float dw = sn * a.dtx * wx + cs * a.dtz * wz;   // the rotated z derivative of w
+    const float px = g2 * a.px[id] - g1 * vv * vv * e2 * du;
+    const float pz = g2 * a.pz[id] - g1 * vv * vv * sd * dw;
+    const float qx = g2 * a.qx[id] - g1 * vv * vv * sd * du;
+    const float qz = g2 * a.qz[id] - g1 * vv * vv * dw;
+    a.px[id] = px; a.pz[id] = pz; a.qx[id] = qx; a.qz[id] = qz;
+    const float Pn = px + pz, Qn = qx + qz;
+    a.P[id] = Pn;
+    a.Q[id] = Qn;
+    illum_add(a, ix, a.inp, iz, Pn, Qn);
+    if (a.rec && iz == a.rz) {
+        const int k = ix - a.rx0;
+        if (k >= 0 && k % a.rdx == 0 && k / a.rdx < a.rnx) {
+            const float r = a.rec_kind == 0 ? Pn : (a.rec_kind == 1 ? Pn + Pn : Pn + Qn);
+            a.rec[k / a.rdx] = r;
+        }
+    }
+}
+
 // ------------------------------------------------------------ helpers ----
 // Edge-replicating pad of an unpadded (ny, nx, nz) model into the padded
 // pitched volume: equal to pad_vv2d / pad_vv3d (GPU_vti2dfd_kernels.cu:49-74,
@@ -258,7 +349,10 @@ hipError_t launch_vel(int ndim, const VelArgs &a, hipStream_t s)
     const bool inj = a.src >= 0;
     if (ndim == 2) {
         dim3 blk(TZ, TX), grd((a.g.nnz + TZ - 1) / TZ, (a.g.nnx + TX - 1) / TX);
-        if (inj) hipLaunchKernelGGL(vel2d_kernel<true>, grd, blk, 0, s, a);
+        if (a.cth) {
+            if (inj) hipLaunchKernelGGL(tti_vel2d_kernel<true>, grd, blk, 0, s, a);
+            else hipLaunchKernelGGL(tti_vel2d_kernel<false>, grd, blk, 0, s, a);
+        } else if (inj) hipLaunchKernelGGL(vel2d_kernel<true>, grd, blk, 0, s, a);
         else hipLaunchKernelGGL(vel2d_kernel<false>, grd, blk, 0, s, a);
     } else {
         int nzt, nt, ntp;
@@ -284,7 +378,8 @@ hipError_t launch_stress(int ndim, const StressArgs &a, hipStream_t s)
 {
     if (ndim == 2) {
         dim3 blk(TZ, TX), grd((a.g.nnz + TZ - 1) / TZ, (a.g.nnx + TX - 1) / TX);
-        if (a.dpq) hipLaunchKernelGGL((stress2d_kernel<false, true>), grd, blk, 0, s, a);
+        if (a.cth) hipLaunchKernelGGL(tti_stress2d_kernel, grd, blk, 0, s, a);
+        else if (a.dpq) hipLaunchKernelGGL((stress2d_kernel<false, true>), grd, blk, 0, s, a);
         else if (a.unsplit && !a.gsrc) hipLaunchKernelGGL((stress2d_kernel<true, false>), grd, blk, 0, s, a);
         else hipLaunchKernelGGL((stress2d_kernel<false, false>), grd, blk, 0, s, a);
     } else {

@@ -164,6 +164,7 @@ int vti_rtm_shot_eb(vti_ctx *c, const vti_rtm_opts *o, const float *obs, double 
     if (k.mode != VTI_MODE_GPU_REF)
         return fail(VTI_ERR_ARG, "vti_rtm_shot_eb: the point-source (GPU) form only (the in-stress source "
                                  "forms are not reversed)");
+    if (c->tti) return fail(VTI_ERR_ARG, "vti_rtm_shot_eb: the VTI update only (the reversal runs the unsplit form)");
     if (k.nx < 2 * kBand + 1 || k.nz < 2 * kBand + 1 || (k.ndim == 3 && k.ny < 2 * kBand + 1))
         return fail(VTI_ERR_ARG, "vti_rtm_shot_eb: interior narrower than two 4-cell bands + 1");
     if (c->adcig_na > 0)

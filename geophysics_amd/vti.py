@@ -121,6 +121,7 @@ def _declare(L):
         "vti_rtm_reset": (C.c_int, [VP]),
         "vti_rtm_shot": (C.c_int, [VP, C.POINTER(RtmOpts), FP, C.POINTER(C.c_longlong)]),
         "vti_rtm_shot_eb": (C.c_int, [VP, C.POINTER(RtmOpts), FP, DP]),
+        "vti_set_theta": (C.c_int, [VP, FP, C.c_int]),
         "vti_rtm_eb_bytes": (C.c_longlong, [VP]),
         "vti_rtm_image": (C.c_int, [VP, C.c_int, FP, FP]),
         "vti_mute3d_window": (C.c_int, [VP] + [C.c_int] * 7),
@@ -195,6 +196,9 @@ def _declare(L):
     L.vti_omp2dfd.restype = C.c_int
     L.vti_omp2dfd.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_float, C.c_float,
                               C.c_int, C.c_int, C.c_int, C.c_float] + [C.c_char_p] * 5
+    L.vti_tti2dfd.restype = C.c_int
+    L.vti_tti2dfd.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_float, C.c_float,
+                              C.c_int, C.c_int, C.c_int, C.c_float] + [C.c_char_p] * 6
     L.vti_bin2su.restype = C.c_int
     L.vti_bin2su.argtypes = [C.c_char_p] * 3 + [C.c_longlong, C.c_int, C.c_int]
     L.vti_su_write.restype = C.c_int
@@ -416,6 +420,17 @@ class Propagator:
         check(lib().vti_rtm_shot(self.h, C.byref(o), p, C.byref(steps)), "rtm_shot")
         return steps.value
 
+    def set_theta(self, theta, degrees=False):
+        """2D TTI tilt of the interior ((nx, nz), radians unless degrees);
+        None: back to VTI."""
+        if theta is None:
+            check(lib().vti_set_theta(self.h, None, 0), "set_theta")
+            return
+        a, p = _f32(theta)
+        if a.shape != (self.cfg.nx, self.cfg.nz):
+            raise VtiError(f"theta must be shaped ({self.cfg.nx}, {self.cfg.nz})")
+        check(lib().vti_set_theta(self.h, p, 1 if degrees else 0), "set_theta")
+
     def rtm_shot_eb(self, opts=None, obs=None):
         """Migrate the current shot with the source field rebuilt backward in
         time from saved 4-cell boundary bands (vti_rtm_shot_eb); returns
@@ -605,6 +620,14 @@ def omp2dfd(FNv, FNe, FNd, FNshot, FNsnap, nx=601, nz=301, dx=5.0, dz=5.0, npd=0
     as defaults (files in, shot / snapshot files out)."""
     check(lib().vti_omp2dfd(nx, nz, dx, dz, npd, favg, pfac, sx, sz, nt, dt, _b(FNv), _b(FNe), _b(FNd),
                             _b(FNshot), _b(FNsnap)), "vti_omp2dfd")
+
+
+def tti2dfd(FNv, FNe, FNd, FNtheta, FNshot, FNsnap, nx=600, nz=300, dx=5.0, dz=5.0, npml=50, favg=40.0,
+            pfac=10.0, fs=200, zs=140, nt=801, dt=0.0005):
+    """main of 2D-tti-fd+rtm-cuda/Toa_gpu_2dtti_fd_1orderfunction.cu:313-499
+    with its hard-coded values as defaults (files in, shot / snapshot files out)."""
+    check(lib().vti_tti2dfd(nx, nz, dx, dz, npml, favg, pfac, fs, zs, nt, dt, _b(FNv), _b(FNe), _b(FNd),
+                            _b(FNtheta), _b(FNshot), _b(FNsnap)), "vti_tti2dfd")
 
 
 def bin2su(su_template, raw, out, ntrace, ns, swap=True):

@@ -593,6 +593,30 @@ const char *vti_last_error(void);
 int vti_rtm_shot_eb(vti_ctx *ctx, const vti_rtm_opts *opts, const float *obs, double *stats);
 long long vti_rtm_eb_bytes(const vti_ctx *ctx);
 
+/* ---- 2D TTI ----------------------------------------------------------- */
+/* 2D-tti-fd+rtm-cuda/Toa_gpu_2dtti_fd_1orderfunction.cu: the GPU form with a
+ * per-cell tilt theta.  The velocity update takes the x and z derivatives of
+ * P (u) and Q (w) rotated by theta, the stress update those of u and w
+ * (update_vel :81-114, update_stress :116-183), and every field is damped by
+ * the product of its x and z factors.  vti_set_theta sets the tilt of the
+ * interior (nx * nz, x-major, z fastest; radians, or degrees converted as
+ * the reference reads them, theta * (pi/180.0) in double), padded by edge
+ * replication; cos and sin of the float angle per cell are host-built
+ * tables, so every later step, record and RTM of the context runs the TTI
+ * update (theta NULL: back to VTI).  2D point-source (GPU) form, split
+ * update only (vti_set_unsplit(0); VTI_ERR_STATE otherwise); not with
+ * vti_rtm_shot_eb. */
+int vti_set_theta(vti_ctx *ctx, const float *theta, int degrees);
+/* main of Toa_gpu_2dtti_fd_1orderfunction.cu:313-499 as a function, its
+ * hard-coded values (:329-359: 600 x 300, 5 m, npml 50, 40 Hz, pfac 10,
+ * source (200, 140), 801 steps of 0.5 ms) as arguments: vp, eps, delta and
+ * the tilt (degrees) read x-major, z fastest; the whole padded P every 100
+ * steps after step 0 appended to FNsnap; the P record of row npml
+ * ([ix][it]) to FNshot.  Returns the status (also in vti_dropin_status). */
+int vti_tti2dfd(int nx, int nz, float dx, float dz, int npml, float favg, float pfac, int fs, int zs, int nt,
+                float dt, const char FNv[], const char FNe[], const char FNd[], const char FNtheta[],
+                const char FNshot[], const char FNsnap[]);
+
 /* ---- shot records to SU / SEG-Y files (host) ----------------------------- */
 /* vti_bin2su: utils-binary2segy/binary2segy.c:43-85 — each of ntrace output
  * traces is the 240-byte header of the template SU file's trace (kept as it

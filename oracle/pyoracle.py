@@ -25,7 +25,7 @@ class OrcParams(C.Structure):
             "sx", "sy", "sz", "sv", "sv_kind", "sv_R", "sv_r",
             "sv_cx", "sv_cy", "sv_cz",
             "rec_z", "rec_x0", "rec_dx", "rec_nx", "rec_y0", "rec_dy", "rec_ny",
-            "rec_kind", "omp_form", "unsplit")]
+            "rec_kind", "omp_form", "unsplit", "tti")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -37,6 +37,7 @@ TABLES = {n: 20 + i for i, n in enumerate(
     ["coffx1", "coffx2", "coffy1", "coffy2", "coffz1", "coffz2",
      "acoffx1", "acoffx2", "acoffy1", "acoffy2", "acoffz1", "acoffz2"])}
 SRCAMP, D0 = 40, 41
+FIELDS.update({"cth": 42, "sth": 43})   # TTI: cos / sin of the padded tilt
 
 _lib = None
 
@@ -71,6 +72,10 @@ def lib():
         L.orc_params_gpu3d.argtypes = [pp] + [C.c_int] * 20
         L.orc_params_omp2d.argtypes = [pp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_float,
                                        C.c_float, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int]
+        L.orc_params_tti2d.argtypes = [pp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_float,
+                                       C.c_float, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int]
+        for sfx in ("_f32", "_f64"):
+            getattr(L, "orc_set_theta" + sfx).argtypes = [C.c_void_p, fp]
         L.orc_mute2d.argtypes = [C.c_int, C.c_int] + [C.c_float] * 4 + [C.c_int] * 4 + \
             [C.c_float, C.c_float, fp, C.c_int, C.c_int, C.c_int]
         L.orc_mute3d.argtypes = [C.c_int] * 3 + [C.c_float] * 5 + [C.c_int] * 6 + \
@@ -130,6 +135,15 @@ def params_omp2d(nx, nz, dx, dz, npd, favg, pfac, sx, sz, nt, dt, rec_depth=1):
     return p
 
 
+def params_tti2d(nx, nz, dx, dz, npml, favg, pfac, fs, zs, nt, dt, SV=0, rec_depth=1):
+    """main of Toa_gpu_2dtti_fd_1orderfunction.cu (reference values: 600,
+    300, 5.0, 5.0, 50, 40, 10, 200, 140, 801, 0.0005, SV off); the tilt is
+    set on the Sim with set_theta."""
+    p = OrcParams()
+    lib().orc_params_tti2d(C.byref(p), nx, nz, dx, dz, npml, favg, pfac, fs, zs, nt, dt, SV, rec_depth)
+    return p
+
+
 def params_gpu3d(nx, ny, nz, dx, dy, dz, npml, SV, favg, is_, fsx, dsx, fsy, dsy,
                  fsz, dsz, nt, dt_us, rec_depth=1, rec_step=1):
     p = OrcParams()
@@ -183,6 +197,13 @@ class Sim:
         assert a.size == self.nny * self.nnx * self.nnz
         getattr(lib(), "orc_set_" + self.prec)(self.h, FIELDS[name], self._ptr(a))
 
+    def set_theta(self, theta):
+        """TTI tilt of the interior (radians, (nx, nz))."""
+        a = np.ascontiguousarray(theta, dtype=np.float32)
+        assert a.shape == (self.p.nx, self.p.nz)
+        if getattr(lib(), "orc_set_theta_" + self.prec)(self.h, _fptr(a)) != 0:
+            raise ValueError("orc_set_theta: not a TTI simulation")
+
     def table(self, name):
         n = max(self.nnx, self.nny, self.nnz)
         out = np.zeros(n, dtype=self.dtype)
@@ -193,6 +214,11 @@ class Sim:
         a = np.ascontiguousarray(arr, dtype=self.dtype)
         assert a.size == self.table(name).size
         getattr(lib(), "orc_set_" + self.prec)(self.h, TABLES[name], self._ptr(a))
+
+    def set_srcamp(self, amp):
+        a = np.ascontiguousarray(amp, dtype=self.dtype)
+        assert a.size == self.p.nt
+        getattr(lib(), "orc_set_" + self.prec)(self.h, SRCAMP, self._ptr(a))
 
     def srcamp(self):
         out = np.zeros(self.p.nt, dtype=self.dtype)
@@ -279,8 +305,9 @@ def adcig_thin(adcigs, d):
 
 def rtm(params, vp, eps, dl, sv_fwd=1, sv_bwd=0, inject_z=None, illum_mode=0, image_every=1,
         obs=None, mute=None, prec="f32", unsplit_source=False, unsplit_receiver=False,
-        nangle=0, dangle=1, dcdp=1):
+        nangle=0, dangle=1, dcdp=1, theta=None):
     """One-shot RTM with every forward snapshot kept (no checkpointing).
+    theta: the TTI tilt (radians, (nx, nz)) for tti params.
 
     mute: callable(modelled_record) applied in place before it is used as
     data when obs is None.  unsplit_source / unsplit_receiver: the source /
@@ -293,6 +320,9 @@ def rtm(params, vp, eps, dl, sv_fwd=1, sv_bwd=0, inject_z=None, illum_mode=0, im
     pg.unsplit = 1 if unsplit_receiver else 0
     s = Sim(ps, vp, eps, dl, prec)
     g = Sim(pg, vp, eps, dl, prec)
+    if theta is not None:
+        s.set_theta(theta)
+        g.set_theta(theta)
     dt = s.dtype
     shape = (params.ny, params.nx, params.nz) if params.ndim == 3 else (params.nx, params.nz)
     img = np.zeros(shape, dtype=dt)
@@ -303,6 +333,8 @@ def rtm(params, vp, eps, dl, sv_fwd=1, sv_bwd=0, inject_z=None, illum_mode=0, im
         # the mute applies to the modelled record between the passes: run the
         # forward once to get it, then migrate it as observed data
         tmp = Sim(params, vp, eps, dl, prec)
+        if theta is not None:
+            tmp.set_theta(theta)
         rec = tmp.run()
         tmp.close()
         mute(rec)

@@ -22,7 +22,10 @@
  *   3  3D stress update: x stencil offsets of the velocity update
  *   4  3D stress update: py scaled by dt/dx instead of dt/dy
  *   5  point source injected after the velocity update
- *   6  3D stress update: qy scaled by (1+2 eps) instead of sqrt(1+2 delta) */
+ *   6  3D stress update: qy scaled by (1+2 eps) instead of sqrt(1+2 delta)
+ *   7  TTI velocity update: the sign of the sin(theta) term of u flipped
+ *   8  TTI stress update: cos and sin swapped in pz
+ *   9  TTI velocity update: u damped by its x factors only (the VTI form) */
 #ifndef ORC_PLANT
 #define ORC_PLANT 0
 #endif
@@ -94,6 +97,39 @@ int orc_params_gpu2d(orc_params *p, int nx, int nz, int dx, int dz, int npd,
     p->rec_x0 = npd; p->rec_dx = 1; p->rec_nx = nx;
     p->rec_y0 = 0; p->rec_dy = 1; p->rec_ny = 1;
     p->rec_kind = 0;
+    return 0;
+}
+
+/* main of Toa_gpu_2dtti_fd_1orderfunction.cu:313-499 (one shot). */
+int orc_params_tti2d(orc_params *p, int nx, int nz, float dx, float dz, int npml,
+                     float favg, float pfac, int fs, int zs, int nt, float dt,
+                     int SV, int rec_depth)
+{
+    memset(p, 0, sizeof(*p));
+    p->ndim = 2;
+    p->nx = nx; p->ny = 1; p->nz = nz; p->npml = npml;
+    p->dx = dx; p->dy = dx; p->dz = dz;
+    p->dt = dt;
+    p->nt = nt;
+    p->favg = favg;
+    p->pfac = pfac;
+    p->wtype = 1;                                          /* :329 */
+    p->sv = SV;                                            /* :443 (false there) */
+    p->d0_kind = 0;                                        /* get_d0 :185-188 */
+    p->acoff_kind = 0;                                     /* initial_coffe :244-275 */
+    p->src_kind = 0;                                       /* add_source :48-79 */
+    p->t0_kind = 0;                                        /* t = dt :437 */
+    float fsf = (float)fs;                                 /* :58 fs=xsn+(is-1)*ds, is = 1 */
+    p->sx = (int)(fsf + 0.5) + npml - 1;                   /* :74 */
+    p->sz = (int)((float)zs + 0.5) + npml - 1;             /* :75 */
+    p->sv_kind = 0; p->sv_R = 15; p->sv_r = 4;             /* :124 */
+    p->sv_cx = fs + npml;                                  /* :136 */
+    p->sv_cz = zs + npml;                                  /* :137 */
+    p->rec_z = rec_depth + npml - 1;                       /* shot_record :283 (row npml) */
+    p->rec_x0 = npml; p->rec_dx = 1; p->rec_nx = nx;
+    p->rec_y0 = 0; p->rec_dy = 1; p->rec_ny = 1;
+    p->rec_kind = 0;
+    p->tti = 1;
     return 0;
 }
 

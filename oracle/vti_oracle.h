@@ -63,6 +63,10 @@ typedef struct orc_params {
                               are updated directly and the split fields are not
                               kept; the point source is seen by the velocity
                               update only).  0 = the reference's split form. */
+    int tti;               /* 2D TTI (2D-tti-fd+rtm-cuda/Toa_gpu_2dtti_fd_1orderfunction.cu):
+                              the x and z derivatives rotated by the per-cell
+                              tilt (orc_set_theta), every field damped by the
+                              product of its x and z factors; split form only */
 } orc_params;
 
 /* Parameter builders restating the reference drivers' conversions. */
@@ -83,6 +87,15 @@ int orc_params_gpu3d(orc_params *p, int nx, int ny, int nz, int dx, int dy,
                      int dz, int npml, int SV, int favg, int is, int fsx,
                      int dsx, int fsy, int dsy, int fsz, int dsz, int nt,
                      int dt_us, int rec_depth, int rec_step);
+/* main of 2D-tti-fd+rtm-cuda/Toa_gpu_2dtti_fd_1orderfunction.cu:313-499:
+ * the 2D GPU form (point source before the velocity update of gain pfac,
+ * first time sample dt, GPU d0 and tables, taper R=15 r=4 centred one cell
+ * off the source) with the TTI update (tti = 1), P recorded at 1-based depth
+ * rec_depth (the reference records row npml, i.e. 1).  The tilt is set
+ * after orc_create with orc_set_theta. */
+int orc_params_tti2d(orc_params *p, int nx, int nz, float dx, float dz, int npml,
+                     float favg, float pfac, int fs, int zs, int nt, float dt,
+                     int SV, int rec_depth);
 
 /* Field / table selectors for orc_get_*. */
 enum {
@@ -90,7 +103,7 @@ enum {
     ORC_P, ORC_Q, ORC_VP, ORC_EPS, ORC_DEL,
     ORC_COFFX1 = 20, ORC_COFFX2, ORC_COFFY1, ORC_COFFY2, ORC_COFFZ1, ORC_COFFZ2,
     ORC_ACOFFX1, ORC_ACOFFX2, ORC_ACOFFY1, ORC_ACOFFY2, ORC_ACOFFZ1, ORC_ACOFFZ2,
-    ORC_SRCAMP = 40, ORC_D0 = 41
+    ORC_SRCAMP = 40, ORC_D0 = 41, ORC_CTH = 42, ORC_STH = 43
 };
 
 void *orc_create_f32(const orc_params *p, const float *vp, const float *eps,
@@ -107,6 +120,9 @@ int orc_set_f32(void *h, int which, const float *in);
 int orc_set_f64(void *h, int which, const double *in);
 void orc_destroy_f32(void *h);
 void orc_destroy_f64(void *h);
+/* TTI tilt of the interior (radians, x-major, z fastest; tti params only) */
+int orc_set_theta_f32(void *h, const float *theta);
+int orc_set_theta_f64(void *h, const float *theta);
 long orc_ncell(void *h32);
 /* OpenMP threads of the stencil loops (n > 0 sets them); returns the number
  * in effect (1 without OpenMP). */

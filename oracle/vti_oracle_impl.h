@@ -30,6 +30,7 @@ typedef struct SIM {
     int tap_n;              /* SV taper factor per integer r^2, 0..tap_n-1 */
     double *tap;
     int split_z;            /* unsplit mode: row kept in the split form (-1: none) */
+    REAL *cth, *sth;        /* TTI: cos and sin of the padded tilt per cell (NULL otherwise) */
 } SIM;
 
 /* Ricker / Gaussian wavelet.
@@ -112,6 +113,11 @@ void *FN(orc_create)(const orc_params *p, const float *vp, const float *eps,
     if (s->nny > nnmax) nnmax = s->nny;
     for (int k = 0; k < 12; k++) s->tab[k] = (REAL *)calloc(nnmax, sizeof(REAL));
     s->amp = (REAL *)calloc(p->nt > 0 ? p->nt : 1, sizeof(REAL));
+    if (p->tti) {   /* tilt 0 until orc_set_theta */
+        s->cth = (REAL *)malloc(n * sizeof(REAL));
+        s->sth = (REAL *)calloc(n, sizeof(REAL));
+        for (long k = 0; k < n; k++) s->cth[k] = 1;
+    }
 
     /* Edge-replicating pad: pad_vv2d GPU_vti2dfd_kernels.cu:49-74 and
      * pad_vv3d GPU_vti3dfd_kernels.cu:208-255 copy x, then y, then z;
@@ -237,7 +243,35 @@ void FN(orc_destroy)(void *h)
     for (int k = 0; k < 11; k++) free(s->f[k]);
     for (int k = 0; k < 12; k++) free(s->tab[k]);
     free(s->amp); free(s->tap);
+    free(s->cth); free(s->sth);
     free(s);
+}
+
+/* Tilt of the interior (radians, x-major, z fastest), padded by edge
+ * replication as the model (pad_vv Toa_gpu_2dtti_fd_1orderfunction.cu:
+ * 190-215 on theta after read_file :217-242 converted it to radians, :235);
+ * the kernels take cos(theta[id]) and sin(theta[id]) of the float angle
+ * (:110-111, :170-177): cosf / sinf here. */
+int FN(orc_set_theta)(void *h, const float *theta)
+{
+    SIM *s = (SIM *)h;
+    if (!s || !s->cth || !theta) return -1;
+    const int np = s->p.npml, nx = s->p.nx, nz = s->p.nz;
+    for (int ix = 0; ix < s->nnx; ix++)
+        for (int iz = 0; iz < s->nnz; iz++) {
+            int jx = ix < np ? 0 : (ix >= np + nx ? nx - 1 : ix - np);
+            int jz = iz < np ? 0 : (iz >= np + nz ? nz - 1 : iz - np);
+            const float th = theta[(long)jz + (long)jx * nz];
+            const long id = (long)iz + (long)ix * s->nnz;
+            if (sizeof(REAL) == sizeof(float)) {
+                s->cth[id] = (REAL)cosf(th);
+                s->sth[id] = (REAL)sinf(th);
+            } else {
+                s->cth[id] = (REAL)cos((double)th);
+                s->sth[id] = (REAL)sin((double)th);
+            }
+        }
+    return 0;
 }
 
 /* Effective eps/delta after the SV taper. */
@@ -269,6 +303,31 @@ static void FN(vel)(SIM *s)
     const REAL *cx1 = s->tab[0], *cx2 = s->tab[1], *cy1 = s->tab[2], *cy2 = s->tab[3];
     const REAL *cz1 = s->tab[4], *cz2 = s->tab[5];
     const REAL dtx = s->dtx, dty = s->dty, dtz = s->dtz;
+    if (s->p.tti) {
+        /* update_vel Toa_gpu_2dtti_fd_1orderfunction.cu:81-114: the x and z
+         * derivatives of P and Q rotated by the tilt, both components
+         * damped by the product of the x and z factors */
+        const REAL *cth = s->cth, *sth = s->sth;
+#pragma omp parallel for schedule(static)
+        for (int ix = 4; ix < nnx - 4; ix++)
+            for (int iz = 4; iz < nnz - 4; iz++) {
+                long id = (long)ix * nnz + iz;
+                REAL Px = 0, Pz = 0, Qz = 0, Qx = 0;
+                for (int im = 0; im < 4; im++) {
+                    Px += c[im] * (P[id + (im + 1) * sx] - P[id - im * sx]);
+                    Pz += c[im] * (P[id + im + 1] - P[id - im]);
+                    Qz += c[im] * (Q[id + im + 1] - Q[id - im]);
+                    Qx += c[im] * (Q[id + (im + 1) * sx] - Q[id - im * sx]);
+                }
+                const REAL cs = cth[id], sn = sth[id];
+                const REAL g2 = ORC_PLANT == 9 ? cx2[ix] : cx2[ix] * cz2[iz];
+                const REAL g1 = ORC_PLANT == 9 ? cx1[ix] : cx1[ix] * cz1[iz];
+                const REAL su = ORC_PLANT == 7 ? -sn : sn;
+                u[id] = g2 * u[id] - g1 * (cs * dtx * Px - su * dtz * Pz);
+                w[id] = cx2[ix] * cz2[iz] * w[id] - cx1[ix] * cz1[iz] * (sn * dtx * Qx + cs * dtz * Qz);
+            }
+        return;
+    }
     if (s->p.ndim == 2) {
 #pragma omp parallel for schedule(static)
         for (int ix = 4; ix < nnx - 4; ix++)
@@ -317,6 +376,35 @@ static void FN(stress)(SIM *s, int it, int inject, int sv)
     const REAL *ax1 = s->tab[6], *ax2 = s->tab[7], *ay1 = s->tab[8], *ay2 = s->tab[9];
     const REAL *az1 = s->tab[10], *az2 = s->tab[11];
     const REAL dtx = s->dtx, dty = s->dty, dtz = s->dtz;
+    if (s->p.tti) {
+        /* update_stress Toa_gpu_2dtti_fd_1orderfunction.cu:116-183 */
+        const REAL *cth = s->cth, *sth = s->sth;
+#pragma omp parallel for schedule(static)
+        for (int ix = 4; ix < nnx - 4; ix++)
+            for (int iz = 4; iz < nnz - 4; iz++) {
+                long id = (long)ix * nnz + iz;
+                REAL ee, dd;
+                FN(taper)(s, sv, ix, 0, iz, id, &ee, &dd);
+                REAL ux = 0, uz = 0, wz = 0, wx = 0;
+                for (int im = 0; im < 4; im++) {
+                    ux += c[im] * (u[id + im * sx] - u[id - (im + 1) * sx]);
+                    uz += c[im] * (u[id + im] - u[id - im - 1]);
+                    wz += c[im] * (w[id + im] - w[id - im - 1]);
+                    wx += c[im] * (w[id + im * sx] - w[id - (im + 1) * sx]);
+                }
+                const REAL vv = vp[id], cs = cth[id], sn = sth[id];
+                const REAL a2 = ax2[ix] * az2[iz], a1 = ax1[ix] * az1[iz];
+                const REAL e2 = 1 + 2 * ee, sd = SQRT_R(1 + 2 * dd);
+                const REAL cw = ORC_PLANT == 8 ? sn : cs, sw = ORC_PLANT == 8 ? cs : sn;
+                px[id] = a2 * px[id] - a1 * vv * vv * e2 * (cs * dtx * ux - sn * dtz * uz);
+                pz[id] = a2 * pz[id] - a1 * vv * vv * sd * (sw * dtx * wx + cw * dtz * wz);
+                qx[id] = a2 * qx[id] - a1 * vv * vv * sd * (cs * dtx * ux - sn * dtz * uz);
+                qz[id] = a2 * qz[id] - a1 * vv * vv * (sn * dtx * wx + cs * dtz * wz);
+                P[id] = px[id] + pz[id];
+                Q[id] = qx[id] + qz[id];
+            }
+        return;
+    }
     if (s->p.ndim == 2) {
 #pragma omp parallel for schedule(static)
         for (int ix = 4; ix < nnx - 4; ix++)
@@ -529,6 +617,8 @@ static REAL *FN(select)(SIM *s, int which, long *len)
     }
     if (which == ORC_SRCAMP) { *len = s->p.nt; return s->amp; }
     if (which == ORC_D0) { *len = 1; return &s->d0; }
+    if (which == ORC_CTH && s->cth) { *len = s->n; return s->cth; }
+    if (which == ORC_STH && s->sth) { *len = s->n; return s->sth; }
     *len = 0;
     return NULL;
 }

@@ -283,3 +283,32 @@ def test_quantized_rtm_image_and_store_against_sparse():
         assert res[tol, 2][2]["store_bytes"] < 0.8 * res[tol, 1][2]["store_bytes"]
     assert st["recomputed"] > 0
     assert np.array_equal(img_r, img_p), rel_l2(img_r, img_p)
+
+
+def test_fitted_basis_needs_fewer_words_than_untrained_bases():
+    """What the learning gains for the quantized codec: the basis fitted on
+    two shots' snapshots codes a third shot's snapshot at the same per-block
+    bound (1 %) in fewer words than the identity basis (the raw samples) or
+    a random orthonormal basis."""
+    import torch
+    cfg, vp, e, d = _setup()
+    with vti.Propagator(cfg, vp, e, d) as prop, vti.Autoencoder(crop=32, batch=1, seed=5) as A:
+        for cx, cy in ((10, 12), (22, 20)):
+            prop.set_shot(_shot(cfg, cx, cy))
+            prop.ae_fit_shot(A, save_every=4, nblocks=64, src_box=0)
+        A.fit_end()
+        fitted = A.basis()
+        prop.set_shot(_shot(cfg, 16, 9))
+        prop.reset()
+        prop.step(0, 150)
+        P = prop.field("P")
+        words = {}
+        for name, V in (("fitted", fitted), ("identity", np.eye(512, dtype=np.float32)),
+                        ("random", _orthonormal(3))):
+            A.set_basis(V)
+            k, err, out, *_ = _codec(A, torch, P, N, 0.01, 64 * MAXW)
+            box = P[NPML:NPML + N, NPML:NPML + N, NPML:NPML + N]
+            assert rel_l2(out, box) <= err * (1 + 1e-3) + 1e-6
+            words[name] = k
+    _say(f"words at 1 % per block: {words} (fp32 box: {N ** 3})")
+    assert words["fitted"] < words["identity"] and words["fitted"] < words["random"]
