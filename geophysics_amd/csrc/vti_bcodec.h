@@ -27,13 +27,19 @@
 namespace vti {
 namespace bc {
 
-constexpr int kBlocksPerWG = 32;   // 8^3 blocks per encode / decode workgroup
-
 // Basis matrices on the device, bf16 hi/lo splits of the fp32 basis V
-// (V[k][j]: voxel k, component j): vt_* = V^T (component rows), v_* = V.
+// (V[k][j]: voxel k, component j): vt_* = V^T (component rows), v_* = V,
+// each 512 x 512 in fragment-major order (frag_off): one MFMA operand of a
+// wavefront (16 rows x 32 columns; lane l: row l % 16, columns
+// 8 (l / 16) .. + 8) is 1 KB contiguous, lane after lane.
 struct Basis {
     const uint16_t *vt_h, *vt_l, *v_h, *v_l;
 };
+
+__host__ __device__ constexpr long long frag_off(int row, int col)
+{
+    return ((long long)(row >> 4) * 16 + (col >> 5)) * 512 + (((col >> 3) & 3) * 16 + (row & 15)) * 8 + (col & 7);
+}
 
 // Box of the snapshot (box coordinates) that is zeroed before encoding.
 struct Mask {
@@ -55,6 +61,8 @@ struct Work {
     float *coef = nullptr;   // [nblk][512]
     Counters *cnt = nullptr;
     long long cap_blocks = 0;
+    void *aux = nullptr;     // the quantized codec's per-block search results and scan sums
+    long long aux_cap = 0;   // bytes
 };
 
 int work_reserve(Work &w, int n);

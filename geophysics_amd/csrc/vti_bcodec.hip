@@ -1,17 +1,17 @@
 // vti_bcodec.hip — error-bounded sparse block-transform codec (gfx950).
 //
-// Encode: a workgroup (4 wavefronts) takes 32 blocks, stages them in LDS as
-// bf16 hi/lo pairs (x = hi + lo to 2^-16), and computes the 32 x 512
-// coefficient tile on v_mfma_f32_16x16x32_bf16 as Xh Vh + Xl Vh + Xh Vl
-// (fp32 accumulation; the dropped Xl Vl term is below 2^-16 relative); the
-// staging also yields the strongest block energy (one atomic per
-// workgroup).  Pack: one wavefront per block finds, by bisection on the
-// float bits of c^2 (31 wave reductions), the largest cut whose dropped
-// energy fits the block's budget, keeps the coefficients above it, places
-// them with a wave prefix sum, a prefix over the workgroup's 8 blocks and
-// one atomic per workgroup.  Decode: the kept
-// values (15 significant bits = bf16 hi + lo exactly) are scattered into
-// zeroed LDS tiles and multiplied by V: Ch Vh + Cl Vh + Ch Vl.
+// Encode (encode64_kernel): a workgroup of 8 wavefronts takes 64 blocks,
+// stages them in LDS as bf16 hi/lo pairs (x = hi + lo to 2^-16), and
+// computes the 64 x 512 coefficient tile on v_mfma_f32_16x16x32_bf16 as
+// Xh Vh + Xl Vh + Xh Vl (fp32 accumulation; the dropped Xl Vl term is below
+// 2^-16 relative); the staging also yields the strongest block energy.
+// Pack: one wavefront per block finds, by bisection on the float bits of
+// c^2 (31 wave reductions), the largest cut whose dropped energy fits the
+// block's budget, keeps the coefficients above it, places them with a wave
+// prefix sum, a prefix over the workgroup's 8 blocks and one atomic per
+// workgroup.  Decode (decode64_kernel): the kept values (15 significant
+// bits = bf16 hi + lo exactly) are scattered into zeroed LDS tiles of 64
+// blocks and multiplied by V: Ch Vh + Cl Vh + Ch Vl.
 #include "vti_bcodec.h"
 
 #include <algorithm>
@@ -29,7 +29,6 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef unsigned short us8 __attribute__((ext_vector_type(8)));
 
-constexpr int BB = kBlocksPerWG;
 constexpr int LROW = 512 + 8;   // LDS row (bf16), padded against bank conflicts
 
 __device__ __forceinline__ bf16x8 as_bf(us8 v) { return __builtin_bit_cast(bf16x8, v); }
@@ -53,86 +52,137 @@ __device__ __forceinline__ uint32_t to_v23(float f)
     return (u + 0xffu + ((u >> 9) & 1u)) & 0xfffffe00u;
 }
 
-__global__ __launch_bounds__(256) void encode_kernel(Basis V, const float *__restrict__ P, long long off,
-                                                     long long sy, long long sx, int n, Mask m,
-                                                     float *__restrict__ coef, Counters *cnt)
+// Encode with 64 blocks per workgroup of 8 wavefronts: the 1 MB basis is
+// streamed from L2 once per 64 blocks, the
+// staging writes 16-byte rows of 8 voxels (one z line of one block per lane:
+// the wavefront's 64 lanes read the same line of 64 consecutive blocks,
+// adjacent in memory), block energies are summed per lane and then over the
+// 8 wavefronts (no LDS atomics), and each wavefront's basis operands are
+// loaded one K step ahead of the MFMAs that use them.  Wavefront w computes
+// blocks 32 (w & 1) .. + 32 against components 128 (w >> 1) .. + 128 as
+// Xh Vh + Xl Vh + Xh Vl.
+constexpr int EB = 64;   // blocks per workgroup
+constexpr int EW = 8;    // wavefronts per workgroup
+
+__device__ __forceinline__ us8 pack8(const uint16_t h[8])
 {
-    __shared__ uint16_t xh[BB][LROW];
-    __shared__ uint16_t xl[BB][LROW];
-    __shared__ float eb[BB];
+    us8 r;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) r[q] = h[q];
+    return r;
+}
+
+__global__ __launch_bounds__(64 * EW) void encode64_kernel(Basis V, const float *__restrict__ P, long long off,
+                                                          long long sy, long long sx, int n, Mask m,
+                                                          float *__restrict__ coef, Counters *cnt)
+{
+    __shared__ __align__(16) uint16_t xh[EB][LROW];
+    __shared__ __align__(16) uint16_t xl[EB][LROW];
+    __shared__ float ep[EW][EB];
     const int nb = n / 8;
     const long long nblk = (long long)nb * nb * nb;
-    const long long b0 = (long long)blockIdx.x * BB;
-    if (threadIdx.x < BB) eb[threadIdx.x] = 0.0f;
-    __syncthreads();
-    // stage: 4 consecutive voxels (same ty, tx) per thread and iteration
-    for (int e = threadIdx.x; e < BB * 128; e += 256) {
-        const int i = e >> 7, k = (e & 127) * 4;
-        const long long b = b0 + i;
-        float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        if (b < nblk) {
-            const int bz = (int)(b % nb), bx = (int)((b / nb) % nb), by = (int)(b / ((long long)nb * nb));
-            const int y = 8 * by + (k >> 6), x = 8 * bx + ((k >> 3) & 7), z = 8 * bz + (k & 7);
-            const float *src = P + off + (long long)y * sy + (long long)x * sx + z;
-            const bool yin = y >= m.y0 && y < m.y0 + m.by && x >= m.x0 && x < m.x0 + m.bx;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const bool masked = yin && z + q >= m.z0 && z + q < m.z0 + m.bz;
-                v[q] = masked ? 0.0f : src[q];
-            }
-        }
-        float e2 = 0.0f;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const uint16_t h = to_bf(v[q]);
-            xh[i][k + q] = h;
-            xl[i][k + q] = to_bf(v[q] - from_bf(h));
-            e2 += v[q] * v[q];
-        }
-        atomicAdd(&eb[i], e2);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float mx = 0.0f;
-        for (int i = 0; i < BB; ++i) mx = fmaxf(mx, eb[i]);
-        atomicMax(&cnt->emax_bits, __float_as_uint(mx));   // non-negative floats order as their bits
-    }
+    const long long b0 = (long long)blockIdx.x * EB;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    {
+        // lane l stages block b0 + l, z lines w, w + 8, .. (line = ty * 8 + tx)
+        const long long b = b0 + l;
+        float e2 = 0.0f;
+        const bool live = b < nblk;
+        int bz = 0, bx = 0, by = 0;
+        if (live) {
+            bz = (int)(b % nb);
+            bx = (int)((b / nb) % nb);
+            by = (int)(b / ((long long)nb * nb));
+        }
+#pragma unroll 2
+        for (int i = 0; i < 8; ++i) {
+            const int line = w + 8 * i;
+            uint16_t hh[8], ll[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) hh[q] = ll[q] = 0;
+            if (live) {
+                const int y = 8 * by + (line >> 3), x = 8 * bx + (line & 7), z0 = 8 * bz;
+                const float *src = P + off + (long long)y * sy + (long long)x * sx + z0;
+                const bool yin = y >= m.y0 && y < m.y0 + m.by && x >= m.x0 && x < m.x0 + m.bx;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const bool masked = yin && z0 + q >= m.z0 && z0 + q < m.z0 + m.bz;
+                    const float v = masked ? 0.0f : src[q];
+                    hh[q] = to_bf(v);
+                    ll[q] = to_bf(v - from_bf(hh[q]));
+                    e2 += v * v;
+                }
+            }
+            *reinterpret_cast<us8 *>(&xh[l][line * 8]) = pack8(hh);
+            *reinterpret_cast<us8 *>(&xl[l][line * 8]) = pack8(ll);
+        }
+        ep[w][l] = e2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {   // wavefront 0: block energies, the workgroup's strongest
+        float e = 0.0f;
+#pragma unroll
+        for (int k = 0; k < EW; ++k) e += ep[k][l];
+        float mx = e;
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) mx = fmaxf(mx, __shfl_xor(mx, d, 64));
+        // non-negative floats order as their bits; the maximum only grows, so
+        // a workgroup below the value already stored skips the atomic (the
+        // same-address atomics of all workgroups would serialise)
+        if (l == 0 && __float_as_uint(mx) > __hip_atomic_load(&cnt->emax_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&cnt->emax_bits, __float_as_uint(mx));
+    }
+    const int rg = w & 1, cg = w >> 1;
     f4 acc[2][8];
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int c = 0; c < 8; ++c) acc[r][c] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    us8 bh[8], bl[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int j = 128 * cg + 16 * c + (l & 15);
+        bh[c] = *reinterpret_cast<const us8 *>(V.vt_h + frag_off(j, 8 * (l >> 4)));
+        bl[c] = *reinterpret_cast<const us8 *>(V.vt_l + frag_off(j, 8 * (l >> 4)));
+    }
     for (int kk = 0; kk < 512; kk += 32) {
         const int kc = kk + 8 * (l >> 4);
+        const int kn = (kk + 32 < 512 ? kk + 32 : kk) + 8 * (l >> 4);   // next K step (the last reloads its own)
+        us8 nh[8], nl[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int j = 128 * cg + 16 * c + (l & 15);
+            nh[c] = *reinterpret_cast<const us8 *>(V.vt_h + frag_off(j, kn));
+            nl[c] = *reinterpret_cast<const us8 *>(V.vt_l + frag_off(j, kn));
+        }
         us8 ah[2], al[2];
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
-            ah[r] = *reinterpret_cast<const us8 *>(&xh[16 * r + (l & 15)][kc]);
-            al[r] = *reinterpret_cast<const us8 *>(&xl[16 * r + (l & 15)][kc]);
+            ah[r] = *reinterpret_cast<const us8 *>(&xh[32 * rg + 16 * r + (l & 15)][kc]);
+            al[r] = *reinterpret_cast<const us8 *>(&xl[32 * rg + 16 * r + (l & 15)][kc]);
         }
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int j = 128 * w + 16 * c + (l & 15);
-            const us8 bh = *reinterpret_cast<const us8 *>(V.vt_h + (long long)j * 512 + kc);
-            const us8 bl = *reinterpret_cast<const us8 *>(V.vt_l + (long long)j * 512 + kc);
+        for (int c = 0; c < 8; ++c)
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
-                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(ah[r]), as_bf(bh), acc[r][c], 0, 0, 0);
-                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(al[r]), as_bf(bh), acc[r][c], 0, 0, 0);
-                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(ah[r]), as_bf(bl), acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(ah[r]), as_bf(bh[c]), acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(al[r]), as_bf(bh[c]), acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(ah[r]), as_bf(bl[c]), acc[r][c], 0, 0, 0);
             }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            bh[c] = nh[c];
+            bl[c] = nl[c];
         }
     }
-    // D row 4*(l/16) + e, column l%16 of each 16 x 16 tile
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const long long b = b0 + 16 * r + 4 * (l >> 4) + e;
+            const long long b = b0 + 32 * rg + 16 * r + 4 * (l >> 4) + e;
             if (b >= nblk) continue;
 #pragma unroll
-            for (int c = 0; c < 8; ++c) coef[b * 512 + 128 * w + 16 * c + (l & 15)] = acc[r][c][e];
+            for (int c = 0; c < 8; ++c) coef[b * 512 + 128 * cg + 16 * c + (l & 15)] = acc[r][c][e];
         }
 }
 
@@ -240,23 +290,28 @@ __global__ __launch_bounds__(64 * PW) void pack_kernel(const float *__restrict__
     }
 }
 
-__global__ __launch_bounds__(256) void decode_kernel(Basis V, const uint32_t *__restrict__ ent,
-                                                     const uint32_t *__restrict__ boff,
-                                                     const uint16_t *__restrict__ bcnt, int n,
-                                                     float *__restrict__ out)
+// Decode with 64 blocks per workgroup of 8 wavefronts: the basis streamed
+// once per 64 blocks, the basis operands loaded one K
+// step ahead; wavefront w scatters blocks w, w + 8, .. into the zeroed LDS
+// tile and computes blocks 32 (w & 1) .. + 32 against voxels
+// 128 (w >> 1) .. + 128.
+__global__ __launch_bounds__(64 * EW) void decode64_kernel(Basis V, const uint32_t *__restrict__ ent,
+                                                          const uint32_t *__restrict__ boff,
+                                                          const uint16_t *__restrict__ bcnt, int n,
+                                                          float *__restrict__ out)
 {
-    __shared__ uint16_t ch[BB][LROW];
-    __shared__ uint16_t cl[BB][LROW];
+    __shared__ __align__(16) uint16_t ch[EB][LROW];
+    __shared__ __align__(16) uint16_t cl[EB][LROW];
     const int nb = n / 8;
     const long long nblk = (long long)nb * nb * nb;
-    const long long b0 = (long long)blockIdx.x * BB;
-    for (int i = threadIdx.x; i < BB * LROW / 2; i += 256) {
+    const long long b0 = (long long)blockIdx.x * EB;
+    for (int i = threadIdx.x; i < EB * LROW / 2; i += 64 * EW) {
         reinterpret_cast<uint32_t *>(&ch[0][0])[i] = 0u;
         reinterpret_cast<uint32_t *>(&cl[0][0])[i] = 0u;
     }
     __syncthreads();
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    for (int i = w; i < BB; i += 4) {
+    for (int i = w; i < EB; i += EW) {
         const long long b = b0 + i;
         if (b >= nblk) break;
         const int cnt = bcnt[b];
@@ -270,42 +325,59 @@ __global__ __launch_bounds__(256) void decode_kernel(Basis V, const uint32_t *__
         }
     }
     __syncthreads();
+    const int rg = w & 1, cg = w >> 1;
     f4 acc[2][8];
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int c = 0; c < 8; ++c) acc[r][c] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    us8 bh[8], bl[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int k = 128 * cg + 16 * c + (l & 15);
+        bh[c] = *reinterpret_cast<const us8 *>(V.v_h + frag_off(k, 8 * (l >> 4)));
+        bl[c] = *reinterpret_cast<const us8 *>(V.v_l + frag_off(k, 8 * (l >> 4)));
+    }
     for (int kk = 0; kk < 512; kk += 32) {
         const int kc = kk + 8 * (l >> 4);
+        const int kn = (kk + 32 < 512 ? kk + 32 : kk) + 8 * (l >> 4);   // next K step (the last reloads its own)
+        us8 nh[8], nl[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int k = 128 * cg + 16 * c + (l & 15);
+            nh[c] = *reinterpret_cast<const us8 *>(V.v_h + frag_off(k, kn));
+            nl[c] = *reinterpret_cast<const us8 *>(V.v_l + frag_off(k, kn));
+        }
         us8 a[2], al[2];
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
-            a[r] = *reinterpret_cast<const us8 *>(&ch[16 * r + (l & 15)][kc]);
-            al[r] = *reinterpret_cast<const us8 *>(&cl[16 * r + (l & 15)][kc]);
+            a[r] = *reinterpret_cast<const us8 *>(&ch[32 * rg + 16 * r + (l & 15)][kc]);
+            al[r] = *reinterpret_cast<const us8 *>(&cl[32 * rg + 16 * r + (l & 15)][kc]);
         }
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int k = 128 * w + 16 * c + (l & 15);
-            const us8 bh = *reinterpret_cast<const us8 *>(V.v_h + (long long)k * 512 + kc);
-            const us8 bl = *reinterpret_cast<const us8 *>(V.v_l + (long long)k * 512 + kc);
+        for (int c = 0; c < 8; ++c)
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
-                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(a[r]), as_bf(bh), acc[r][c], 0, 0, 0);
-                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(al[r]), as_bf(bh), acc[r][c], 0, 0, 0);
-                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(a[r]), as_bf(bl), acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(a[r]), as_bf(bh[c]), acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(al[r]), as_bf(bh[c]), acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(a[r]), as_bf(bl[c]), acc[r][c], 0, 0, 0);
             }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            bh[c] = nh[c];
+            bl[c] = nl[c];
         }
     }
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const long long b = b0 + 16 * r + 4 * (l >> 4) + e;
+            const long long b = b0 + 32 * rg + 16 * r + 4 * (l >> 4) + e;
             if (b >= nblk) continue;
             const int bz = (int)(b % nb), bx = (int)((b / nb) % nb), by = (int)(b / ((long long)nb * nb));
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
-                const int k = 128 * w + 16 * c + (l & 15);
+                const int k = 128 * cg + 16 * c + (l & 15);
                 const int y = 8 * by + (k >> 6), x = 8 * bx + ((k >> 3) & 7), z = 8 * bz + (k & 7);
                 out[((long long)y * n + x) * n + z] = acc[r][c][e];
             }
@@ -353,6 +425,7 @@ void work_free(Work &w)
 {
     (void)hipFree(w.coef);
     (void)hipFree(w.cnt);
+    (void)hipFree(w.aux);
     w = Work{};
 }
 
@@ -364,10 +437,10 @@ void basis_split(const float *V, uint16_t *out)
             const float x = V[k * 512 + j];
             const uint16_t h = host_bf(x);
             const uint16_t lo = host_bf(x - host_from_bf(h));
-            v_h[k * 512 + j] = h;
-            v_l[k * 512 + j] = lo;
-            vt_h[j * 512 + k] = h;
-            vt_l[j * 512 + k] = lo;
+            v_h[frag_off(k, j)] = h;
+            v_l[frag_off(k, j)] = lo;
+            vt_h[frag_off(j, k)] = h;
+            vt_l[frag_off(j, k)] = lo;
         }
 }
 
@@ -382,8 +455,8 @@ hipError_t launch_encode(const Basis &V, const float *P, long long off, long lon
     const long long nb = n / 8, nblk = nb * nb * nb;
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t *>(w.cnt),
                        (long long)(sizeof(Counters) / 4));
-    hipLaunchKernelGGL(encode_kernel, dim3((unsigned)((nblk + BB - 1) / BB)), dim3(256), 0, s, V, P, off, sy, sx, n, m,
-                       w.coef, w.cnt);
+    hipLaunchKernelGGL(encode64_kernel, dim3((unsigned)((nblk + EB - 1) / EB)), dim3(64 * EW), 0, s, V, P, off, sy, sx,
+                       n, m, w.coef, w.cnt);
     return hipGetLastError();
 }
 
@@ -402,8 +475,8 @@ hipError_t launch_decode(const Basis &V, const uint32_t *ent, const uint32_t *bo
                          float *out, hipStream_t s)
 {
     const long long nb = n / 8, nblk = nb * nb * nb;
-    hipLaunchKernelGGL(decode_kernel, dim3((unsigned)((nblk + BB - 1) / BB)), dim3(256), 0, s, V, ent, boff, bcnt, n,
-                       out);
+    hipLaunchKernelGGL(decode64_kernel, dim3((unsigned)((nblk + EB - 1) / EB)), dim3(64 * EW), 0, s, V, ent, boff, bcnt,
+                       n, out);
     return hipGetLastError();
 }
 
