@@ -44,6 +44,7 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 BYTES_STRESS = 80              # stress kernel: 12 reads + 8 writes of fp32 per cell
 BYTES_VEL = 32                 # velocity kernel: 5 reads + 3 writes
 BYTES_STEP = 112
+BYTES_STEP_2D = 84             # 2D step: velocity 16 B read + 8 written, stress 36 read + 24 written
 
 
 def log(*a):
@@ -363,7 +364,16 @@ def bench_batch_rtm(args, torch, vti):
                                                                "injection row split)",
                    "tile": list(cfgd["tile"]), "grid_mult": cfgd["grid_mult"], "seconds": round(secs, 3),
                    "shots_per_s": round(nshot / secs, 3)},
-        "roofline": None, "cpu_baseline": None,
+        # one forward step of every shot (the velocity and stress work-list
+        # launches), its time from the autotuner's HIP events on the batch's
+        # stream; bytes of the reference structure's 2D step (velocity: P, Q,
+        # u, w read, u, w written; stress: u, w, vp, eps, delta and the four
+        # split fields read, the split fields and P, Q written)
+        "roofline": {"bound": "hbm", "kernel": "forward step of all shots (vti_batch velocity + stress launches)",
+                     "achieved": round(BYTES_STEP_2D * cells / (ms_tune / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(BYTES_STEP_2D * cells / (ms_tune / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "bytes_per_cell_step": BYTES_STEP_2D, "step_ms": round(ms_tune, 4)},
+        "cpu_baseline": None,
     }
     B.close()
     print(json.dumps(out), flush=True)

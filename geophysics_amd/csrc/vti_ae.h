@@ -41,6 +41,24 @@ int ae_bq_encode(vti_ae *A, const float *P, long long off, long long sy, long lo
                  double *err, hipStream_t s);
 int ae_bq_decode(vti_ae *A, const uint32_t *ent, const uint32_t *boff, const uint16_t *bcnt, int n, float *out,
                  hipStream_t s);
+// Device bytes of the codecs' work buffers (coefficients, counters, the
+// quantized codec's search results) as currently allocated.
+long long ae_codec_work_bytes(const vti_ae *A);
+// The two decodes into a sink (bc::TileSink: here the image accumulation
+// of the decoded snapshot times the receiver field, the exact box included).
+int ae_bc_decode_sink(vti_ae *A, const uint32_t *ent, const uint32_t *boff, const uint16_t *bcnt, int n,
+                      const bc::TileSink &sk, hipStream_t s);
+int ae_bq_decode_sink(vti_ae *A, const uint32_t *ent, const uint32_t *boff, const uint16_t *bcnt, int n,
+                      const bc::TileSink &sk, hipStream_t s);
+// The two encodes without their host sync (bc::encode_start,
+// bq::encode_start): `read` recorded once P has been read, the counters in
+// *hc (pinned) for bc::encode_finish once s has passed their copy.
+int ae_bc_encode_start(vti_ae *A, const float *P, long long off, long long sy, long long sx, int n,
+                       const bc::Mask &m, float tol, float flo, uint32_t *ent, long long cap, uint32_t *boff,
+                       uint16_t *bcnt, hipEvent_t read, bc::Counters *hc, hipStream_t s);
+int ae_bq_encode_start(vti_ae *A, const float *P, long long off, long long sy, long long sx, int n,
+                       const bc::Mask &m, float tol, float flo, uint32_t *ent, long long cap, uint32_t *boff,
+                       uint16_t *bcnt, hipEvent_t read, bc::Counters *hc, hipStream_t s);
 
 }  // namespace vti
 

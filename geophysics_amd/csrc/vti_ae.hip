@@ -510,6 +510,53 @@ int ae_bq_encode(vti_ae *A, const float *P, long long off, long long sy, long lo
     return VTI_OK;
 }
 
+long long ae_codec_work_bytes(const vti_ae *A)
+{
+    const bc::Work &w = A->bcw;
+    return (w.coef ? w.cap_blocks * 512 * (long long)sizeof(float) : 0) + (w.cnt ? (long long)sizeof(bc::Counters) : 0) +
+           w.aux_cap;
+}
+
+int ae_bc_decode_sink(vti_ae *A, const uint32_t *ent, const uint32_t *boff, const uint16_t *bcnt, int n,
+                      const bc::TileSink &sk, hipStream_t s)
+{
+    if (!A->basis) return aefail(VTI_ERR_STATE, "sparse codec: no basis (fit or vti_ae_set_basis first)");
+    AE_HIP(bc::launch_decode(bc::basis_view(A->basis), ent, boff, bcnt, n, sk, s));
+    return VTI_OK;
+}
+
+int ae_bq_decode_sink(vti_ae *A, const uint32_t *ent, const uint32_t *boff, const uint16_t *bcnt, int n,
+                      const bc::TileSink &sk, hipStream_t s)
+{
+    if (!A->basis) return aefail(VTI_ERR_STATE, "quantized codec: no basis (fit or vti_ae_set_basis first)");
+    AE_HIP(bq::launch_decode(bc::basis_view(A->basis), ent, boff, bcnt, n, sk, s));
+    return VTI_OK;
+}
+
+int ae_bc_encode_start(vti_ae *A, const float *P, long long off, long long sy, long long sx, int n,
+                       const bc::Mask &m, float tol, float flo, uint32_t *ent, long long cap, uint32_t *boff,
+                       uint16_t *bcnt, hipEvent_t read, bc::Counters *hc, hipStream_t s)
+{
+    if (!A->basis) return aefail(VTI_ERR_STATE, "sparse codec: no basis (fit or vti_ae_set_basis first)");
+    int st = bc::work_reserve(A->bcw, n);
+    if (!st)
+        st = bc::encode_start(bc::basis_view(A->basis), P, off, sy, sx, n, m, tol, flo, A->bcw, ent, cap, boff, bcnt,
+                              read, hc, s);
+    if (st) return aefail(st, "sparse codec: encode");
+    return VTI_OK;
+}
+
+int ae_bq_encode_start(vti_ae *A, const float *P, long long off, long long sy, long long sx, int n,
+                       const bc::Mask &m, float tol, float flo, uint32_t *ent, long long cap, uint32_t *boff,
+                       uint16_t *bcnt, hipEvent_t read, bc::Counters *hc, hipStream_t s)
+{
+    if (!A->basis) return aefail(VTI_ERR_STATE, "quantized codec: no basis (fit or vti_ae_set_basis first)");
+    const int st = bq::encode_start(bc::basis_view(A->basis), P, off, sy, sx, n, m, tol, flo, A->bcw, ent, cap, boff,
+                                    bcnt, read, hc, s);
+    if (st) return aefail(st, "quantized codec: encode");
+    return VTI_OK;
+}
+
 int ae_bq_decode(vti_ae *A, const uint32_t *ent, const uint32_t *boff, const uint16_t *bcnt, int n, float *out,
                  hipStream_t s)
 {

@@ -29,7 +29,6 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef unsigned short us8 __attribute__((ext_vector_type(8)));
 
-constexpr int LROW = 512 + 8;   // LDS row (bf16), padded against bank conflicts
 
 __device__ __forceinline__ bf16x8 as_bf(us8 v) { return __builtin_bit_cast(bf16x8, v); }
 
@@ -61,8 +60,8 @@ __device__ __forceinline__ uint32_t to_v23(float f)
 // loaded one K step ahead of the MFMAs that use them.  Wavefront w computes
 // blocks 32 (w & 1) .. + 32 against components 128 (w >> 1) .. + 128 as
 // Xh Vh + Xl Vh + Xh Vl.
-constexpr int EB = 64;   // blocks per workgroup
-constexpr int EW = 8;    // wavefronts per workgroup
+constexpr int EB = kTileBlocks;   // blocks per workgroup
+constexpr int EW = kTileWaves;    // wavefronts per workgroup
 
 __device__ __forceinline__ us8 pack8(const uint16_t h[8])
 {
@@ -132,57 +131,16 @@ __global__ __launch_bounds__(64 * EW) void encode64_kernel(Basis V, const float 
         if (l == 0 && __float_as_uint(mx) > __hip_atomic_load(&cnt->emax_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             atomicMax(&cnt->emax_bits, __float_as_uint(mx));
     }
-    const int rg = w & 1, cg = w >> 1;
-    f4 acc[2][8];
+    tile_f4 acc[4][4];
+    tile_mfma(xh, xl, V.vt_h, V.vt_l, w, l, acc);
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int c = 0; c < 8; ++c) acc[r][c] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    us8 bh[8], bl[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const int j = 128 * cg + 16 * c + (l & 15);
-        bh[c] = *reinterpret_cast<const us8 *>(V.vt_h + frag_off(j, 8 * (l >> 4)));
-        bl[c] = *reinterpret_cast<const us8 *>(V.vt_l + frag_off(j, 8 * (l >> 4)));
-    }
-    for (int kk = 0; kk < 512; kk += 32) {
-        const int kc = kk + 8 * (l >> 4);
-        const int kn = (kk + 32 < 512 ? kk + 32 : kk) + 8 * (l >> 4);   // next K step (the last reloads its own)
-        us8 nh[8], nl[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int j = 128 * cg + 16 * c + (l & 15);
-            nh[c] = *reinterpret_cast<const us8 *>(V.vt_h + frag_off(j, kn));
-            nl[c] = *reinterpret_cast<const us8 *>(V.vt_l + frag_off(j, kn));
-        }
-        us8 ah[2], al[2];
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            ah[r] = *reinterpret_cast<const us8 *>(&xh[32 * rg + 16 * r + (l & 15)][kc]);
-            al[r] = *reinterpret_cast<const us8 *>(&xl[32 * rg + 16 * r + (l & 15)][kc]);
-        }
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-#pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(ah[r]), as_bf(bh[c]), acc[r][c], 0, 0, 0);
-                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(al[r]), as_bf(bh[c]), acc[r][c], 0, 0, 0);
-                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(ah[r]), as_bf(bl[c]), acc[r][c], 0, 0, 0);
-            }
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            bh[c] = nh[c];
-            bl[c] = nl[c];
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
+    for (int r = 0; r < 4; ++r)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const long long b = b0 + 32 * rg + 16 * r + 4 * (l >> 4) + e;
+            const long long b = b0 + 16 * r + 4 * (l >> 4) + e;
             if (b >= nblk) continue;
 #pragma unroll
-            for (int c = 0; c < 8; ++c) coef[b * 512 + 128 * cg + 16 * c + (l & 15)] = acc[r][c][e];
+            for (int c = 0; c < 4; ++c) coef[b * 512 + 64 * w + 16 * c + (l & 15)] = acc[r][c][e];
         }
 }
 
@@ -297,18 +255,15 @@ __global__ __launch_bounds__(64 * PW) void pack_kernel(const float *__restrict__
 // 128 (w >> 1) .. + 128.
 __global__ __launch_bounds__(64 * EW) void decode64_kernel(Basis V, const uint32_t *__restrict__ ent,
                                                           const uint32_t *__restrict__ boff,
-                                                          const uint16_t *__restrict__ bcnt, int n,
-                                                          float *__restrict__ out)
+                                                          const uint16_t *__restrict__ bcnt, int n, TileSink sk)
 {
-    __shared__ __align__(16) uint16_t ch[EB][LROW];
-    __shared__ __align__(16) uint16_t cl[EB][LROW];
+    __shared__ __align__(16) uint16_t lds[2][EB][LROW];   // coefficient tiles (hi, lo), then the output tile
+    uint16_t(*ch)[LROW] = lds[0];
+    uint16_t(*cl)[LROW] = lds[1];
     const int nb = n / 8;
     const long long nblk = (long long)nb * nb * nb;
     const long long b0 = (long long)blockIdx.x * EB;
-    for (int i = threadIdx.x; i < EB * LROW / 2; i += 64 * EW) {
-        reinterpret_cast<uint32_t *>(&ch[0][0])[i] = 0u;
-        reinterpret_cast<uint32_t *>(&cl[0][0])[i] = 0u;
-    }
+    for (int i = threadIdx.x; i < EB * LROW; i += 64 * EW) reinterpret_cast<uint32_t *>(&lds[0][0][0])[i] = 0u;
     __syncthreads();
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     for (int i = w; i < EB; i += EW) {
@@ -325,63 +280,10 @@ __global__ __launch_bounds__(64 * EW) void decode64_kernel(Basis V, const uint32
         }
     }
     __syncthreads();
-    const int rg = w & 1, cg = w >> 1;
-    f4 acc[2][8];
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int c = 0; c < 8; ++c) acc[r][c] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    us8 bh[8], bl[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const int k = 128 * cg + 16 * c + (l & 15);
-        bh[c] = *reinterpret_cast<const us8 *>(V.v_h + frag_off(k, 8 * (l >> 4)));
-        bl[c] = *reinterpret_cast<const us8 *>(V.v_l + frag_off(k, 8 * (l >> 4)));
-    }
-    for (int kk = 0; kk < 512; kk += 32) {
-        const int kc = kk + 8 * (l >> 4);
-        const int kn = (kk + 32 < 512 ? kk + 32 : kk) + 8 * (l >> 4);   // next K step (the last reloads its own)
-        us8 nh[8], nl[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int k = 128 * cg + 16 * c + (l & 15);
-            nh[c] = *reinterpret_cast<const us8 *>(V.v_h + frag_off(k, kn));
-            nl[c] = *reinterpret_cast<const us8 *>(V.v_l + frag_off(k, kn));
-        }
-        us8 a[2], al[2];
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            a[r] = *reinterpret_cast<const us8 *>(&ch[32 * rg + 16 * r + (l & 15)][kc]);
-            al[r] = *reinterpret_cast<const us8 *>(&cl[32 * rg + 16 * r + (l & 15)][kc]);
-        }
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-#pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(a[r]), as_bf(bh[c]), acc[r][c], 0, 0, 0);
-                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(al[r]), as_bf(bh[c]), acc[r][c], 0, 0, 0);
-                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(a[r]), as_bf(bl[c]), acc[r][c], 0, 0, 0);
-            }
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            bh[c] = nh[c];
-            bl[c] = nl[c];
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const long long b = b0 + 32 * rg + 16 * r + 4 * (l >> 4) + e;
-            if (b >= nblk) continue;
-            const int bz = (int)(b % nb), bx = (int)((b / nb) % nb), by = (int)(b / ((long long)nb * nb));
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const int k = 128 * cg + 16 * c + (l & 15);
-                const int y = 8 * by + (k >> 6), x = 8 * bx + ((k >> 3) & 7), z = 8 * bz + (k & 7);
-                out[((long long)y * n + x) * n + z] = acc[r][c][e];
-            }
-        }
+    tile_f4 acc[4][4];
+    tile_mfma(ch, cl, V.v_h, V.v_l, w, l, acc);
+    __syncthreads();
+    tile_store(&lds[0][0][0], acc, w, l, b0, n, sk);
 }
 
 __global__ void zero_kernel(uint32_t *p, long long n)
@@ -472,12 +374,20 @@ hipError_t launch_pack(const Work &w, int n, float tol2, float flo, uint32_t *en
 }
 
 hipError_t launch_decode(const Basis &V, const uint32_t *ent, const uint32_t *boff, const uint16_t *bcnt, int n,
-                         float *out, hipStream_t s)
+                         const TileSink &sk, hipStream_t s)
 {
     const long long nb = n / 8, nblk = nb * nb * nb;
     hipLaunchKernelGGL(decode64_kernel, dim3((unsigned)((nblk + EB - 1) / EB)), dim3(64 * EW), 0, s, V, ent, boff, bcnt,
-                       n, out);
+                       n, sk);
     return hipGetLastError();
+}
+
+hipError_t launch_decode(const Basis &V, const uint32_t *ent, const uint32_t *boff, const uint16_t *bcnt, int n,
+                         float *out, hipStream_t s)
+{
+    TileSink sk;
+    sk.out = out;
+    return launch_decode(V, ent, boff, bcnt, n, sk, s);
 }
 
 int encode(const Basis &V, const float *P, long long off, long long sy, long long sx, int n, const Mask &m,
@@ -494,27 +404,44 @@ int encode_in(const Basis &V, const float *P, long long off, long long sy, long 
               float tol, float flo, const Work &w, uint32_t *ent, long long cap, uint32_t *boff, uint16_t *bcnt,
               long long *kept, double *err, double *energy, hipStream_t s)
 {
+    Counters h;
+    const int st = encode_start(V, P, off, sy, sx, n, m, tol, flo, w, ent, cap, boff, bcnt, nullptr, &h, s);
+    if (st) return st;
+    if (hipStreamSynchronize(s) != hipSuccess) return VTI_ERR_HIP;
+    encode_finish(h, cap, kept, err, energy);
+    return VTI_OK;
+}
+
+// a kept value is stored to 2^-15 of itself and the two transforms add
+// ~2^-16 of the block norm each: 2^-26 of every block's budget is set aside
+// for them
+static double rounding_share() { return std::ldexp(1.0, -26); }
+
+int encode_start(const Basis &V, const float *P, long long off, long long sy, long long sx, int n, const Mask &m,
+                 float tol, float flo, const Work &w, uint32_t *ent, long long cap, uint32_t *boff, uint16_t *bcnt,
+                 hipEvent_t read, Counters *hc, hipStream_t s)
+{
     const long long nb = n / 8;
-    if (n < 8 || n % 8 || !(tol >= 0.0f) || !(flo >= 0.0f) || !w.cnt || !w.coef || nb * nb * nb > w.cap_blocks)
+    if (n < 8 || n % 8 || !(tol >= 0.0f) || !(flo >= 0.0f) || !w.cnt || !w.coef || nb * nb * nb > w.cap_blocks || !hc)
         return VTI_ERR_ARG;
-    // a kept value is stored to 2^-15 of itself and the two transforms add
-    // ~2^-16 of the block norm each: 2^-26 of every block's budget is set
-    // aside for them
-    const double q2 = std::ldexp(1.0, -26);
+    const double q2 = rounding_share();
     const double t2 = (double)tol * tol;
     const float tol2 = t2 > q2 ? (float)(t2 - q2) : 0.0f;
-    if (launch_encode(V, P, off, sy, sx, n, m, w, s) != hipSuccess ||
-        launch_pack(w, n, tol2, flo, ent, cap, boff, bcnt, s) != hipSuccess)
+    if (launch_encode(V, P, off, sy, sx, n, m, w, s) != hipSuccess) return VTI_ERR_HIP;
+    if (read && hipEventRecord(read, s) != hipSuccess) return VTI_ERR_HIP;
+    if (launch_pack(w, n, tol2, flo, ent, cap, boff, bcnt, s) != hipSuccess ||
+        hipMemcpyAsync(hc, w.cnt, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess)
         return VTI_ERR_HIP;
-    Counters h;
-    if (hipMemcpyAsync(&h, w.cnt, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-        return VTI_ERR_HIP;
+    return VTI_OK;
+}
+
+void encode_finish(const Counters &h, long long cap, long long *kept, double *err, double *energy)
+{
+    const double q2 = rounding_share();
     const long long k = (long long)h.cursor;
     if (kept) *kept = (h.overflow && k <= cap) ? cap + 1 : k;
     if (energy) *energy = h.energy;
     if (err) *err = h.energy > 0.0 ? std::sqrt(std::min(1.0, (h.dropped + q2 * (h.energy - h.dropped)) / h.energy)) : 0.0;
-    return VTI_OK;
 }
 
 }  // namespace bc

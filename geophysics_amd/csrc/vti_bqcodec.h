@@ -21,9 +21,19 @@ constexpr int kMaxBlockWords = 19 + 512;
 int encode(const bc::Basis &V, const float *P, long long off, long long sy, long long sx, int n,
            const bc::Mask &m, float tol, float flo, bc::Work &w, uint32_t *ent, long long cap, uint32_t *boff,
            uint16_t *bcnt, long long *kept, double *err, hipStream_t s);
-// Dense n^3 box (z fastest) from the block streams.
+// encode without the host sync, as bc::encode_start: `read` (when not null)
+// recorded once the snapshot has been read, the counters copied into *hc
+// (pinned host memory), bc::encode_finish (energy unused) once the stream
+// has passed the copy.
+int encode_start(const bc::Basis &V, const float *P, long long off, long long sy, long long sx, int n,
+                 const bc::Mask &m, float tol, float flo, bc::Work &w, uint32_t *ent, long long cap, uint32_t *boff,
+                 uint16_t *bcnt, hipEvent_t read, bc::Counters *hc, hipStream_t s);
+// Dense n^3 box (z fastest) from the block streams, or the decoded box into
+// the sink (a dense box, or an image accumulation).
 hipError_t launch_decode(const bc::Basis &V, const uint32_t *ent, const uint32_t *boff, const uint16_t *bcnt, int n,
                          float *out, hipStream_t s);
+hipError_t launch_decode(const bc::Basis &V, const uint32_t *ent, const uint32_t *boff, const uint16_t *bcnt, int n,
+                         const bc::TileSink &sk, hipStream_t s);
 
 }  // namespace bq
 }  // namespace vti

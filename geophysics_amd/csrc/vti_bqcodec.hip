@@ -56,7 +56,6 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef unsigned short us8 __attribute__((ext_vector_type(8)));
 
-constexpr int LROW = 512 + 8;
 constexpr int PW = 8;                 // blocks (wavefronts) per pack workgroup
 constexpr int kHead = 19;             // header + bitmap words
 
@@ -441,8 +440,8 @@ __global__ __launch_bounds__(64 * PW) void qwrite_kernel(const float *__restrict
 // piece, the basis operands loaded one K step ahead.  Wavefront w parses
 // blocks w, w + 8, .. and computes blocks 32 (w & 1) .. + 32 against voxels
 // 128 (w >> 1) .. + 128.
-constexpr int DB = 64;   // blocks per workgroup
-constexpr int DW = 8;    // wavefronts per workgroup
+constexpr int DB = bc::kTileBlocks;   // blocks per workgroup
+constexpr int DW = bc::kTileWaves;    // wavefronts per workgroup
 
 __device__ __forceinline__ us8 pack8(const uint16_t h[8])
 {
@@ -454,11 +453,11 @@ __device__ __forceinline__ us8 pack8(const uint16_t h[8])
 
 __global__ __launch_bounds__(64 * DW) void qdecode64_kernel(bc::Basis V, const uint32_t *__restrict__ ent,
                                                            const uint32_t *__restrict__ boff,
-                                                           const uint16_t *__restrict__ bcnt, int n,
-                                                           float *__restrict__ out)
+                                                           const uint16_t *__restrict__ bcnt, int n, bc::TileSink sk)
 {
-    __shared__ __align__(16) uint16_t ch[DB][LROW];
-    __shared__ __align__(16) uint16_t cl[DB][LROW];
+    __shared__ __align__(16) uint16_t lds[2][DB][bc::LROW];   // coefficient tiles (hi, lo), then the output tile
+    uint16_t(*ch)[bc::LROW] = lds[0];
+    uint16_t(*cl)[bc::LROW] = lds[1];
     const int nb = n / 8;
     const long long nblk = (long long)nb * nb * nb;
     const long long b0 = (long long)blockIdx.x * DB;
@@ -508,63 +507,10 @@ __global__ __launch_bounds__(64 * DW) void qdecode64_kernel(bc::Basis V, const u
         *reinterpret_cast<us8 *>(&cl[i][8 * l]) = pack8(ll);
     }
     __syncthreads();
-    const int rg = w & 1, cg = w >> 1;
-    f4 acc[2][8];
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int c = 0; c < 8; ++c) acc[r][c] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    us8 bh[8], bl[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const int k = 128 * cg + 16 * c + (l & 15);
-        bh[c] = *reinterpret_cast<const us8 *>(V.v_h + bc::frag_off(k, 8 * (l >> 4)));
-        bl[c] = *reinterpret_cast<const us8 *>(V.v_l + bc::frag_off(k, 8 * (l >> 4)));
-    }
-    for (int kk = 0; kk < 512; kk += 32) {
-        const int kc = kk + 8 * (l >> 4);
-        const int kn = (kk + 32 < 512 ? kk + 32 : kk) + 8 * (l >> 4);   // next K step (the last reloads its own)
-        us8 nh[8], nl[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int k = 128 * cg + 16 * c + (l & 15);
-            nh[c] = *reinterpret_cast<const us8 *>(V.v_h + bc::frag_off(k, kn));
-            nl[c] = *reinterpret_cast<const us8 *>(V.v_l + bc::frag_off(k, kn));
-        }
-        us8 a[2], al[2];
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            a[r] = *reinterpret_cast<const us8 *>(&ch[32 * rg + 16 * r + (l & 15)][kc]);
-            al[r] = *reinterpret_cast<const us8 *>(&cl[32 * rg + 16 * r + (l & 15)][kc]);
-        }
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-#pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(a[r]), as_bf(bh[c]), acc[r][c], 0, 0, 0);
-                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(al[r]), as_bf(bh[c]), acc[r][c], 0, 0, 0);
-                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(a[r]), as_bf(bl[c]), acc[r][c], 0, 0, 0);
-            }
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            bh[c] = nh[c];
-            bl[c] = nl[c];
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const long long b = b0 + 32 * rg + 16 * r + 4 * (l >> 4) + e;
-            if (b >= nblk) continue;
-            const int bz = (int)(b % nb), bx = (int)((b / nb) % nb), by = (int)(b / ((long long)nb * nb));
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const int k = 128 * cg + 16 * c + (l & 15);
-                const int y = 8 * by + (k >> 6), x = 8 * bx + ((k >> 3) & 7), z = 8 * bz + (k & 7);
-                out[((long long)y * n + x) * n + z] = acc[r][c][e];
-            }
-        }
+    bc::tile_f4 acc[4][4];
+    bc::tile_mfma(ch, cl, V.v_h, V.v_l, w, l, acc);
+    __syncthreads();
+    bc::tile_store(&lds[0][0][0], acc, w, l, b0, n, sk);
 }
 
 }  // namespace
@@ -573,7 +519,19 @@ int encode(const bc::Basis &V, const float *P, long long off, long long sy, long
            const bc::Mask &m, float tol, float flo, bc::Work &w, uint32_t *ent, long long cap, uint32_t *boff,
            uint16_t *bcnt, long long *kept, double *err, hipStream_t s)
 {
-    if (n < 8 || n % 8 || !(tol >= 0.0f) || !(flo >= 0.0f)) return VTI_ERR_ARG;
+    bc::Counters h;
+    const int st = encode_start(V, P, off, sy, sx, n, m, tol, flo, w, ent, cap, boff, bcnt, nullptr, &h, s);
+    if (st) return st;
+    if (hipStreamSynchronize(s) != hipSuccess) return VTI_ERR_HIP;
+    bc::encode_finish(h, cap, kept, err, nullptr);
+    return VTI_OK;
+}
+
+int encode_start(const bc::Basis &V, const float *P, long long off, long long sy, long long sx, int n,
+                 const bc::Mask &m, float tol, float flo, bc::Work &w, uint32_t *ent, long long cap, uint32_t *boff,
+                 uint16_t *bcnt, hipEvent_t read, bc::Counters *hc, hipStream_t s)
+{
+    if (n < 8 || n % 8 || !(tol >= 0.0f) || !(flo >= 0.0f) || !hc) return VTI_ERR_ARG;
     const long long nb = n / 8, nblk = nb * nb * nb;
     if (nblk * kMaxBlockWords > 0xffffffffLL) return VTI_ERR_ARG;   // 32-bit word offsets
     int st = bc::work_reserve(w, n);
@@ -591,11 +549,13 @@ int encode(const bc::Basis &V, const float *P, long long off, long long sy, long
     }
     // the rounding of d q into bf16 hi + lo (2^-16 of each value) and the
     // two transforms (~2^-16 of the block norm each): 2^-26 of every
-    // block's budget is set aside for them, as in codec 1
+    // block's budget is set aside for them, as in codec 1 (whose
+    // encode_finish turns the counters into the error bound)
     const double q2 = std::ldexp(1.0, -26);
     const double t2 = (double)tol * tol;
     const double tol2 = t2 > q2 ? t2 - q2 : 0.0;
     if (bc::launch_encode(V, P, off, sy, sx, n, m, w, s) != hipSuccess) return VTI_ERR_HIP;
+    if (read && hipEventRecord(read, s) != hipSuccess) return VTI_ERR_HIP;
     const unsigned long long c = (unsigned long long)std::max(0LL, std::min(cap, nblk * (long long)kMaxBlockWords));
     const Aux ax = aux_view(w.aux, nblk);
     const long long ng = (nblk + kScanGroup - 1) / kScanGroup;
@@ -604,24 +564,27 @@ int encode(const bc::Basis &V, const float *P, long long off, long long sy, long
     hipLaunchKernelGGL(qscan_kernel, dim3((unsigned)ng), dim3(1024), 0, s, ax, nblk, boff);
     hipLaunchKernelGGL(qtop_kernel, dim3(1), dim3(1024), 0, s, ax, ng, c, w.cnt);
     hipLaunchKernelGGL(qwrite_kernel, gb, dim3(64 * PW), 0, s, w.coef, nblk, ax, ent, c, boff, bcnt);
-    if (hipGetLastError() != hipSuccess) return VTI_ERR_HIP;
-    bc::Counters h;
-    if (hipMemcpyAsync(&h, w.cnt, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(hc, w.cnt, sizeof(bc::Counters), hipMemcpyDeviceToHost, s) != hipSuccess)
         return VTI_ERR_HIP;
-    const long long k = (long long)h.cursor;
-    if (kept) *kept = (h.overflow && k <= cap) ? cap + 1 : k;
-    if (err) *err = h.energy > 0.0 ? std::sqrt(std::min(1.0, (h.dropped + q2 * (h.energy - h.dropped)) / h.energy)) : 0.0;
     return VTI_OK;
+}
+
+hipError_t launch_decode(const bc::Basis &V, const uint32_t *ent, const uint32_t *boff, const uint16_t *bcnt, int n,
+                         const bc::TileSink &sk, hipStream_t s)
+{
+    const long long nb = n / 8, nblk = nb * nb * nb;
+    hipLaunchKernelGGL(qdecode64_kernel, dim3((unsigned)((nblk + DB - 1) / DB)), dim3(64 * DW), 0, s, V, ent, boff,
+                       bcnt, n, sk);
+    return hipGetLastError();
 }
 
 hipError_t launch_decode(const bc::Basis &V, const uint32_t *ent, const uint32_t *boff, const uint16_t *bcnt, int n,
                          float *out, hipStream_t s)
 {
-    const long long nb = n / 8, nblk = nb * nb * nb;
-    hipLaunchKernelGGL(qdecode64_kernel, dim3((unsigned)((nblk + DB - 1) / DB)), dim3(64 * DW), 0, s, V, ent, boff,
-                       bcnt, n, out);
-    return hipGetLastError();
+    bc::TileSink sk;
+    sk.out = out;
+    return bq::launch_decode(V, ent, boff, bcnt, n, sk, s);
 }
 
 }  // namespace bq

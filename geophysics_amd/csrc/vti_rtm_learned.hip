@@ -194,6 +194,35 @@ struct EvRing {
     }
 };
 
+// A stream of the codec's own (default priority) with the events that order
+// it against the shot's stream, and pinned host slots for the encodes'
+// counters.  The destructor waits for the stream before anything it used is
+// freed, so it is declared after the buffers it works on.
+struct CodecStream {
+    hipStream_t s = nullptr;
+    hipEvent_t p_ready = nullptr;     // the shot stream has finished the snapshot's stress update
+    hipEvent_t p_read = nullptr;      // the encode has read the snapshot
+    hipEvent_t done[2] = {};          // encode counters copied (per slot) / decoded snapshot imaged ([0])
+    bc::Counters *hc = nullptr;       // [2], pinned
+    bool ok = false;
+    CodecStream()
+    {
+        ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&p_ready, hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&p_read, hipEventDisableTiming) == hipSuccess;
+        for (int i = 0; i < 2 && ok; ++i) ok = hipEventCreateWithFlags(&done[i], hipEventDisableTiming) == hipSuccess;
+        ok = ok && hipHostMalloc(reinterpret_cast<void **>(&hc), 2 * sizeof(bc::Counters), hipHostMallocDefault) == hipSuccess;
+    }
+    ~CodecStream()
+    {
+        if (s) (void)hipStreamSynchronize(s);
+        for (hipEvent_t e : {p_ready, p_read, done[0], done[1]})
+            if (e) (void)hipEventDestroy(e);
+        if (hc) (void)hipHostFree(hc);
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -245,7 +274,7 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
     Buf lat, scl, dense, dense2, acc, exa, boxes, boffs, bcnts;
     if ((!sparse && (hipMalloc(&lat.p, (size_t)nsave * lat_n * sizeof(uint16_t)) != hipSuccess ||
                      hipMalloc(&scl.p, (size_t)nsave * 64) != hipSuccess)) ||
-        hipMalloc(&dense.p, (size_t)n * n * n * sizeof(float)) != hipSuccess ||
+        (!sparse && hipMalloc(&dense.p, (size_t)n * n * n * sizeof(float)) != hipSuccess) ||
         hipMalloc(&acc.p, 2 * sizeof(double)) != hipSuccess)
         return fail(VTI_ERR_ALLOC, "vti_rtm_shot_learned: latent store");
     // codec 1 / 2 store: words (entries, or quantized block streams; 4 bytes
@@ -295,7 +324,7 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
     }
     const long long box_n = bx.size();
     if ((box_n && hipMalloc(&boxes.p, (size_t)nsave * box_n * sizeof(float)) != hipSuccess) ||
-        hipMalloc(&dense2.p, nn3 * sizeof(float)) != hipSuccess)
+        (!sparse && hipMalloc(&dense2.p, nn3 * sizeof(float)) != hipSuccess))
         return fail(VTI_ERR_ALLOC, "vti_rtm_shot_learned: source boxes");
     float *BX = static_cast<float *>(boxes.p);
     float *D2 = static_cast<float *>(dense2.p);
@@ -323,10 +352,48 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
     const int clamp = o->illum_mode == 0;
     auto hip = [](hipError_t e, const char *w) { return e == hipSuccess ? VTI_OK : fail(VTI_ERR_HIP, w); };
     std::vector<char> bad(nsave, 0);
-    EvRing t_enc, t_dec;   // the codec's GPU time (encode: up to its host synchronisation)
+    EvRing t_enc, t_dec;   // the codec's GPU time (encode: up to its counters' copy to the host)
     if (!t_enc.ok || !t_dec.ok) return fail(VTI_ERR_HIP, "vti_rtm_shot_learned: events");
     double err_sum = 0.0;
     int nchecked = 0, nbad = 0;
+    // Codecs 1, 2: the codec runs on a stream of its own.  A snapshot's
+    // encode waits for the stress update that made it and overlaps the next
+    // velocity update; the next stress update (the first write of P) waits
+    // for the encode's projection.  Its counters are read on the host one
+    // snapshot later (the next encode's words start where its words end).
+    // In the reverse sweep a snapshot's decode, fused with its image, overlaps
+    // the receiver side's next velocity update.  serial_codec puts it all on
+    // the shot's stream (the same results).
+    const bool side = sparse && !lo->serial_codec;
+    CodecStream cs;
+    if (sparse && !cs.ok) return fail(VTI_ERR_HIP, "vti_rtm_shot_learned: codec stream");
+    hipStream_t es = side ? cs.s : s;   // the codec's stream
+    bc::Counters *hcs = cs.hc;
+    int pend = -1;                      // snapshot whose encode has not been settled on the host
+    long long pend_cap = 0;
+    bool wait_read = false;             // the next stress update waits for an encode's projection
+    auto settle = [&]() -> int {        // the host side of the pending encode: place its words
+        if (pend < 0) return VTI_OK;
+        const int j = pend;
+        pend = -1;
+        if (hipEventSynchronize(cs.done[j & 1]) != hipSuccess) return fail(VTI_ERR_HIP, "vti_rtm_shot_learned: encode");
+        long long need = 0;
+        double e = 0.0;
+        bc::encode_finish(hcs[j & 1], pend_cap, &need, &e, nullptr);
+        if (need > pend_cap) {   // over budget: recompute this snapshot from a checkpoint
+            if (!nck) return fail(VTI_ERR_ALLOC, "vti_rtm_shot_learned: sparse store budget exhausted (set ck_every)");
+            bad[j] = 1;
+            ++nbad;
+        } else {
+            ebase[j] = cur;
+            cur += need;
+            chunk_room -= need;
+            used += need;
+            err_sum += e;
+            ++nchecked;
+        }
+        return VTI_OK;
+    };
 
     // ---- first sweep
     if ((st = hip(state_zero(c, W), "zero"))) return st;
@@ -334,16 +401,24 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         if (nck && it % lo->ck_every == 0 && (st = hip(state_copy(c, ck[it / lo->ck_every], W), "checkpoint")))
             return st;
         if (launch_vel(3, vel_args_for(c, W, it, true), s) != hipSuccess) return fail(VTI_ERR_HIP, "vel");
+        if (wait_read) {
+            if ((st = hip(hipStreamWaitEvent(s, cs.p_read, 0), "wait"))) return st;
+            wait_read = false;
+        }
         float *row = c->rec + (size_t)it * nrec;
         if (launch_stress(3, with_illum(c, src_stress_args(c, W, it, svf, row), clamp), s) != hipSuccess)
             return fail(VTI_ERR_HIP, "stress");
         if (it % se) continue;
         const int j = it / se;
         const float *P = W.f[VTI_F_P];
-        if ((st = hip(t_enc.begin(s), "event"))) return st;
         if (sparse) {   // error-bounded sparse coefficients; the source box exactly
+            if ((st = settle())) return st;
+            if (side && ((st = hip(hipEventRecord(cs.p_ready, s), "event")) ||
+                         (st = hip(hipStreamWaitEvent(es, cs.p_ready, 0), "wait"))))
+                return st;
+            if ((st = hip(t_enc.begin(es), "event"))) return st;
             if (box_n)
-                hipLaunchKernelGGL(box_save_kernel, dim3(64), dim3(256), 0, s, c->g, np, P, bx, BX + (size_t)j * box_n);
+                hipLaunchKernelGGL(box_save_kernel, dim3(64), dim3(256), 0, es, c->g, np, P, bx, BX + (size_t)j * box_n);
             const bc::Mask m{bx.y0, bx.x0, bx.z0, bx.by, bx.bx, bx.bz};
             if (chunk_room < snap_max) {   // a new chunk: 4 GB, or what the budget has left
                 const long long take = std::min(chunk_ents, budget / 4 - alloc_ents);
@@ -359,28 +434,20 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
                     }
                 }
             }
-            long long need = 0;
-            double e = 0.0;
-            st = quant ? ae_bq_encode(A, P, off, c->g.ldxy, c->g.ldz, n, m, lo->tol, lo->floor, cur, chunk_room,
-                                      BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, &need, &e, s)
-                       : ae_bc_encode(A, P, off, c->g.ldxy, c->g.ldz, n, m, lo->tol, lo->floor, cur, chunk_room,
-                                      BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, &need, &e, nullptr, s);
+            hipEvent_t rd = side ? cs.p_read : nullptr;
+            st = quant ? ae_bq_encode_start(A, P, off, c->g.ldxy, c->g.ldz, n, m, lo->tol, lo->floor, cur, chunk_room,
+                                            BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, rd, hcs + (j & 1), es)
+                       : ae_bc_encode_start(A, P, off, c->g.ldxy, c->g.ldz, n, m, lo->tol, lo->floor, cur, chunk_room,
+                                            BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, rd, hcs + (j & 1), es);
             if (st) return st;
-            if ((st = hip(t_enc.end(s), "event"))) return st;
-            if (need > chunk_room) {   // over budget: recompute this snapshot from a checkpoint
-                if (!nck) return fail(VTI_ERR_ALLOC, "vti_rtm_shot_learned: sparse store budget exhausted (set ck_every)");
-                bad[j] = 1;
-                ++nbad;
-            } else {
-                ebase[j] = cur;
-                cur += need;
-                chunk_room -= need;
-                used += need;
-                err_sum += e;
-                ++nchecked;
-            }
+            if ((st = hip(t_enc.end(es), "event"))) return st;
+            if ((st = hip(hipEventRecord(cs.done[j & 1], es), "event"))) return st;
+            pend = j;
+            pend_cap = chunk_room;
+            wait_read = side;
             continue;
         }
+        if ((st = hip(t_enc.begin(s), "event"))) return st;
         if (box_n) {   // the source box is kept exactly; the autoencoder sees the rest
             hipLaunchKernelGGL(box_save_kernel, dim3(64), dim3(256), 0, s, c->g, np, P, bx, BX + (size_t)j * box_n);
             hipLaunchKernelGGL(masked_interior_kernel, dim3(2048), dim3(256), 0, s, c->g, np, n, P, bx, D);
@@ -418,6 +485,7 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
             }
         }
     }
+    if ((st = settle())) return st;
     // ---- the record: observed or modelled + mute (as vti_rtm_shot)
     if (obs) {
         if (hipMemcpyAsync(c->rec_t, obs, (size_t)nrec * nt * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess)
@@ -428,10 +496,25 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         if ((st = vti_mute3d_window(c, sh.sx - np + 1, 0, sh.sy - np + 1, 0, sh.sz - np + 1, 0, o->mute_tt))) return st;
     }
     // ---- reverse sweep
+    // codecs 1, 2: snapshot j decoded straight into the image (times the
+    // receiver field of its step, the exact box included); overlapped, on
+    // the codec stream while the receiver side runs the next velocity update,
+    // and the next write of the receiver P (its injection row, when that lies
+    // in the interior, else the stress update) waits for it
+    const bool inj_interior = inj_z >= np && inj_z < np + n;
+    bool wait_img = false;
     if ((st = hip(state_zero(c, G), "zero"))) return st;
     for (int it = nt - 1; it >= 0; --it) {
+        if (wait_img && inj_interior) {
+            if ((st = hip(hipStreamWaitEvent(s, cs.done[0], 0), "wait"))) return st;
+            wait_img = false;
+        }
         if (launch_inject_rec(c, G, c->rec + (size_t)it * nrec, inj_z) != hipSuccess) return fail(VTI_ERR_HIP, "inject");
         if (launch_vel(3, vel_args_for(c, G, it, false), s) != hipSuccess) return fail(VTI_ERR_HIP, "vel");
+        if (wait_img) {
+            if ((st = hip(hipStreamWaitEvent(s, cs.done[0], 0), "wait"))) return st;
+            wait_img = false;
+        }
         StressArgs ra = rcv_stress_args(c, G, it, svb, inj_z);
         if (o->illum_mode == 1) ra = with_illum(c, ra, 0);   // receiver illumination in the epilogue
         if (launch_stress(3, ra, s) != hipSuccess) return fail(VTI_ERR_HIP, "stress");
@@ -451,12 +534,32 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
                     return fail(VTI_ERR_HIP, "stress");
             }
             if ((st = hip(launch_image(c, W.f[VTI_F_P], G.f[VTI_F_P], c->img), "image"))) return st;
-        } else {
-            if ((st = hip(t_dec.begin(s), "event"))) return st;
-            if (quant) st = ae_bq_decode(A, ebase[j], BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, n, D, s);
-            else if (sparse) st = ae_bc_decode(A, ebase[j], BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, n, D, s);
-            else st = ae_decode(A, L + (size_t)j * lat_n, Sc + (size_t)j * 16, n, D, s);
+        } else if (sparse) {
+            bc::TileSink sk;
+            sk.img = c->img;
+            sk.rp = G.f[VTI_F_P];
+            sk.off = off;
+            sk.sy = c->g.ldxy;
+            sk.sx = c->g.ldz;
+            if (box_n) {
+                sk.box = BX + (size_t)j * box_n;
+                sk.y0 = bx.y0, sk.x0 = bx.x0, sk.z0 = bx.z0, sk.by = bx.by, sk.bx = bx.bx, sk.bz = bx.bz;
+            }
+            if (side && ((st = hip(hipEventRecord(cs.p_ready, s), "event")) ||
+                         (st = hip(hipStreamWaitEvent(es, cs.p_ready, 0), "wait"))))
+                return st;
+            if ((st = hip(t_dec.begin(es), "event"))) return st;
+            st = quant ? ae_bq_decode_sink(A, ebase[j], BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, n, sk, es)
+                       : ae_bc_decode_sink(A, ebase[j], BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, n, sk, es);
             if (st) return st;
+            if ((st = hip(t_dec.end(es), "event"))) return st;
+            if (side) {
+                if ((st = hip(hipEventRecord(cs.done[0], es), "event"))) return st;
+                wait_img = true;
+            }
+        } else {   // latents: decoded into the dense buffer, then imaged
+            if ((st = hip(t_dec.begin(s), "event"))) return st;
+            if ((st = ae_decode(A, L + (size_t)j * lat_n, Sc + (size_t)j * 16, n, D, s))) return st;
             if (box_n)
                 hipLaunchKernelGGL(box_restore_kernel, dim3(64), dim3(256), 0, s, n, bx, BX + (size_t)j * box_n, D);
             if ((st = hip(t_dec.end(s), "event"))) return st;
@@ -465,7 +568,7 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         if ((st = hip(hipGetLastError(), "image"))) return st;
     }
     c->rtm_illum_mode = o->illum_mode;
-    if ((st = hip(hipStreamSynchronize(s), "sync"))) return st;
+    if ((st = hip(hipStreamSynchronize(s), "sync")) || (st = hip(hipStreamSynchronize(es), "sync"))) return st;
     if (stats) {
         stats[0] = t_enc.total();
         stats[1] = t_dec.total();
@@ -480,11 +583,12 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         stats[7] = nrec_snap;
         // device memory held for the stored source field, as allocated: the
         // store (entry chunks and block tables, or latents, scales and exact
-        // slots), the source boxes, the two dense n^3 work buffers and the
-        // fallback checkpoints
+        // slots), the source boxes, the work buffers (the codec's
+        // coefficients and search results, or the latents' two dense n^3
+        // buffers) and the fallback checkpoints
         const double ckb = (double)nck * (double)vti_state_bytes(c);
         stats[8] = sparse ? (double)alloc_ents * 4 + (double)nsave * nblk * 6 + (double)nsave * box_n * 4 +
-                                2.0 * nn3 * 4 + ckb
+                                (double)ae_codec_work_bytes(A) + ckb
                           : (double)nsave * lat_n * 2 + (double)nsave * 64 + (double)lo->max_exact * nn3 * 4 +
                                 (double)nsave * box_n * 4 + 2.0 * nn3 * 4 + ckb;
     }

@@ -62,7 +62,7 @@ _lib = None
 class LearnedOpts(C.Structure):
     _fields_ = [("save_every", C.c_int), ("check_every", C.c_int), ("tol", C.c_float), ("ck_every", C.c_int),
                 ("max_exact", C.c_int), ("src_box", C.c_int), ("codec", C.c_int), ("floor", C.c_float),
-                ("budget", C.c_longlong)]
+                ("budget", C.c_longlong), ("serial_codec", C.c_int)]
 
 
 class SuGeometry(C.Structure):

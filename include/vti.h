@@ -456,20 +456,26 @@ typedef struct {
     long long budget;  /* codecs 1, 2: bytes for the coded snapshots (0: nsnap * n^3, a quarter
                           of the fp32 snapshots); past it, snapshots are recomputed from the
                           ck_every checkpoints (or the call fails without them) */
+    int serial_codec;  /* codecs 1, 2: 0 runs the codec on a stream of its own, each encode
+                          overlapping the next velocity update and each decode the receiver
+                          steps before its image; 1 runs it on the shot's stream between the
+                          steps.  The results are the same. */
 } vti_learned_opts;
 int vti_learned_defaults(vti_learned_opts *o);
 /* stats (optional, 9 doubles): encode ms, decode ms (GPU time between
- * HIP events around the codec's launches; the encode's up to the host
- * synchronisation it needs to place the entries), mean
+ * HIP events around the codec's launches on its stream; the encode's up to
+ * the copy of its counters to the host, which places the entries), mean
  * checked relative error (codecs 1, 2: mean error bound of the stored
  * snapshots), snapshots above tol (codecs 1, 2: over budget), snapshots,
  * snapshots kept exactly, bytes of the compressed store (latents or
  * entries + block tables, exactly kept snapshots, source boxes), snapshots
  * recomputed from checkpoints, and the device bytes the call held for the
  * stored source field as allocated (the store, its sparse entries in 4 GB
- * chunks taken as needed up to the budget, source boxes, two dense n^3 work
- * buffers, fallback checkpoints): the memory a plain checkpointed RTM would
- * be given for a comparison at equal memory. */
+ * chunks taken as needed up to the budget, source boxes, the work buffers:
+ * the codec's coefficients and search results for codecs 1, 2, two dense
+ * n^3 buffers for the latents; fallback checkpoints): the memory a plain
+ * checkpointed RTM would be given for a comparison at equal memory.  Codecs
+ * 1, 2 decode each snapshot straight into the image. */
 int vti_rtm_shot_learned(vti_ctx *ctx, vti_ae *ae, const vti_rtm_opts *opts,
                          const vti_learned_opts *lo, const float *obs, double *stats);
 /* Forward modelling of the current shot; every save_every-th snapshot, the

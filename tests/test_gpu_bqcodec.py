@@ -202,6 +202,28 @@ def test_zero_box_and_capacity():
         parse_block(ent2[boff2[b]:ends[b]])
 
 
+def test_decode_into_unaligned_output():
+    """The decode's 16-byte line stores fall back to single stores for an
+    output that is not 16-byte aligned: the same values, nothing written
+    around the box (24^3: 27 blocks, a workgroup partly idle)."""
+    import torch
+    V = _orthonormal(9)
+    n = 24
+    big = np.random.default_rng(4).standard_normal((n + 2, n + 3, n + 1)).astype(np.float32)
+    with vti.Autoencoder(crop=16, batch=1, seed=1) as A:
+        A.set_basis(V)
+        kept, err, out, ent, boff, bcnt = _codec(A, torch, big, n, 0.02, 27 * MAXW, off_yxz=(1, 2, 0))
+        dev = torch.device("cuda:0")
+        e, bo, bc_ = (torch.from_numpy(a.view(t)).to(dev) for a, t in ((ent, np.int32), (boff, np.int32), (bcnt, np.int16)))
+        buf = torch.full((n ** 3 + 8,), np.nan, dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+        A.bq_decode(e.data_ptr(), bo.data_ptr(), bc_.data_ptr(), n, buf.data_ptr() + 4)   # one float past 16-byte alignment
+        torch.cuda.synchronize()
+        got = buf.cpu().numpy()
+    assert np.isnan(got[0]) and np.all(np.isnan(got[1 + n ** 3:]))
+    assert np.array_equal(got[1:1 + n ** 3].reshape(n, n, n), out)
+
+
 @pytest.mark.parametrize("n", [24, 8])
 def test_block_counts_off_the_pack_workgroup(n):
     """27 and 1 blocks: the pack's last workgroup has idle wavefronts.  The
@@ -283,6 +305,50 @@ def test_quantized_rtm_image_and_store_against_sparse():
         assert res[tol, 2][2]["store_bytes"] < 0.8 * res[tol, 1][2]["store_bytes"]
     assert st["recomputed"] > 0
     assert np.array_equal(img_r, img_p), rel_l2(img_r, img_p)
+
+
+def test_codec_stream_overlap_equals_serial():
+    """The codec on a stream of its own (encodes overlapping the next
+    velocity update, decodes prefetched into two dense buffers while the
+    receiver side runs) gives the image, illumination and store of the codec
+    run between the steps on the shot's stream, bit for bit: codecs 1 and 2
+    with room for every snapshot, and codec 2 with a budget that stores some
+    snapshots and recomputes the rest from checkpoints (the reverse sweep
+    mixing decoded and recomputed snapshots)."""
+    cfg, vp, e, d = _setup()
+    keys = ("mean_checked_err", "over_tol", "snapshots", "exact", "store_bytes", "recomputed", "alloc_bytes")
+    nsave, nblk = NT // 4, (N // 8) ** 3
+    with vti.Propagator(cfg, vp, e, d) as prop, vti.Autoencoder(crop=32, batch=1, seed=3) as A:
+        for cx, cy in ((10, 12), (22, 20)):
+            prop.set_shot(_shot(cfg, cx, cy))
+            prop.ae_fit_shot(A, save_every=4, nblocks=64)
+        A.fit_end()
+        prop.set_shot(_shot(cfg, 16, 16))
+
+        def run(codec, budget, ck, box, serial):
+            lo = vti.learned_defaults()
+            lo.codec, lo.tol, lo.budget, lo.ck_every, lo.src_box, lo.serial_codec = codec, 0.01, budget, ck, box, serial
+            prop.rtm_reset()
+            st = prop.rtm_shot_learned(A, prop.rtm_defaults(), lo, None)
+            img, ill = prop.rtm_image(finalize=False)
+            return img, ill, st
+
+        full = nsave * vti.Autoencoder.bq_max_words(N) * 4
+        # the words of codec 2 without a source box: the store less its 6-byte block tables
+        words = run(2, full, 0, 0, 1)[2]["store_bytes"] - nsave * nblk * 6
+        for codec, budget, ck, box in ((1, full, 0, 8), (2, full, 0, 8), (2, int(0.4 * words), 16, 0)):
+            i1, l1, s1 = run(codec, budget, ck, box, 1)
+            i0, l0, s0 = run(codec, budget, ck, box, 0)
+            _say(f"codec {codec}, budget {budget}, box {box}: serial {s1}, overlapped {s0}")
+            assert np.abs(i1).max() > 0
+            assert np.array_equal(i0, i1), rel_l2(i0, i1)
+            assert np.array_equal(l0, l1)
+            for k in keys:
+                assert s0[k] == s1[k], k
+            if budget == full:
+                assert s1["recomputed"] == 0
+            else:
+                assert 0 < s1["recomputed"] < s1["snapshots"]
 
 
 def test_fitted_basis_needs_fewer_words_than_untrained_bases():
