@@ -837,6 +837,30 @@ int vti_bq_decode(vti_ae *A, const unsigned *ent, const unsigned *boff, const un
     return VTI_OK;
 }
 
+int vti_codec_decode_image(vti_ae *A, int codec, const unsigned *ent, const unsigned *boff, const unsigned short *bcnt,
+                           int n, float *img, const float *rp, long long off, long long sy, long long sx,
+                           const float *box, const int *bg)
+{
+    if (!A || !ent || !boff || !bcnt || !img || !rp || n < 8 || n % 8 || (codec != 1 && codec != 2) ||
+        (box && (!bg || bg[3] < 0 || bg[4] < 0 || bg[5] < 0)))
+        return aefail(VTI_ERR_ARG, "vti_codec_decode_image: bad arguments");
+    bc::TileSink sk;
+    sk.img = img;
+    sk.rp = rp;
+    sk.off = off;
+    sk.sy = sy;
+    sk.sx = sx;
+    if (box) {
+        sk.box = box;
+        sk.y0 = bg[0], sk.x0 = bg[1], sk.z0 = bg[2], sk.by = bg[3], sk.bx = bg[4], sk.bz = bg[5];
+    }
+    const int st = codec == 2 ? ae_bq_decode_sink(A, ent, boff, bcnt, n, sk, A->stream)
+                              : ae_bc_decode_sink(A, ent, boff, bcnt, n, sk, A->stream);
+    if (st) return st;
+    AE_HIP(hipStreamSynchronize(A->stream));
+    return VTI_OK;
+}
+
 long long vti_bq_max_words(int n)
 {
     if (n < 8 || n % 8) return 0;

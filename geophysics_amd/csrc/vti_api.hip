@@ -194,6 +194,16 @@ StressArgs with_illum(vti_ctx *c, StressArgs a, int clamp)
     return a;
 }
 
+StressArgs with_snap(vti_ctx *c, StressArgs a, float *snap)
+{
+    a.snap = snap;
+    a.inp = c->cfg.npml;
+    a.inx = c->cfg.nx;
+    a.iny = c->cfg.ndim == 3 ? c->cfg.ny : 1;
+    a.inz = c->cfg.nz;
+    return a;
+}
+
 StressArgs rcv_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, int inj_z)
 {
     StressArgs a = stress_args_for(c, s, it, false, sv, nullptr);

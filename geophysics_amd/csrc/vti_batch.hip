@@ -62,8 +62,25 @@ __device__ __forceinline__ int find_shot(const BShot *S, int ns, int t)
     return lo;
 }
 
-// which = 0: source state f, 1: receiver state g
-template <int TZ, int TXB, int WHICH>
+__device__ __forceinline__ void compiler_fence() { asm volatile("" ::: "memory"); }
+
+// First and one-past-last updated row of a thread's strip of XS rows
+// (x in [4, nnx - 4)).
+template <int TZ, int TXB, int XS>
+__device__ __forceinline__ bool strip(const BShot &b, int lt, int &iz, int &x0, int &x1)
+{
+    iz = (lt % b.ntz) * TZ + threadIdx.x;
+    const int xs = (lt / b.ntz) * (TXB * XS) + threadIdx.y * XS;
+    x0 = max(xs, 4);
+    x1 = min(xs + XS, b.nnx - 4);
+    return iz >= 4 && iz < b.nnz - 4 && x0 < x1;
+}
+
+// which = 0: source state f, 1: receiver state g.  XS = 1: one cell per
+// thread; XS > 1: each thread marches XS consecutive rows (x) of its
+// column, holding the 8-deep x-stencil of P in a register queue (one load of
+// P per cell instead of eight).  The arithmetic is the same in both.
+template <int TZ, int TXB, int XS, int WHICH>
 __global__ __launch_bounds__(TZ *TXB) void bvel(const BShot *__restrict__ S, int ns, int ntile,
                                                const int *__restrict__ itp, int inject)
 {
@@ -72,31 +89,49 @@ __global__ __launch_bounds__(TZ *TXB) void bvel(const BShot *__restrict__ S, int
         const int s = find_shot(S, ns, t);
         const BShot &b = S[s];
         const int lt = t - b.tile0;
-        const int iz = (lt % b.ntz) * TZ + threadIdx.x;
-        const int ix = (lt / b.ntz) * TXB + threadIdx.y;
-        if (iz < 4 || iz >= b.nnz - 4 || ix < 4 || ix >= b.nnx - 4) continue;
+        int iz, x0, x1;
+        if (!strip<TZ, TXB, XS>(b, lt, iz, x0, x1)) continue;
         float *const *F = WHICH ? b.g : b.f;
         const float *P = F[6], *Q = F[7];
-        const long long id = iz + (long long)ix * b.ldz;
         const float amp = (inject && WHICH == 0) ? b.amp[it] : 0.0f;
         const long long src = amp != 0.0f ? b.src : -1;
-        float xx = 0.0f, zz = 0.0f;
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            const long long i1 = id + (long long)(m + 1) * b.ldz, i0 = id - (long long)m * b.ldz;
-            float p1 = P[i1], p0 = P[i0], q1 = Q[id + m + 1], q0 = Q[id - m];
-            if (src >= 0) {
-                if (i1 == src) p1 = p1 + amp;
-                if (i0 == src) p0 = p0 + amp;
-                if (id + m + 1 == src) q1 = q1 + amp;
-                if (id - m == src) q0 = q0 + amp;
-            }
-            xx += cst(m) * (p1 - p0);
-            zz += cst(m) * (q1 - q0);
-        }
         const float *tb = b.tab;
-        F[0][id] = tb[1 * b.M + ix] * F[0][id] - tb[0 * b.M + ix] * b.dtx * xx;
-        F[1][id] = tb[5 * b.M + iz] * F[1][id] - tb[4 * b.M + iz] * b.dtz * zz;
+        const long long L = b.ldz;
+        float q[8];   // q[k] = P at row x - 3 + k
+#pragma unroll
+        for (int k = 0; k < 8; ++k) q[k] = XS > 1 ? P[iz + (long long)(x0 - 3 + k) * L] : 0.0f;
+        for (int ix = x0; ix < x1; ++ix) {
+            const long long id = iz + (long long)ix * L;
+            float pp[8], qz[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                pp[k] = XS > 1 ? q[k] : P[id + (long long)(k - 3) * L];
+                qz[k] = Q[id + k - 3];
+            }
+            const float u0 = F[0][id], w0 = F[1][id];
+            const float qn = XS > 1 && ix + 1 < x1 ? P[id + 5 * L] : 0.0f;
+            compiler_fence();
+            if (src >= 0) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (id + (long long)(k - 3) * L == src) pp[k] = pp[k] + amp;
+                    if (id + k - 3 == src) qz[k] = qz[k] + amp;
+                }
+            }
+            float xx = 0.0f, zz = 0.0f;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                xx += cst(m) * (pp[m + 4] - pp[3 - m]);
+                zz += cst(m) * (qz[m + 4] - qz[3 - m]);
+            }
+            F[0][id] = tb[1 * b.M + ix] * u0 - tb[0 * b.M + ix] * b.dtx * xx;
+            F[1][id] = tb[5 * b.M + iz] * w0 - tb[4 * b.M + iz] * b.dtz * zz;
+            if (XS > 1) {
+#pragma unroll
+                for (int k = 0; k < 7; ++k) q[k] = q[k + 1];
+                q[7] = qn;
+            }
+        }
     }
 }
 
@@ -106,7 +141,7 @@ __global__ __launch_bounds__(TZ *TXB) void bvel(const BShot *__restrict__ S, int
 // directly where the cell's damping factors are all 1, the split fields
 // only in the absorbing layer and, on the receiver side, on the injection
 // row (as stress2d_kernel<true>).
-template <int TZ, int TXB, int MODE>
+template <int TZ, int TXB, int XS, int MODE>
 __global__ __launch_bounds__(TZ *TXB) void bstress(const BShot *__restrict__ S, int ns, int ntile,
                                                   const int *__restrict__ itp, int sv_on, int image_every,
                                                   int illum, int unsplit)
@@ -116,16 +151,28 @@ __global__ __launch_bounds__(TZ *TXB) void bstress(const BShot *__restrict__ S, 
         const int s = find_shot(S, ns, t);
         const BShot &b = S[s];
         const int lt = t - b.tile0;
-        const int iz = (lt % b.ntz) * TZ + threadIdx.x;
-        const int ix = (lt / b.ntz) * TXB + threadIdx.y;
-        if (iz < 4 || iz >= b.nnz - 4 || ix < 4 || ix >= b.nnx - 4) continue;
+        int iz, x0, x1;
+        if (!strip<TZ, TXB, XS>(b, lt, iz, x0, x1)) continue;
         float *const *F = MODE == 2 ? b.g : b.f;
-        const long long id = iz + (long long)ix * b.ldz;
+        const long long L = b.ldz;
+        float uq[8];   // uq[k] = u at row x - 4 + k
+#pragma unroll
+        for (int k = 0; k < 8; ++k) uq[k] = XS > 1 ? F[0][iz + (long long)(x0 - 4 + k) * L] : 0.0f;
+        for (int ix = x0; ix < x1; ++ix) {
+        const long long id = iz + (long long)ix * L;
+        float ux[8], wz[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            ux[k] = XS > 1 ? uq[k] : F[0][id + (long long)(k - 4) * L];
+            wz[k] = F[1][id + k - 4];
+        }
+        const float un = XS > 1 && ix + 1 < x1 ? F[0][id + 4 * L] : 0.0f;
+        compiler_fence();
         float xx = 0.0f, zz = 0.0f;
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
-            xx += cst(m) * (F[0][id + (long long)m * b.ldz] - F[0][id - (long long)(m + 1) * b.ldz]);
-            zz += cst(m) * (F[1][id + m] - F[1][id - m - 1]);
+            xx += cst(m) * (ux[4 + m] - ux[3 - m]);
+            zz += cst(m) * (wz[4 + m] - wz[3 - m]);
         }
         const float vv = b.vp[id];
         float ee = b.ep[id], dd = b.de[id];
@@ -180,6 +227,12 @@ __global__ __launch_bounds__(TZ *TXB) void bstress(const BShot *__restrict__ S, 
             if (jx >= 0 && jx < b.nx && jz >= 0 && jz < b.nz)
                 b.img[jz + (long long)jx * b.nz] += b.f[6][id] * Pn;
         }
+        if (XS > 1) {
+#pragma unroll
+            for (int k = 0; k < 7; ++k) uq[k] = uq[k + 1];
+            uq[7] = un;
+        }
+        }
     }
 }
 
@@ -214,11 +267,14 @@ __global__ void bzero(float4 *p, size_t n4)
         p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-// Tile shapes the autotuner chooses from.
+// Tile shapes the autotuner chooses from: tz x tx threads, each thread
+// marching xs rows (the tile covers tz x tx*xs cells).
 struct Shape {
-    int tz, tx;
+    int tz, tx, xs;
 };
-constexpr Shape kShapes[] = {{64, 4}, {64, 2}, {64, 8}, {128, 2}, {32, 8}};
+constexpr Shape kShapes[] = {{64, 4, 1},  {64, 2, 1},  {64, 8, 1},  {128, 2, 1}, {32, 8, 1},
+                             {64, 4, 16}, {64, 4, 32}, {64, 2, 32}, {32, 8, 16}, {128, 2, 16}};
+constexpr int kNumShapes = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
 
 }  // namespace
 }  // namespace vti
@@ -271,34 +327,37 @@ namespace {
 
 int bfail(int code, const std::string &m) { return fail(code, m); }
 
-template <int TZ, int TXB>
+template <int TZ, int TXB, int XS>
 void launch_phase(vti_batch *B, int phase, int sv_on, int grid)
 {
     const dim3 blk(TZ, TXB);
     const int us = B->unsplit ? 1 : 0;
     switch (phase) {
-    case 0: hipLaunchKernelGGL((bvel<TZ, TXB, 0>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, 1); break;
-    case 1: hipLaunchKernelGGL((bstress<TZ, TXB, 0>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 0, us); break;
-    case 2: hipLaunchKernelGGL((bstress<TZ, TXB, 1>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 0, us); break;
-    case 3: hipLaunchKernelGGL((bvel<TZ, TXB, 1>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, 0); break;
-    case 4: hipLaunchKernelGGL((bstress<TZ, TXB, 2>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 0, us); break;
-    case 5: hipLaunchKernelGGL((bstress<TZ, TXB, 1>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 2, us); break;
+    case 0: hipLaunchKernelGGL((bvel<TZ, TXB, XS, 0>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, 1); break;
+    case 1: hipLaunchKernelGGL((bstress<TZ, TXB, XS, 0>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 0, us); break;
+    case 2: hipLaunchKernelGGL((bstress<TZ, TXB, XS, 1>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 0, us); break;
+    case 3: hipLaunchKernelGGL((bvel<TZ, TXB, XS, 1>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, 0); break;
+    case 4: hipLaunchKernelGGL((bstress<TZ, TXB, XS, 2>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 0, us); break;
+    case 5: hipLaunchKernelGGL((bstress<TZ, TXB, XS, 1>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 2, us); break;
     }
 }
 
 void phase(vti_batch *B, int ph, int sv_on)
 {
-    const Shape sh = kShapes[B->shape];
     int grid = B->ntile;
     if (B->grid_mult > 0) grid = std::min(grid, B->grid_mult * 256);
     switch (B->shape) {
-    case 0: launch_phase<64, 4>(B, ph, sv_on, grid); break;
-    case 1: launch_phase<64, 2>(B, ph, sv_on, grid); break;
-    case 2: launch_phase<64, 8>(B, ph, sv_on, grid); break;
-    case 3: launch_phase<128, 2>(B, ph, sv_on, grid); break;
-    case 4: launch_phase<32, 8>(B, ph, sv_on, grid); break;
+    case 0: launch_phase<64, 4, 1>(B, ph, sv_on, grid); break;
+    case 1: launch_phase<64, 2, 1>(B, ph, sv_on, grid); break;
+    case 2: launch_phase<64, 8, 1>(B, ph, sv_on, grid); break;
+    case 3: launch_phase<128, 2, 1>(B, ph, sv_on, grid); break;
+    case 4: launch_phase<32, 8, 1>(B, ph, sv_on, grid); break;
+    case 5: launch_phase<64, 4, 16>(B, ph, sv_on, grid); break;
+    case 6: launch_phase<64, 4, 32>(B, ph, sv_on, grid); break;
+    case 7: launch_phase<64, 2, 32>(B, ph, sv_on, grid); break;
+    case 8: launch_phase<32, 8, 16>(B, ph, sv_on, grid); break;
+    case 9: launch_phase<128, 2, 16>(B, ph, sv_on, grid); break;
     }
-    (void)sh;
 }
 
 // tiles of the current shape; rewrites the device descriptors
@@ -310,7 +369,7 @@ int set_shape(vti_batch *B, int shape)
     for (auto &b : B->hb) {
         b.tile0 = t;
         b.ntz = (b.nnz + sh.tz - 1) / sh.tz;
-        b.ntx = (b.nnx + sh.tx - 1) / sh.tx;
+        b.ntx = (b.nnx + sh.tx * sh.xs - 1) / (sh.tx * sh.xs);
         t += b.ntz * b.ntx;
     }
     B->ntile = t;
@@ -542,7 +601,7 @@ int vti_batch_destroy(vti_batch *B)
 
 int vti_batch_config(vti_batch *B, int tile_shape, int grid_mult, int graph_steps)
 {
-    if (!B || tile_shape < 0 || tile_shape >= (int)(sizeof(kShapes) / sizeof(kShapes[0])) || grid_mult < 0 ||
+    if (!B || tile_shape < 0 || tile_shape >= kNumShapes || grid_mult < 0 ||
         graph_steps < 0)
         return bfail(VTI_ERR_ARG, "vti_batch_config: bad arguments");
     B->grid_mult = grid_mult;
@@ -554,7 +613,7 @@ int vti_batch_get_config(vti_batch *B, int *tile_z, int *tile_x, int *grid_mult,
 {
     if (!B) return bfail(VTI_ERR_ARG, "null");
     if (tile_z) *tile_z = kShapes[B->shape].tz;
-    if (tile_x) *tile_x = kShapes[B->shape].tx;
+    if (tile_x) *tile_x = kShapes[B->shape].tx * kShapes[B->shape].xs;
     if (grid_mult) *grid_mult = B->grid_mult;
     if (graph_steps) *graph_steps = B->graph_k;
     return VTI_OK;
@@ -592,7 +651,7 @@ int vti_batch_autotune(vti_batch *B, int steps, double *best_ms)
 {
     if (!B || steps < 1 || steps > B->nt || B->nt < 2)
         return bfail(VTI_ERR_ARG, "vti_batch_autotune: need 1 <= steps <= nt and nt >= 2");
-    const int nshape = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
+    const int nshape = kNumShapes;
     const int mults[] = {0, 8, 16};
     double best = 1e30;
     int bs = 0, bm = 0;

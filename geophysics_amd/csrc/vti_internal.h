@@ -72,6 +72,9 @@ struct StressArgs {
     // the interior is inx x iny x inz cells from padded index inp (y only in 3D)
     float *illum;
     int illum_clamp, inp, inx, iny, inz;
+    // 3D: the updated P of every interior cell (the illumination's interior)
+    // also written to snap[z + x inz + y inz inx] (nullptr: none)
+    float *snap;
     // 2D TTI (Toa_gpu_2dtti_fd_1orderfunction.cu:116-183): as VelArgs
     const float *cth, *sth;
 };
@@ -208,6 +211,8 @@ StressArgs src_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, float
 StressArgs rcv_stress_args(vti_ctx *c, const VtiState &s, int it, bool sv, int inj_z);
 // a with the context's source illumination accumulated in its epilogue
 StressArgs with_illum(vti_ctx *c, StressArgs a, int clamp);
+// a 3D stress update that also copies the interior P into snap (dense, z fastest)
+StressArgs with_snap(vti_ctx *c, StressArgs a, float *snap);
 // one angle-gather accumulation (vti_adcig.hip) from a source and a receiver
 // state of the context's grid, when the context has gathers configured
 hipError_t launch_adcig(vti_ctx *c, const VtiState &src, const VtiState &rcv);

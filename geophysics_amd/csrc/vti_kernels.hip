@@ -385,11 +385,12 @@ hipError_t launch_stress(int ndim, const StressArgs &a, hipStream_t s)
     } else {
         int nzt, nt, ntp;
         const int grd = s3d::grid3d(a.g, kYChunk, nzt, nt, ntp);
-        if (a.unsplit && !a.gsrc)
-            hipLaunchKernelGGL((s3d::stress3d<kYChunk, true, true>), dim3(grd), dim3(s3d::TZ, s3d::TX), 0, s, a, nzt,
-                               nt, ntp);
-        else
-            hipLaunchKernelGGL((s3d::stress3d<kYChunk, true>), dim3(grd), dim3(s3d::TZ, s3d::TX), 0, s, a, nzt, nt, ntp);
+        const dim3 blk(s3d::TZ, s3d::TX);
+        const bool us = a.unsplit && !a.gsrc;
+        if (a.snap && us) hipLaunchKernelGGL((s3d::stress3d<kYChunk, true, true, true>), dim3(grd), blk, 0, s, a, nzt, nt, ntp);
+        else if (a.snap) hipLaunchKernelGGL((s3d::stress3d<kYChunk, true, false, true>), dim3(grd), blk, 0, s, a, nzt, nt, ntp);
+        else if (us) hipLaunchKernelGGL((s3d::stress3d<kYChunk, true, true>), dim3(grd), blk, 0, s, a, nzt, nt, ntp);
+        else hipLaunchKernelGGL((s3d::stress3d<kYChunk, true>), dim3(grd), blk, 0, s, a, nzt, nt, ntp);
     }
     return hipGetLastError();
 }

@@ -98,6 +98,18 @@ __global__ void box_save_kernel(Grid g, int np, const float *__restrict__ P, Box
 }
 
 // D (dense n^3) over the source box = the kept exact values
+// box of a dense n^3 volume (z fastest) -> out (box layout, z fastest)
+__global__ void box_save_dense_kernel(int n, const float *__restrict__ src, Box b, float *out)
+{
+    const long long tot = b.size();
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < tot; i += (long long)gridDim.x * blockDim.x) {
+        const int z = (int)(i % b.bz);
+        const long long r = i / b.bz;
+        const int x = (int)(r % b.bx), y = (int)(r / b.bx);
+        out[i] = src[(b.z0 + z) + (long long)(b.x0 + x) * n + (long long)(b.y0 + y) * n * n];
+    }
+}
+
 __global__ void box_restore_kernel(int n, Box b, const float *__restrict__ in, float *D)
 {
     const long long tot = b.size();
@@ -271,7 +283,7 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
     const int nsave = (nt + se - 1) / se;
     const long long off = np + (long long)np * c->g.ldz + (long long)np * c->g.ldxy;
     hipStream_t s = c->stream;
-    Buf lat, scl, dense, dense2, acc, exa, boxes, boffs, bcnts;
+    Buf lat, scl, dense, dense2, acc, exa, boxes, boffs, bcnts, snapb;
     if ((!sparse && (hipMalloc(&lat.p, (size_t)nsave * lat_n * sizeof(uint16_t)) != hipSuccess ||
                      hipMalloc(&scl.p, (size_t)nsave * 64) != hipSuccess)) ||
         (!sparse && hipMalloc(&dense.p, (size_t)n * n * n * sizeof(float)) != hipSuccess) ||
@@ -356,22 +368,26 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
     if (!t_enc.ok || !t_dec.ok) return fail(VTI_ERR_HIP, "vti_rtm_shot_learned: events");
     double err_sum = 0.0;
     int nchecked = 0, nbad = 0;
-    // Codecs 1, 2: the codec runs on a stream of its own.  A snapshot's
-    // encode waits for the stress update that made it and overlaps the next
-    // velocity update; the next stress update (the first write of P) waits
-    // for the encode's projection.  Its counters are read on the host one
-    // snapshot later (the next encode's words start where its words end).
-    // In the reverse sweep a snapshot's decode, fused with its image, overlaps
-    // the receiver side's next velocity update.  serial_codec puts it all on
-    // the shot's stream (the same results).
+    // Codecs 1, 2: the codec runs on a stream of its own.  The stress update
+    // of a snapshot step also writes the interior P into the dense copy SN
+    // (the source side forward, the receiver side in reverse); the codec
+    // works from the copy while the steps go on: the encode (its counters
+    // read on the host one snapshot later, where the next encode's words
+    // start), and in reverse the decode fused with the image.  The next
+    // snapshot step's stress update, the copy's next writer, waits for that
+    // work (4 steps earlier).  serial_codec puts the codec on the shot's
+    // stream between the steps, reading P itself (the same results).
     const bool side = sparse && !lo->serial_codec;
     CodecStream cs;
     if (sparse && !cs.ok) return fail(VTI_ERR_HIP, "vti_rtm_shot_learned: codec stream");
+    if (side && hipMalloc(&snapb.p, nn3 * sizeof(float)) != hipSuccess)
+        return fail(VTI_ERR_ALLOC, "vti_rtm_shot_learned: snapshot copy");
+    float *SN = static_cast<float *>(snapb.p);
     hipStream_t es = side ? cs.s : s;   // the codec's stream
     bc::Counters *hcs = cs.hc;
     int pend = -1;                      // snapshot whose encode has not been settled on the host
     long long pend_cap = 0;
-    bool wait_read = false;             // the next stress update waits for an encode's projection
+    bool wait_read = false;             // the next snapshot step's stress update waits for an encode's projection
     auto settle = [&]() -> int {        // the host side of the pending encode: place its words
         if (pend < 0) return VTI_OK;
         const int j = pend;
@@ -401,13 +417,16 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         if (nck && it % lo->ck_every == 0 && (st = hip(state_copy(c, ck[it / lo->ck_every], W), "checkpoint")))
             return st;
         if (launch_vel(3, vel_args_for(c, W, it, true), s) != hipSuccess) return fail(VTI_ERR_HIP, "vel");
-        if (wait_read) {
-            if ((st = hip(hipStreamWaitEvent(s, cs.p_read, 0), "wait"))) return st;
-            wait_read = false;
-        }
         float *row = c->rec + (size_t)it * nrec;
-        if (launch_stress(3, with_illum(c, src_stress_args(c, W, it, svf, row), clamp), s) != hipSuccess)
-            return fail(VTI_ERR_HIP, "stress");
+        StressArgs sa = with_illum(c, src_stress_args(c, W, it, svf, row), clamp);
+        if (side && it % se == 0) {
+            if (wait_read) {
+                if ((st = hip(hipStreamWaitEvent(s, cs.p_read, 0), "wait"))) return st;
+                wait_read = false;
+            }
+            sa = with_snap(c, sa, SN);
+        }
+        if (launch_stress(3, sa, s) != hipSuccess) return fail(VTI_ERR_HIP, "stress");
         if (it % se) continue;
         const int j = it / se;
         const float *P = W.f[VTI_F_P];
@@ -417,9 +436,13 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
                          (st = hip(hipStreamWaitEvent(es, cs.p_ready, 0), "wait"))))
                 return st;
             if ((st = hip(t_enc.begin(es), "event"))) return st;
-            if (box_n)
+            if (box_n && side)
+                hipLaunchKernelGGL(box_save_dense_kernel, dim3(64), dim3(256), 0, es, n, SN, bx, BX + (size_t)j * box_n);
+            else if (box_n)
                 hipLaunchKernelGGL(box_save_kernel, dim3(64), dim3(256), 0, es, c->g, np, P, bx, BX + (size_t)j * box_n);
             const bc::Mask m{bx.y0, bx.x0, bx.z0, bx.by, bx.bx, bx.bz};
+            const float *src = side ? SN : P;   // the box to encode: the copy, or P itself
+            const long long so = side ? 0 : off, ssy = side ? (long long)n * n : c->g.ldxy, ssx = side ? n : c->g.ldz;
             if (chunk_room < snap_max) {   // a new chunk: 4 GB, or what the budget has left
                 const long long take = std::min(chunk_ents, budget / 4 - alloc_ents);
                 void *q = nullptr;
@@ -435,9 +458,9 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
                 }
             }
             hipEvent_t rd = side ? cs.p_read : nullptr;
-            st = quant ? ae_bq_encode_start(A, P, off, c->g.ldxy, c->g.ldz, n, m, lo->tol, lo->floor, cur, chunk_room,
+            st = quant ? ae_bq_encode_start(A, src, so, ssy, ssx, n, m, lo->tol, lo->floor, cur, chunk_room,
                                             BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, rd, hcs + (j & 1), es)
-                       : ae_bc_encode_start(A, P, off, c->g.ldxy, c->g.ldz, n, m, lo->tol, lo->floor, cur, chunk_room,
+                       : ae_bc_encode_start(A, src, so, ssy, ssx, n, m, lo->tol, lo->floor, cur, chunk_room,
                                             BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, rd, hcs + (j & 1), es);
             if (st) return st;
             if ((st = hip(t_enc.end(es), "event"))) return st;
@@ -497,29 +520,28 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
     }
     // ---- reverse sweep
     // codecs 1, 2: snapshot j decoded straight into the image (times the
-    // receiver field of its step, the exact box included); overlapped, on
-    // the codec stream while the receiver side runs the next velocity update,
-    // and the next write of the receiver P (its injection row, when that lies
-    // in the interior, else the stress update) waits for it
-    const bool inj_interior = inj_z >= np && inj_z < np + n;
+    // receiver field of its step, the exact box included); overlapped, from
+    // the copy of the receiver P the step's stress update wrote, while the
+    // receiver side steps on.  Any later write of the image or of the copy
+    // is on the shot's stream after the next snapshot step's wait for it.
     bool wait_img = false;
     if ((st = hip(state_zero(c, G), "zero"))) return st;
     for (int it = nt - 1; it >= 0; --it) {
-        if (wait_img && inj_interior) {
-            if ((st = hip(hipStreamWaitEvent(s, cs.done[0], 0), "wait"))) return st;
-            wait_img = false;
-        }
         if (launch_inject_rec(c, G, c->rec + (size_t)it * nrec, inj_z) != hipSuccess) return fail(VTI_ERR_HIP, "inject");
         if (launch_vel(3, vel_args_for(c, G, it, false), s) != hipSuccess) return fail(VTI_ERR_HIP, "vel");
-        if (wait_img) {
-            if ((st = hip(hipStreamWaitEvent(s, cs.done[0], 0), "wait"))) return st;
-            wait_img = false;
-        }
         StressArgs ra = rcv_stress_args(c, G, it, svb, inj_z);
         if (o->illum_mode == 1) ra = with_illum(c, ra, 0);   // receiver illumination in the epilogue
+        const int j = it / se;
+        const bool coded = sparse && it % se == 0 && exact[j] < 0 && !bad[j];
+        if (side && it % se == 0) {
+            if (wait_img) {
+                if ((st = hip(hipStreamWaitEvent(s, cs.done[0], 0), "wait"))) return st;
+                wait_img = false;
+            }
+            if (coded) ra = with_snap(c, ra, SN);
+        }
         if (launch_stress(3, ra, s) != hipSuccess) return fail(VTI_ERR_HIP, "stress");
         if (it % se) continue;
-        const int j = it / se;
         const dim3 blk(64, 4), grd((k.nz + 63) / 64, (k.nx + 3) / 4, k.ny);
         if (exact[j] >= 0) {
             hipLaunchKernelGGL(image_dense_kernel, grd, blk, 0, s, c->g, np, k.nx, k.ny, k.nz, X + (size_t)exact[j] * nn3,
@@ -534,13 +556,13 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
                     return fail(VTI_ERR_HIP, "stress");
             }
             if ((st = hip(launch_image(c, W.f[VTI_F_P], G.f[VTI_F_P], c->img), "image"))) return st;
-        } else if (sparse) {
+        } else if (coded) {
             bc::TileSink sk;
             sk.img = c->img;
-            sk.rp = G.f[VTI_F_P];
-            sk.off = off;
-            sk.sy = c->g.ldxy;
-            sk.sx = c->g.ldz;
+            sk.rp = side ? SN : G.f[VTI_F_P];
+            sk.off = side ? 0 : off;
+            sk.sy = side ? (long long)n * n : c->g.ldxy;
+            sk.sx = side ? n : c->g.ldz;
             if (box_n) {
                 sk.box = BX + (size_t)j * box_n;
                 sk.y0 = bx.y0, sk.x0 = bx.x0, sk.z0 = bx.z0, sk.by = bx.by, sk.bx = bx.bx, sk.bz = bx.bz;
@@ -584,11 +606,11 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         // device memory held for the stored source field, as allocated: the
         // store (entry chunks and block tables, or latents, scales and exact
         // slots), the source boxes, the work buffers (the codec's
-        // coefficients and search results, or the latents' two dense n^3
-        // buffers) and the fallback checkpoints
+        // coefficients and search results and the snapshot copy, or the
+        // latents' two dense n^3 buffers) and the fallback checkpoints
         const double ckb = (double)nck * (double)vti_state_bytes(c);
         stats[8] = sparse ? (double)alloc_ents * 4 + (double)nsave * nblk * 6 + (double)nsave * box_n * 4 +
-                                (double)ae_codec_work_bytes(A) + ckb
+                                (double)ae_codec_work_bytes(A) + (side ? (double)nn3 * 4 : 0.0) + ckb
                           : (double)nsave * lat_n * 2 + (double)nsave * 64 + (double)lo->max_exact * nn3 * 4 +
                                 (double)nsave * box_n * 4 + 2.0 * nn3 * 4 + ckb;
     }

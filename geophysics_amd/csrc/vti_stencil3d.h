@@ -34,6 +34,17 @@ __device__ __forceinline__ void illum_add(const StressArgs &a, int ix, int iy, i
     a.illum[o] = v;
 }
 
+// The epilogue's copy of P into the dense interior snapshot (the
+// illumination's interior), in the kernels instantiated for it only.
+template <bool SNAP>
+__device__ __forceinline__ void snap_put(const StressArgs &a, int ix, int iy, int iz, float Pn)
+{
+    if (!SNAP) return;
+    const int jx = ix - a.inp, jy = iy - a.inp, jz = iz - a.inp;
+    if (jx < 0 || jx >= a.inx || jy < 0 || jy >= a.iny || jz < 0 || jz >= a.inz) return;
+    a.snap[jz + (long long)jx * a.inz + (long long)jy * a.inz * a.inx] = Pn;
+}
+
 namespace s3d {
 
 __device__ __forceinline__ float cst(int m)
@@ -155,7 +166,7 @@ __global__ __launch_bounds__(256) void vel3d(VelArgs a, int nzt, int ntile, int 
 // the undamped (tx ..) axes (56 or 72 bytes per cell).  Corners (three damped
 // axes) and row a.split_z keep the full split form.  Point-source
 // (load-time) form only.
-template <int YCHUNK, bool NT, bool US = false>
+template <int YCHUNK, bool NT, bool US = false, bool SNAP = false>
 __global__ __launch_bounds__(256) void stress3d(StressArgs a, int nzt, int ntile, int ntile_p)
 {
     const Grid g = a.g;
@@ -229,6 +240,7 @@ __global__ __launch_bounds__(256) void stress3d(StressArgs a, int nzt, int ntile
             st<NT>(a.P + id, Pn);
             st<NT>(a.Q + id, Qn);
             illum_add(a, ix, iy, iz, Pn, Qn);
+            snap_put<SNAP>(a, ix, iy, iz, Pn);
             if (rk >= 0) {
                 const int j = iy - a.ry0;
                 if (j >= 0 && j % a.rdy == 0 && j / a.rdy < a.rny) {
@@ -258,6 +270,7 @@ __global__ __launch_bounds__(256) void stress3d(StressArgs a, int nzt, int ntile
             st<NT>(a.P + id, Pn);
             st<NT>(a.Q + id, Qn);
             illum_add(a, ix, iy, iz, Pn, Qn);
+            snap_put<SNAP>(a, ix, iy, iz, Pn);
             if (rk >= 0) {
                 const int j = iy - a.ry0;
                 if (j >= 0 && j % a.rdy == 0 && j / a.rdy < a.rny) {
@@ -283,6 +296,7 @@ __global__ __launch_bounds__(256) void stress3d(StressArgs a, int nzt, int ntile
             a.P[id] = Pn;
             a.Q[id] = Qn;
             illum_add(a, ix, iy, iz, Pn, Qn);
+            snap_put<SNAP>(a, ix, iy, iz, Pn);
             if (rk >= 0) {
                 const int j = iy - a.ry0;
                 if (j >= 0 && j % a.rdy == 0 && j / a.rdy < a.rny) {

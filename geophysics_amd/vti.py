@@ -164,6 +164,8 @@ def _declare(L):
                                     C.c_float, VP, C.c_longlong, VP, VP, C.POINTER(C.c_longlong), DP]),
         "vti_bq_decode": (C.c_int, [VP, VP, VP, VP, C.c_int, VP]),
         "vti_bq_max_words": (C.c_longlong, [C.c_int]),
+        "vti_codec_decode_image": (C.c_int, [VP, C.c_int, VP, VP, VP, C.c_int, VP, VP, C.c_longlong, C.c_longlong,
+                                             C.c_longlong, VP, C.POINTER(C.c_int)]),
         "vti_adcig_config": (C.c_int, [VP, C.c_int, C.c_int, C.c_int]),
         "vti_adcig_elems": (C.c_longlong, [VP]),
         "vti_adcig_tables": (C.c_int, [C.c_int, C.c_int, FP, FP]),
@@ -950,6 +952,14 @@ class Autoencoder:
 
     def bq_decode(self, ent, boff, bcnt, n, out):
         check(lib().vti_bq_decode(self.h, ent, boff, bcnt, n, out), "bq_decode")
+
+    def decode_image(self, codec, ent, boff, bcnt, n, img, rp, off, sy, sx, box=None, geom=None):
+        """img += decoded snapshot (the box's values inside geom = (y0, x0,
+        z0, by, bx, bz) when box is given) times rp at off + y sy + x sx + z
+        (device pointers)."""
+        g = None if geom is None else (C.c_int * 6)(*geom)
+        check(lib().vti_codec_decode_image(self.h, codec, ent, boff, bcnt, n, img, rp, off, sy, sx, box, g),
+              "codec_decode_image")
 
     @staticmethod
     def bq_max_words(n):

@@ -278,7 +278,10 @@ int vti_batch_create(int nshot, const vti_cfg *cfgs, const vti_shot *shots,
                      const float *const *vp, const float *const *eps,
                      const float *const *del, vti_batch **out);
 int vti_batch_destroy(vti_batch *b);
-/* tile_shape: 0..4 = 64x4, 64x2, 64x8, 128x2, 32x8 (z x x); grid_mult:
+/* tile_shape: 0..4 = 64x4, 64x2, 64x8, 128x2, 32x8 threads (z x x), one cell
+ * each; 5..9 = 64x4, 64x4, 64x2, 32x8, 128x2 threads, each marching 16, 32,
+ * 32, 16, 16 consecutive rows in x with the x-stencil in registers
+ * (vti_batch_get_config's tile_x: the tile's extent in x); grid_mult:
  * persistent grid of grid_mult*256 workgroups (0 = one per tile);
  * graph_steps: steps per captured hipGraph block (0/1 = direct launches). */
 int vti_batch_config(vti_batch *b, int tile_shape, int grid_mult, int graph_steps);
@@ -421,6 +424,19 @@ int vti_bq_encode(vti_ae *ae, const float *P, long long off, long long sy, long 
 int vti_bq_decode(vti_ae *ae, const unsigned *ent, const unsigned *boff, const unsigned short *bcnt, int n,
                   float *out);
 long long vti_bq_max_words(int n);
+/* A stored snapshot decoded straight into an image, as the reverse sweep of
+ * vti_rtm_shot_learned does (cal_migration's img += s * g,
+ * Toa_gpu2DvtiFdRtmAdcigsLaplace.cu:679-698, with the source sample s
+ * decoded): img[o] += v * rp[off + y*sy + x*sx + z] for every voxel (y, x,
+ * z) of the n^3 box, o = (y*n + x)*n + z, v the decoded value or, when box
+ * is not NULL, inside the by x bx x bz voxels at (y0, x0, z0) = box_geom[0..5]
+ * = {y0, x0, z0, by, bx, bz} (host array), box[((y - y0)*bx + (x - x0))*bz +
+ * (z - z0)] (an exactly kept box).  codec 1: entries of vti_bc_encode, 2:
+ * streams of vti_bq_encode.  Device pointers except box_geom; returns after
+ * the autoencoder's stream has finished. */
+int vti_codec_decode_image(vti_ae *ae, int codec, const unsigned *ent, const unsigned *boff,
+                           const unsigned short *bcnt, int n, float *img, const float *rp, long long off,
+                           long long sy, long long sx, const float *box, const int *box_geom);
 /* Symmetric eigen-decomposition used by the fit (cyclic Jacobi, host):
  * a n x n row-major, w eigenvalues, v eigenvectors as columns, unsorted. */
 int vti_sym_eigen(int n, const double *a, double *w, double *v);

@@ -23,7 +23,8 @@ def _batch(nt=260, sv=1):
     return cfgs, shots, models
 
 
-@pytest.mark.parametrize("shape,mult,graph", [(0, 0, 0), (2, 8, 16), (3, 16, 5), (4, 0, 32), (1, 8, 0)])
+@pytest.mark.parametrize("shape,mult,graph", [(0, 0, 0), (2, 8, 16), (3, 16, 5), (4, 0, 32), (1, 8, 0),
+                                               (5, 0, 0), (6, 8, 16), (7, 0, 5), (8, 16, 0), (9, 0, 32)])
 def test_batch_forward_records_equal_single_shot(shape, mult, graph):
     cfgs, shots, models = _batch()
     nt = cfgs[0].nt
@@ -40,14 +41,15 @@ def test_batch_forward_records_equal_single_shot(shape, mult, graph):
                 assert np.array_equal(recs[s], ref), (rep, s, rel_l2(recs[s], ref))
 
 
-def test_batch_rtm_images_equal_single_shot():
+@pytest.mark.parametrize("shape", [0, 5])
+def test_batch_rtm_images_equal_single_shot(shape):
     cfgs, shots, models = _batch(nt=220)
     rng = np.random.default_rng(0)
     obs = [(rng.standard_normal((s.rec_nx, c.nt)) * 0.1).astype(np.float32) for c, s in zip(cfgs, shots)]
     with vti.Batch(cfgs, shots, models) as B:
         o = vti.rtm_defaults(2)
         o.nsnap = 3
-        B.config(0, 8, 0)
+        B.config(shape, 8, 0)
         steps = B.rtm(o, obs)
         imgs = [B.image(s) for s in range(B.n)]
     for s, (cfg, shot, m) in enumerate(zip(cfgs, shots, models)):
@@ -67,18 +69,19 @@ def test_batch_autotune_picks_a_shape():
     cfgs, shots, models = _batch(nt=40)
     with vti.Batch(cfgs, shots, models) as B:
         shape, ms = B.autotune(steps=3)
-        assert 0 <= shape < 5 and ms > 0
+        assert 0 <= shape < 10 and ms > 0
         print("autotuned", B.get_config(), f"{ms:.3f} ms/step", flush=True)
 
 
-def test_batch_rtm_modelled_records_with_mute_equal_single_shot():
+@pytest.mark.parametrize("shape", [2, 7])
+def test_batch_rtm_modelled_records_with_mute_equal_single_shot(shape):
     """obs=None: each shot migrates its own modelled record after the
     direct-wave mute (tt=20), as the single-shot RTM does."""
     cfgs, shots, models = _batch(nt=200, sv=0)
     with vti.Batch(cfgs, shots, models) as B:
         o = vti.rtm_defaults(2)
         o.nsnap = 5
-        B.config(2, 0, 0)
+        B.config(shape, 0, 0)
         B.rtm(o, None)
         imgs = [B.image(s) for s in range(B.n)]
     for s, (cfg, shot, m) in enumerate(zip(cfgs, shots, models)):
@@ -92,8 +95,8 @@ def test_batch_rtm_modelled_records_with_mute_equal_single_shot():
         assert np.array_equal(imgs[s][0], img), (s, rel_l2(imgs[s][0], img))
 
 
-@pytest.mark.parametrize("graph", [0, 4])
-def test_batch_rtm_unsplit_equals_single_shot_unsplit(graph):
+@pytest.mark.parametrize("shape,graph", [(0, 0), (0, 4), (6, 4), (9, 0)])
+def test_batch_rtm_unsplit_equals_single_shot_unsplit(shape, graph):
     """Unsplit interior in the batch (vti_batch_set_unsplit): each shot's
     image and illumination equal the single-shot RTM with vti_set_unsplit
     bit for bit; against the FP64 oracle RTM the unsplit image is as close
@@ -105,7 +108,7 @@ def test_batch_rtm_unsplit_equals_single_shot_unsplit(graph):
     with vti.Batch(cfgs, shots, models) as B:
         o = vti.rtm_defaults(2)
         o.nsnap = 3
-        B.config(0, 8, graph)
+        B.config(shape, 8, graph)
         B.set_unsplit(True)
         B.rtm(o, obs)
         imgs = [B.image(s) for s in range(B.n)]
