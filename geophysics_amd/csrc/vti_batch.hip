@@ -43,6 +43,10 @@ struct BShot {
     int nrec, rz, rx0, rdx, inj_z;
     float *img, *illum; // interior nx*nz
     int tile0, ntz, ntx;
+    // checkpoint copies of the unsplit interior: first grid row of the shot
+    // in the batch's row numbering; rows [xlo, xhi) x columns [zlo, zhi) are
+    // the cells whose stress tables are all 1 (no split fields kept there)
+    int row0, xlo, xhi, zlo, zhi;
 };
 
 __device__ __forceinline__ float cst(int m)
@@ -267,6 +271,45 @@ __global__ void bzero(float4 *p, size_t n4)
         p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// grid row -> shot by binary search over row0
+__device__ __forceinline__ int find_row(const BShot *S, int ns, int r)
+{
+    int lo = 0, hi = ns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (S[mid].row0 <= r) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Checkpoint copy of the source states under the unsplit interior (store:
+// states -> slot, else slot -> states; the slot has the states' layout):
+// u, w, P and Q whole, the split fields px, pz, qx, qz only outside
+// [xlo, xhi) x [zlo, zhi), the cells that never read or write them.  One
+// workgroup per grid row of every shot, 16-byte pieces (rows are ldz floats,
+// a multiple of 32).
+__global__ void bckpt(const BShot *__restrict__ S, int ns, int nrows, const float *__restrict__ base,
+                      float *__restrict__ slot, int store)
+{
+    for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
+        const BShot &b = S[find_row(S, ns, r)];
+        const int ix = r - b.row0;
+        const bool mid = ix >= b.xlo && ix < b.xhi;
+        const int n4 = b.ldz / 4;
+        for (int k = 0; k < 8; ++k) {
+            const size_t o = (size_t)(b.f[k] - base) + (size_t)ix * b.ldz;
+            const float4 *src = reinterpret_cast<const float4 *>(store ? base + o : slot + o);
+            float4 *dst = reinterpret_cast<float4 *>(store ? slot + o : const_cast<float *>(base) + o);
+            const bool part = mid && k >= 2 && k <= 5;
+            for (int i = threadIdx.x; i < n4; i += blockDim.x) {
+                if (part && 4 * i >= b.zlo && 4 * i + 4 <= b.zhi) continue;
+                dst[i] = src[i];
+            }
+        }
+    }
+}
+
 // Tile shapes the autotuner chooses from: tz x tx threads, each thread
 // marching xs rows (the tile covers tz x tx*xs cells).
 struct Shape {
@@ -303,6 +346,7 @@ struct vti_batch {
     long long cells = 0;            // updated cells per step, all shots
     int shape = 0, grid_mult = 0;   // autotuned
     int ntile = 0;
+    int nrows = 0;                  // grid rows of all shots (checkpoint copies)
     std::vector<float *> slots;
     std::vector<float> vp1, eps1;   // padded model at (ix 0, iz 1): the direct-wave mute's velocity
     int graph_k = 0;
@@ -550,6 +594,24 @@ int vti_batch_create(int nshot, const vti_cfg *cfgs, const vti_shot *shots, cons
         const int M = b.M;
         axis_tables(c.nx, c.npml, dtd0, 0, &htab[0 * M], &htab[1 * M], &htab[6 * M], &htab[7 * M]);
         axis_tables(c.nz, c.npml, dtd0, 0, &htab[4 * M], &htab[5 * M], &htab[10 * M], &htab[11 * M]);
+        // the undamped range of each axis (the stress tables exactly 1), as
+        // the stress kernels test it; not one interval: every cell counts as
+        // damped (whole-row checkpoint copies)
+        auto undamped = [&](int t1, int t2, int nn, int &lo, int &hi) {
+            lo = hi = 0;
+            int i = 0;
+            auto one = [&](int j) { return htab[t1 * M + j] == 1.0f && htab[t2 * M + j] == 1.0f; };
+            while (i < nn && !one(i)) ++i;
+            lo = i;
+            while (i < nn && one(i)) ++i;
+            hi = i;
+            for (; i < nn; ++i)
+                if (one(i)) { lo = hi = 0; return; }
+        };
+        undamped(6, 7, b.nnx, b.xlo, b.xhi);
+        undamped(10, 11, b.nnz, b.zlo, b.zhi);
+        b.row0 = B->nrows;
+        B->nrows += b.nnx;
         if (hipMemcpy(tab, htab.data(), htab.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
             return bfail(VTI_ERR_HIP, "tables");
         float *stage = nullptr;
@@ -729,11 +791,22 @@ int vti_batch_rtm(vti_batch *B, const vti_rtm_opts *o, const float *const *obs, 
         return ok(hipGetLastError(), "zero");
     };
     std::vector<int> slot_it(nslot, 0);
-    op.store = [&](int i) { slot_it[i] = cur_it; return ok(hipMemcpyAsync(B->slots[i], B->state_base, sb, hipMemcpyDeviceToDevice, B->stream), "store"); };
+    // checkpoints: whole states, or (unsplit interior) all but the split
+    // fields of the cells that never use them
+    auto ckpt = [&](int i, int store) {
+        if (!B->unsplit)
+            return ok(store ? hipMemcpyAsync(B->slots[i], B->state_base, sb, hipMemcpyDeviceToDevice, B->stream)
+                            : hipMemcpyAsync(B->state_base, B->slots[i], sb, hipMemcpyDeviceToDevice, B->stream),
+                      "checkpoint");
+        hipLaunchKernelGGL(bckpt, dim3((unsigned)std::min(B->nrows, 8192)), dim3(256), 0, B->stream, B->db, B->nshot,
+                           B->nrows, B->state_base, B->slots[i], store);
+        return ok(hipGetLastError(), "checkpoint");
+    };
+    op.store = [&](int i) { slot_it[i] = cur_it; return ckpt(i, 1); };
     op.restore = [&](int i) {
         cur_it = slot_it[i];
         hipLaunchKernelGGL(bset_it, dim3(1), dim3(64), 0, B->stream, B->itp, cur_it);
-        return ok(hipMemcpyAsync(B->state_base, B->slots[i], sb, hipMemcpyDeviceToDevice, B->stream), "restore");
+        return ckpt(i, 0);
     };
     op.step = [&](int it) {
         if (it != cur_it) return fail(VTI_ERR_STATE, "batch: step index");

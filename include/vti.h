@@ -200,13 +200,20 @@ typedef struct vti_rtm_opts {
                           divide (2D ref :654-677, :700-715); 1: source +
                           receiver energy, divide where nonzero (3D ref)       */
     int image_every;   /* cross-correlate every k-th step                      */
-    int nsnap;         /* full-state checkpoint slots; 0 = one per step        */
+    int nsnap;         /* checkpoint slots of the source state; 0 = one per
+                          step (vti_rtm_checkpoint_bytes each)                 */
 } vti_rtm_opts;
 int vti_rtm_defaults(vti_rtm_opts *opts, int ndim);
 /* Zero the accumulated image and illumination (interior nx*ny*nz). */
 int vti_rtm_reset(vti_ctx *ctx);
+/* Device bytes of one checkpoint slot of vti_rtm_shot under the current
+ * settings: a full state, or, with the reduced split of vti_set_unsplit,
+ * u, v, w, P, Q whole and the split fields of the absorbing layer's cells
+ * only (the interior never reads them); 0 for a NULL context. */
+long long vti_rtm_checkpoint_bytes(vti_ctx *ctx);
 /* One shot; obs = observed record [irec][it] in host memory, or NULL to
- * migrate the modelled (muted) record.  *fwd_steps = forward steps taken. */
+ * migrate the modelled (muted) record.  *fwd_steps = forward steps taken.
+ * A checkpointed state is imaged from its slot (not restored first). */
 int vti_rtm_shot(vti_ctx *ctx, const vti_rtm_opts *opts, const float *obs,
                  long long *fwd_steps);
 /* Image and illumination to host memory; finalize = divide by the
