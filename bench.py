@@ -515,7 +515,7 @@ def learned_rtm(args, torch, vti, nshot, nt, n, rows_out=None):
     A = vti.Autoencoder(crop=64, batch=4, seed=args.seed)
     nsave = (nt + 3) // 4
     max_exact = max(1, int(nsave * args.exact_frac))   # exact-storage fallback budget (latent codec)
-    state_bytes = prop.state_bytes()                   # one checkpoint slot of the plain path
+    state_bytes = prop.checkpoint_bytes()              # one checkpoint slot of the plain path
     fp32_bytes = nsave * n ** 3 * 4
     rows = []
     t_fit = 0.0

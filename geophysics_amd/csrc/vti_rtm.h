@@ -18,6 +18,9 @@ struct SweepOps {
     std::function<int(int)> first_step;   // the same in the first sweep (record, illumination)
     std::function<int()> after_first;     // between the first sweep and the first consume
     std::function<int(int)> consume;      // working state holds X_k, k = nt .. 1
+    // optional: X_k read from checkpoint slot i without restoring it (a
+    // checkpointed state is consumed once, then its slot is freed)
+    std::function<int(int, int)> consume_slot;
 };
 
 // Runs the first sweep, then calls consume(k) for k = nt, nt-1, ..., 1 in

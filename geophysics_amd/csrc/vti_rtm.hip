@@ -146,9 +146,171 @@ using namespace vti;
 
 namespace {
 
+// ---------------------------------------------------- checkpoint slots ----
+// With the point-source form's reduced split in effect (vti_set_unsplit), a
+// cell whose stress damping factors are all exactly 1 never reads or writes
+// its split fields (px .. qz); the other cells (the absorbing layer) do.  A
+// checkpoint of the source state then keeps u, v, w, P, Q as whole volumes
+// and the split fields only of those cells: the whole z row of every (y, x)
+// outside the undamped [ylo, yhi) x [xlo, xhi), the z ends [0, zlo) and
+// [zhi, nnz) of the others, packed row by row.  384^3 with a 30-cell layer:
+// 2.5 instead of 3.9 GB per slot.  Otherwise a slot is a full state.
+struct CkGeom {
+    int nnx, nny, nnz;
+    long long ldz, ldxy;
+    int ylo, yhi, xlo, xhi, zlo, zhi;
+    long long split_n;   // floats of one packed split field
+};
+struct SplitPtrs {
+    float *f[6];
+    int n;
+};
+
+__device__ __forceinline__ long long ck_row_offset(const CkGeom &q, int y, int x, bool &full)
+{
+    const bool yin = y >= q.ylo && y < q.yhi, xin = x >= q.xlo && x < q.xhi;
+    full = !(yin && xin);
+    const long long r = (long long)y * q.nnx + x;
+    const long long fp = (long long)min(y, q.ylo) + max(0, y - q.yhi);   // whole-row planes before y
+    const long long full_before = fp * q.nnx + ((long long)y - fp) * (q.nnx - (q.xhi - q.xlo)) +
+                                  (yin ? (long long)min(x, q.xlo) + max(0, x - q.xhi) : (long long)x);
+    return full_before * q.nnz + (r - full_before) * (q.zlo + (q.nnz - q.zhi));
+}
+
+// Packs (store) or unpacks the split fields' layer cells; one wavefront per
+// (y, x) row.
+__global__ __launch_bounds__(256) void ckpt_split_kernel(CkGeom q, SplitPtrs sp, float *packed, int store)
+{
+    const int lane = threadIdx.x & 63;
+    const long long nrows = (long long)q.nny * q.nnx;
+    for (long long r = blockIdx.x * 4LL + (threadIdx.x >> 6); r < nrows; r += gridDim.x * 4LL) {
+        const int y = (int)(r / q.nnx), x = (int)(r % q.nnx);
+        bool full;
+        const long long po = ck_row_offset(q, y, x, full);
+        const long long ro = (long long)y * q.ldxy + (long long)x * q.ldz;
+        const int len = full ? q.nnz : q.zlo + (q.nnz - q.zhi);
+        for (int k = 0; k < sp.n; ++k) {
+            float *f = sp.f[k] + ro;
+            float *pk = packed + (long long)k * q.split_n + po;
+            for (int i = lane; i < len; i += 64) {
+                const int z = full || i < q.zlo ? i : q.zhi + (i - q.zlo);
+                if (store) pk[i] = f[z];
+                else f[z] = pk[i];
+            }
+        }
+    }
+}
+
+const int kBaseF[5] = {VTI_F_U, VTI_F_V, VTI_F_W, VTI_F_P, VTI_F_Q};
+const int kSplitF[6] = {VTI_F_PX, VTI_F_PY, VTI_F_PZ, VTI_F_QX, VTI_F_QY, VTI_F_QZ};
+
+struct CkSlots {
+    bool compact = false;
+    CkGeom q{};
+    int nbase = 0, nsplit = 0;
+    long long slot_floats = 0;
+    std::vector<float *> mem;      // one allocation per slot
+    std::vector<VtiState> view;    // whole-volume fields of each slot (split fields: full slots only)
+    ~CkSlots()
+    {
+        for (float *p : mem) (void)hipFree(p);
+    }
+};
+
+// The layout vti_rtm_shot uses for the context's current settings.
+int ck_layout(vti_ctx *c, CkSlots &k)
+{
+    k = CkSlots{};
+    const int nf = state_nfield(c);
+    k.slot_floats = (long long)nf * (long long)c->nvol;
+    if (!(c->unsplit && c->cfg.mode == VTI_MODE_GPU_REF) || c->tti) return VTI_OK;
+    const int M = c->nnmax;
+    std::vector<float> t(12 * (size_t)M);
+    if (hipMemcpy(t.data(), c->tab, t.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(VTI_ERR_HIP, "checkpoint layout: tables");
+    // the undamped interval of an axis (stress tables exactly 1); no single
+    // interval: no compact slots
+    auto undamped = [&](int t1, int n, int &lo, int &hi) {
+        auto one = [&](int j) { return t[t1 * M + j] == 1.0f && t[(t1 + 1) * M + j] == 1.0f; };
+        int i = 0;
+        while (i < n && !one(i)) ++i;
+        lo = i;
+        while (i < n && one(i)) ++i;
+        hi = i;
+        for (; i < n; ++i)
+            if (one(i)) return false;
+        return true;
+    };
+    CkGeom &q = k.q;
+    q.nnx = c->g.nnx;
+    q.nny = c->cfg.ndim == 3 ? c->g.nny : 1;
+    q.nnz = c->g.nnz;
+    q.ldz = c->g.ldz;
+    q.ldxy = c->g.ldxy;
+    if (!undamped(6, q.nnx, q.xlo, q.xhi) || !undamped(10, q.nnz, q.zlo, q.zhi)) return VTI_OK;
+    if (c->cfg.ndim == 3) {
+        if (!undamped(8, q.nny, q.ylo, q.yhi)) return VTI_OK;
+    } else {
+        q.ylo = 0, q.yhi = 1;
+    }
+    const long long mid = (long long)(q.yhi - q.ylo) * (q.xhi - q.xlo);
+    const long long rows = (long long)q.nny * q.nnx;
+    q.split_n = (rows - mid) * q.nnz + mid * (q.zlo + (q.nnz - q.zhi));
+    for (int f : kBaseF) k.nbase += c->f[f] ? 1 : 0;
+    for (int f : kSplitF) k.nsplit += c->f[f] ? 1 : 0;
+    k.compact = true;
+    k.slot_floats = (long long)k.nbase * (long long)c->nvol + (long long)k.nsplit * q.split_n;
+    return VTI_OK;
+}
+
+int ck_alloc(vti_ctx *c, CkSlots &k, int nslot)
+{
+    k.mem.assign(nslot, nullptr);
+    k.view.assign(nslot, VtiState{});
+    for (int i = 0; i < nslot; ++i) {
+        if (hipMalloc(&k.mem[i], (size_t)k.slot_floats * sizeof(float)) != hipSuccess) {
+            k.mem[i] = nullptr;
+            return fail(VTI_ERR_ALLOC, "checkpoint slots");
+        }
+        if (!k.compact) {   // a full state, laid out as state_alloc lays it out
+            static const int order3[11] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10};
+            static const int order2[8] = {0, 2, 3, 5, 6, 8, 9, 10};
+            const int nf = state_nfield(c);
+            const int *ord = nf == 11 ? order3 : order2;
+            for (int j = 0; j < nf; ++j) k.view[i].f[ord[j]] = k.mem[i] + (size_t)j * c->nvol;
+        } else {
+            int j = 0;
+            for (int f : kBaseF)
+                if (c->f[f]) k.view[i].f[f] = k.mem[i] + (size_t)(j++) * c->nvol;
+        }
+    }
+    return VTI_OK;
+}
+
+// slot i <- W (store) or W <- slot i
+hipError_t ck_copy(vti_ctx *c, const CkSlots &k, int i, const VtiState &W, bool store)
+{
+    const VtiState &S = k.view[i];
+    if (!k.compact) return store ? state_copy(c, S, W) : state_copy(c, W, S);
+    for (int f : kBaseF)
+        if (S.f[f]) {
+            const hipError_t e = hipMemcpyAsync(store ? S.f[f] : W.f[f], store ? W.f[f] : S.f[f],
+                                                c->nvol * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
+            if (e != hipSuccess) return e;
+        }
+    SplitPtrs sp{};
+    for (int f : kSplitF)
+        if (W.f[f]) sp.f[sp.n++] = W.f[f];
+    float *packed = k.mem[i] + (size_t)k.nbase * c->nvol;
+    const long long rows = (long long)k.q.nny * k.q.nnx;
+    hipLaunchKernelGGL(ckpt_split_kernel, dim3((unsigned)std::min<long long>((rows + 3) / 4, 16384)), dim3(256), 0,
+                       c->stream, k.q, sp, packed, store ? 1 : 0);
+    return hipGetLastError();
+}
+
 struct RtmCache {
     VtiState g;                  // receiver wavefield
-    std::vector<VtiState> slots; // checkpoints
+    CkSlots ck;                  // checkpoints of the source state
 };
 
 }  // namespace
@@ -185,6 +347,14 @@ int vti_rtm_reset(vti_ctx *c)
     return VTI_OK;
 }
 
+long long vti_rtm_checkpoint_bytes(vti_ctx *c)
+{
+    if (!c) return 0;
+    CkSlots k;
+    if (ck_layout(c, k)) return 0;
+    return k.slot_floats * (long long)sizeof(float);
+}
+
 int vti_rtm_shot(vti_ctx *c, const vti_rtm_opts *o, const float *obs, long long *fwd_steps)
 {
     if (!c || !o) return fail(VTI_ERR_ARG, "vti_rtm_shot: null argument");
@@ -200,12 +370,10 @@ int vti_rtm_shot(vti_ctx *c, const vti_rtm_opts *o, const float *obs, long long 
     RtmCache rc;
     struct Guard {
         RtmCache &r;
-        ~Guard() { state_free(r.g); for (auto &s : r.slots) state_free(s); }
+        ~Guard() { state_free(r.g); }
     } guard{rc};
     if ((st = state_alloc(c, rc.g))) return st;
-    rc.slots.resize(nslot);
-    for (int i = 0; i < nslot; ++i)
-        if ((st = state_alloc(c, rc.slots[i]))) return st;
+    if ((st = ck_layout(c, rc.ck)) || (st = ck_alloc(c, rc.ck, nslot))) return st;
     const VtiState W = main_state(c);
     const int nd = c->cfg.ndim;
     const bool svf = o->sv_forward != 0 && c->cfg.sv != 0;
@@ -215,8 +383,8 @@ int vti_rtm_shot(vti_ctx *c, const vti_rtm_opts *o, const float *obs, long long 
 
     SweepOps op;
     op.zero = [&]() { return hip(state_zero(c, W), "zero"); };
-    op.store = [&](int i) { return hip(state_copy(c, rc.slots[i], W), "store"); };
-    op.restore = [&](int i) { return hip(state_copy(c, W, rc.slots[i]), "restore"); };
+    op.store = [&](int i) { return hip(ck_copy(c, rc.ck, i, W, true), "store"); };
+    op.restore = [&](int i) { return hip(ck_copy(c, rc.ck, i, W, false), "restore"); };
     op.step = [&](int it) {
         if (launch_vel(nd, vel_args_for(c, W, it, true), c->stream) != hipSuccess) return fail(VTI_ERR_HIP, "vel");
         return hip(launch_stress(nd, src_stress_args(c, W, it, svf, nullptr), c->stream), "stress");
@@ -242,7 +410,9 @@ int vti_rtm_shot(vti_ctx *c, const vti_rtm_opts *o, const float *obs, long long 
         }
         return VTI_OK;
     };
-    op.consume = [&](int k) {
+    // X_k of the source side: the working state, or a checkpoint slot
+    // imaged from where it lies (it is not needed again)
+    auto consume_from = [&](const VtiState &S, int k) {
         const int it = k - 1;
         if (launch_inject_rec(c, rc.g, c->rec + (size_t)it * nrec, inj_z) != hipSuccess)
             return fail(VTI_ERR_HIP, "inject");
@@ -254,13 +424,15 @@ int vti_rtm_shot(vti_ctx *c, const vti_rtm_opts *o, const float *obs, long long 
         if (o->illum_mode == 1) ra = with_illum(c, ra, 0);
         if (launch_stress(nd, ra, c->stream) != hipSuccess) return fail(VTI_ERR_HIP, "stress");
         if (it % o->image_every == 0) {
-            const int r = hip(launch_image(c, W.f[VTI_F_P], rc.g.f[VTI_F_P], c->img), "image");
+            const int r = hip(launch_image(c, S.f[VTI_F_P], rc.g.f[VTI_F_P], c->img), "image");
             if (r) return r;
             // Poynting_Adcigs (:1475-1477) on the same source and receiver states
-            return hip(launch_adcig(c, W, rc.g), "adcig");
+            return hip(launch_adcig(c, S, rc.g), "adcig");
         }
         return VTI_OK;
     };
+    op.consume = [&](int k) { return consume_from(W, k); };
+    op.consume_slot = [&](int k, int i) { return consume_from(rc.ck.view[i], k); };
     c->rtm_illum_mode = o->illum_mode;
     if ((st = reverse_sweep(nt, nslot, op, fwd_steps))) return st;
     return hip(hipStreamSynchronize(c->stream), "sync");

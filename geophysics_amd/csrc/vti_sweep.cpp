@@ -95,7 +95,11 @@ int reverse_sweep(int nt, int nslot, const SweepOps &op, long long *fwd_steps)
         todo.pop_back();
         int e;
         if (t.kind == 1) {
-            if ((e = restore(t.r)) || (e = consume(t.r))) return e;
+            if (op.consume_slot && first_done) {
+                if ((e = op.consume_slot(t.r, pos_slot[t.r]))) return e;
+            } else if ((e = restore(t.r)) || (e = consume(t.r))) {
+                return e;
+            }
             free_slots.push_back(pos_slot[t.r]);
             pos_slot[t.r] = -1;
             todo.push_back({0, t.l, t.r - 1, t.s});
@@ -132,48 +136,64 @@ extern "C" {
 
 // Runs the schedule on integer positions: the working "state" is the number
 // of steps taken from X_0, a slot holds a position.  Fails unless consume(k)
-// sees X_k for k = nt .. 1 in that order with at most nslot slots in use.
+// sees X_k for k = nt .. 1 in that order with at most nslot slots in use,
+// both with checkpointed states restored before they are consumed and
+// consumed from their slots (the same forward steps either way).
 int vti_sweep_check(int nt, int nslot, long long *fwd_steps, int *max_slots)
 {
     if (nt < 1 || nslot < 0) return vti::fail(VTI_ERR_ARG, "vti_sweep_check: nt >= 1 and nslot >= 0");
-    int work = -1, next = nt, used = 0, peak = 0;
-    std::vector<int> slot(nslot, -1);
-    bool first_done = false, bad = false;
-    vti::SweepOps op;
-    op.zero = [&]() { work = 0; return VTI_OK; };
-    op.store = [&](int i) {
-        if (i < 0 || i >= nslot) { bad = true; return VTI_ERR_STATE; }
-        if (slot[i] < 0) peak = std::max(peak, ++used);
-        slot[i] = work;
-        return VTI_OK;
-    };
-    op.restore = [&](int i) {
-        if (i < 0 || i >= nslot || slot[i] < 0) { bad = true; return VTI_ERR_STATE; }
-        work = slot[i];
-        return VTI_OK;
-    };
-    op.step = [&](int it) {
-        if (it != work || !first_done) { bad = true; return VTI_ERR_STATE; }
-        ++work;
-        return VTI_OK;
-    };
-    op.first_step = [&](int it) {
-        if (it != work || first_done) { bad = true; return VTI_ERR_STATE; }
-        ++work;
-        return VTI_OK;
-    };
-    op.after_first = [&]() {
-        if (work != nt) { bad = true; return VTI_ERR_STATE; }
-        first_done = true;
-        return VTI_OK;
-    };
-    op.consume = [&](int k) {
-        if (k != next || work != k) { bad = true; return VTI_ERR_STATE; }
-        --next;
-        return VTI_OK;
-    };
-    const int st = vti::reverse_sweep(nt, nslot, op, fwd_steps);
-    if (st || bad || next != 0) return vti::fail(VTI_ERR_STATE, "vti_sweep_check: schedule out of order");
+    long long steps[2] = {0, 0};
+    int peak = 0;
+    for (int from_slot = 0; from_slot < 2; ++from_slot) {
+        int work = -1, next = nt, used = 0;
+        peak = 0;
+        std::vector<int> slot(nslot, -1);
+        bool first_done = false, bad = false;
+        vti::SweepOps op;
+        op.zero = [&]() { work = 0; return VTI_OK; };
+        op.store = [&](int i) {
+            if (i < 0 || i >= nslot) { bad = true; return VTI_ERR_STATE; }
+            if (slot[i] < 0) peak = std::max(peak, ++used);
+            slot[i] = work;
+            return VTI_OK;
+        };
+        op.restore = [&](int i) {
+            if (i < 0 || i >= nslot || slot[i] < 0) { bad = true; return VTI_ERR_STATE; }
+            work = slot[i];
+            return VTI_OK;
+        };
+        op.step = [&](int it) {
+            if (it != work || !first_done) { bad = true; return VTI_ERR_STATE; }
+            ++work;
+            return VTI_OK;
+        };
+        op.first_step = [&](int it) {
+            if (it != work || first_done) { bad = true; return VTI_ERR_STATE; }
+            ++work;
+            return VTI_OK;
+        };
+        op.after_first = [&]() {
+            if (work != nt) { bad = true; return VTI_ERR_STATE; }
+            first_done = true;
+            return VTI_OK;
+        };
+        op.consume = [&](int k) {
+            if (k != next || work != k) { bad = true; return VTI_ERR_STATE; }
+            --next;
+            return VTI_OK;
+        };
+        if (from_slot)
+            op.consume_slot = [&](int k, int i) {
+                // the slot holds X_k; the working state is left as it is
+                if (k != next || i < 0 || i >= nslot || slot[i] != k) { bad = true; return VTI_ERR_STATE; }
+                --next;
+                return VTI_OK;
+            };
+        const int st = vti::reverse_sweep(nt, nslot, op, &steps[from_slot]);
+        if (st || bad || next != 0) return vti::fail(VTI_ERR_STATE, "vti_sweep_check: schedule out of order");
+    }
+    if (steps[0] != steps[1]) return vti::fail(VTI_ERR_STATE, "vti_sweep_check: slot consumes change the steps");
+    if (fwd_steps) *fwd_steps = steps[0];
     if (max_slots) *max_slots = peak;
     return VTI_OK;
 }

@@ -110,6 +110,7 @@ def _declare(L):
         "vti_set_unsplit": (C.c_int, [VP, C.c_int]),
         "vti_batch_set_unsplit": (C.c_int, [VP, C.c_int]),
         "vti_state_bytes": (C.c_longlong, [VP]),
+        "vti_rtm_checkpoint_bytes": (C.c_longlong, [VP]),
         "vti_save_state": (C.c_int, [VP, VP]),
         "vti_load_state": (C.c_int, [VP, VP]),
         "vti_image_accumulate": (C.c_int, [VP, VP, VP, VP, C.c_int]),
@@ -489,6 +490,10 @@ class Propagator:
 
     def state_bytes(self):
         return int(lib().vti_state_bytes(self.h))
+
+    def checkpoint_bytes(self):
+        """Bytes of one checkpoint slot of rtm_shot under the current settings."""
+        return int(lib().vti_rtm_checkpoint_bytes(self.h))
 
     def save_state(self, dst_ptr):
         """Copy the wavefield state to a device buffer of state_bytes() (waits)."""
