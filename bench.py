@@ -536,7 +536,7 @@ def learned_rtm(args, torch, vti, nshot, nt, n, rows_out=None):
         if sparse:
             lo.codec, lo.tol, lo.floor = (2 if args.codec == "quant" else 1), args.tol, args.floor
             lo.budget = int(fp32_bytes * args.store_frac)
-            lo.serial_codec = 1 if args.serial_codec else 0
+            lo.serial_codec = {"beside": 0, "between": 1, "decode-beside": 2}[args.codec_stream]
         else:
             lo.check_every, lo.tol, lo.ck_every, lo.max_exact = 1, 0.02, 0, max_exact
         ro = prop.rtm_defaults()
@@ -596,8 +596,7 @@ def learned_rtm(args, torch, vti, nshot, nt, n, rows_out=None):
         "workload": f"3D VTI RTM {n}^3, {nt} steps, dt 0.8 ms, 15 Hz, {ntrain} fitting shots then "
                     f"{len(rows)} shot(s) imaged both ways",
         "codec": args.codec, "block_tol": args.tol if sparse else None, "block_floor": args.floor if sparse else None,
-        "codec_stream": ("the shot's stream, between the steps" if args.serial_codec else
-                         "its own stream, from snapshot copies the steps' stress updates write") if sparse else None,
+        "codec_stream": args.codec_stream if sparse else None,
         "stress_update": "split" if args.split else "reduced split (both sides; the receiver injection row split)",
         "src_box": args.src_box,
         "plain_s_per_shot": round(tp, 3) if rows else None,
@@ -687,9 +686,9 @@ def main():
                     help="fd3d: shots of the learned-vs-plain record (3 fitting, the rest imaged both ways)")
     ap.add_argument("--no-eb", action="store_true",
                     help="learned RTM: leave out the effective-boundary (band reversal) leg")
-    ap.add_argument("--serial-codec", action="store_true",
-                    help="learned RTM: run the codec on the shot's stream between the steps instead of on its own "
-                         "stream beside them (the same images)")
+    ap.add_argument("--codec-stream", default="beside", choices=["beside", "between", "decode-beside"],
+                    help="learned RTM: the codec on its own stream beside the steps, on the shot's stream between "
+                         "them, or encodes between and decodes beside (the same images)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the library has no multi-GPU path "

@@ -377,13 +377,15 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
     // snapshot step's stress update, the copy's next writer, waits for that
     // work (4 steps earlier).  serial_codec puts the codec on the shot's
     // stream between the steps, reading P itself (the same results).
-    const bool side = sparse && !lo->serial_codec;
+    const bool side = sparse && lo->serial_codec == 0;                          // encodes
+    const bool side_dec = sparse && (lo->serial_codec == 0 || lo->serial_codec == 2);   // decodes
     CodecStream cs;
     if (sparse && !cs.ok) return fail(VTI_ERR_HIP, "vti_rtm_shot_learned: codec stream");
-    if (side && hipMalloc(&snapb.p, nn3 * sizeof(float)) != hipSuccess)
+    if ((side || side_dec) && hipMalloc(&snapb.p, nn3 * sizeof(float)) != hipSuccess)
         return fail(VTI_ERR_ALLOC, "vti_rtm_shot_learned: snapshot copy");
     float *SN = static_cast<float *>(snapb.p);
-    hipStream_t es = side ? cs.s : s;   // the codec's stream
+    hipStream_t es = side ? cs.s : s;        // the encodes' stream
+    hipStream_t ds = side_dec ? cs.s : s;    // the decodes' stream
     bc::Counters *hcs = cs.hc;
     int pend = -1;                      // snapshot whose encode has not been settled on the host
     long long pend_cap = 0;
@@ -533,7 +535,7 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         if (o->illum_mode == 1) ra = with_illum(c, ra, 0);   // receiver illumination in the epilogue
         const int j = it / se;
         const bool coded = sparse && it % se == 0 && exact[j] < 0 && !bad[j];
-        if (side && it % se == 0) {
+        if (side_dec && it % se == 0) {
             if (wait_img) {
                 if ((st = hip(hipStreamWaitEvent(s, cs.done[0], 0), "wait"))) return st;
                 wait_img = false;
@@ -559,24 +561,24 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         } else if (coded) {
             bc::TileSink sk;
             sk.img = c->img;
-            sk.rp = side ? SN : G.f[VTI_F_P];
-            sk.off = side ? 0 : off;
-            sk.sy = side ? (long long)n * n : c->g.ldxy;
-            sk.sx = side ? n : c->g.ldz;
+            sk.rp = side_dec ? SN : G.f[VTI_F_P];
+            sk.off = side_dec ? 0 : off;
+            sk.sy = side_dec ? (long long)n * n : c->g.ldxy;
+            sk.sx = side_dec ? n : c->g.ldz;
             if (box_n) {
                 sk.box = BX + (size_t)j * box_n;
                 sk.y0 = bx.y0, sk.x0 = bx.x0, sk.z0 = bx.z0, sk.by = bx.by, sk.bx = bx.bx, sk.bz = bx.bz;
             }
-            if (side && ((st = hip(hipEventRecord(cs.p_ready, s), "event")) ||
-                         (st = hip(hipStreamWaitEvent(es, cs.p_ready, 0), "wait"))))
+            if (side_dec && ((st = hip(hipEventRecord(cs.p_ready, s), "event")) ||
+                             (st = hip(hipStreamWaitEvent(ds, cs.p_ready, 0), "wait"))))
                 return st;
-            if ((st = hip(t_dec.begin(es), "event"))) return st;
-            st = quant ? ae_bq_decode_sink(A, ebase[j], BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, n, sk, es)
-                       : ae_bc_decode_sink(A, ebase[j], BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, n, sk, es);
+            if ((st = hip(t_dec.begin(ds), "event"))) return st;
+            st = quant ? ae_bq_decode_sink(A, ebase[j], BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, n, sk, ds)
+                       : ae_bc_decode_sink(A, ebase[j], BOFF + (size_t)j * nblk, BCNT + (size_t)j * nblk, n, sk, ds);
             if (st) return st;
-            if ((st = hip(t_dec.end(es), "event"))) return st;
-            if (side) {
-                if ((st = hip(hipEventRecord(cs.done[0], es), "event"))) return st;
+            if ((st = hip(t_dec.end(ds), "event"))) return st;
+            if (side_dec) {
+                if ((st = hip(hipEventRecord(cs.done[0], ds), "event"))) return st;
                 wait_img = true;
             }
         } else {   // latents: decoded into the dense buffer, then imaged
@@ -590,7 +592,8 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         if ((st = hip(hipGetLastError(), "image"))) return st;
     }
     c->rtm_illum_mode = o->illum_mode;
-    if ((st = hip(hipStreamSynchronize(s), "sync")) || (st = hip(hipStreamSynchronize(es), "sync"))) return st;
+    if ((st = hip(hipStreamSynchronize(s), "sync")) || (st = hip(hipStreamSynchronize(cs.s ? cs.s : s), "sync")))
+        return st;
     if (stats) {
         stats[0] = t_enc.total();
         stats[1] = t_dec.total();
@@ -610,7 +613,7 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         // latents' two dense n^3 buffers) and the fallback checkpoints
         const double ckb = (double)nck * (double)vti_state_bytes(c);
         stats[8] = sparse ? (double)alloc_ents * 4 + (double)nsave * nblk * 6 + (double)nsave * box_n * 4 +
-                                (double)ae_codec_work_bytes(A) + (side ? (double)nn3 * 4 : 0.0) + ckb
+                                (double)ae_codec_work_bytes(A) + (SN ? (double)nn3 * 4 : 0.0) + ckb
                           : (double)nsave * lat_n * 2 + (double)nsave * 64 + (double)lo->max_exact * nn3 * 4 +
                                 (double)nsave * box_n * 4 + 2.0 * nn3 * 4 + ckb;
     }

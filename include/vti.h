@@ -479,10 +479,11 @@ typedef struct {
     long long budget;  /* codecs 1, 2: bytes for the coded snapshots (0: nsnap * n^3, a quarter
                           of the fp32 snapshots); past it, snapshots are recomputed from the
                           ck_every checkpoints (or the call fails without them) */
-    int serial_codec;  /* codecs 1, 2: 0 runs the codec on a stream of its own, each encode
-                          overlapping the next velocity update and each decode the receiver
-                          steps before its image; 1 runs it on the shot's stream between the
-                          steps.  The results are the same. */
+    int serial_codec;  /* codecs 1, 2: where the codec runs (the results are the same):
+                          0: encodes and decodes on a stream of their own, from snapshot
+                          copies the steps' stress updates write, beside the steps;
+                          1: both on the shot's stream between the steps; 2: encodes
+                          between the steps, decodes beside them */
 } vti_learned_opts;
 int vti_learned_defaults(vti_learned_opts *o);
 /* stats (optional, 9 doubles): encode ms, decode ms (GPU time between

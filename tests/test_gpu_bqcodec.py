@@ -366,9 +366,10 @@ def test_quantized_rtm_image_and_store_against_sparse():
 
 def test_codec_stream_overlap_equals_serial():
     """The codec on a stream of its own, working from the copy of P that
-    each snapshot step's stress update writes while the steps go on, gives
-    the image, illumination and store of the codec run between the steps on
-    the shot's stream from P itself, bit for bit: codecs 1 and 2
+    each snapshot step's stress update writes while the steps go on (all of
+    it, or the decodes only), gives the image, illumination and store of the
+    codec run between the steps on the shot's stream from P itself, bit for
+    bit: codecs 1 and 2
     with room for every snapshot, and codec 2 with a budget that stores some
     snapshots and recomputes the rest from checkpoints (the reverse sweep
     mixing decoded and recomputed snapshots)."""
@@ -395,14 +396,15 @@ def test_codec_stream_overlap_equals_serial():
         words = run(2, full, 0, 0, 1)[2]["store_bytes"] - nsave * nblk * 6
         for codec, budget, ck, box in ((1, full, 0, 8), (2, full, 0, 8), (2, int(0.4 * words), 16, 0)):
             i1, l1, s1 = run(codec, budget, ck, box, 1)
-            i0, l0, s0 = run(codec, budget, ck, box, 0)
-            _say(f"codec {codec}, budget {budget}, box {box}: serial {s1}, overlapped {s0}")
             assert np.abs(i1).max() > 0
-            assert np.array_equal(i0, i1), rel_l2(i0, i1)
-            assert np.array_equal(l0, l1)
-            for k in keys:
-                assert s0[k] == s1[k], k
-            assert s0["alloc_bytes"] == s1["alloc_bytes"] + N ** 3 * 4   # the overlapped codec's snapshot copy
+            for mode in (0, 2):   # both beside the steps; encodes between, decodes beside
+                i0, l0, s0 = run(codec, budget, ck, box, mode)
+                _say(f"codec {codec}, budget {budget}, box {box}, mode {mode}: serial {s1}, other {s0}")
+                assert np.array_equal(i0, i1), (mode, rel_l2(i0, i1))
+                assert np.array_equal(l0, l1), mode
+                for k in keys:
+                    assert s0[k] == s1[k], (mode, k)
+                assert s0["alloc_bytes"] == s1["alloc_bytes"] + N ** 3 * 4   # the snapshot copy
             if budget == full:
                 assert s1["recomputed"] == 0
             else:
