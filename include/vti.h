@@ -486,9 +486,10 @@ typedef struct {
                           between the steps, decodes beside them */
 } vti_learned_opts;
 int vti_learned_defaults(vti_learned_opts *o);
-/* stats (optional, 9 doubles): encode ms, decode ms (GPU time between
- * HIP events around the codec's launches on its stream; the encode's up to
- * the copy of its counters to the host, which places the entries), mean
+/* stats (optional, 9 doubles): encode ms, decode ms (time between HIP
+ * events around the codec's launches on its stream, summed: beside the
+ * steps these spans include the GPU time the stencil takes meanwhile; the
+ * encode's up to the copy of its counters to the host), mean
  * checked relative error (codecs 1, 2: mean error bound of the stored
  * snapshots), snapshots above tol (codecs 1, 2: over budget), snapshots,
  * snapshots kept exactly, bytes of the compressed store (latents or
@@ -496,8 +497,9 @@ int vti_learned_defaults(vti_learned_opts *o);
  * recomputed from checkpoints, and the device bytes the call held for the
  * stored source field as allocated (the store, its sparse entries in 4 GB
  * chunks taken as needed up to the budget, source boxes, the work buffers:
- * the codec's coefficients and search results for codecs 1, 2, two dense
- * n^3 buffers for the latents; fallback checkpoints): the memory a plain
+ * the codec's coefficients and search results for codecs 1, 2 and, with
+ * the codec beside the steps, the n^3 snapshot copy; two dense n^3 buffers
+ * for the latents; fallback checkpoints): the memory a plain
  * checkpointed RTM would be given for a comparison at equal memory.  Codecs
  * 1, 2 decode each snapshot straight into the image. */
 int vti_rtm_shot_learned(vti_ctx *ctx, vti_ae *ae, const vti_rtm_opts *opts,
