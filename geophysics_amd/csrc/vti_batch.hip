@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <functional>
 #include <memory>
@@ -145,10 +146,12 @@ __global__ __launch_bounds__(TZ *TXB) void bvel(const BShot *__restrict__ S, int
 // directly where the cell's damping factors are all 1, the split fields
 // only in the absorbing layer and, on the receiver side, on the injection
 // row (as stress2d_kernel<true>).
+// src_off (MODE 2): floats from each shot's source P to the source P imaged
+// (0: the states themselves; a checkpoint slot's offset from the states)
 template <int TZ, int TXB, int XS, int MODE>
 __global__ __launch_bounds__(TZ *TXB) void bstress(const BShot *__restrict__ S, int ns, int ntile,
                                                   const int *__restrict__ itp, int sv_on, int image_every,
-                                                  int illum, int unsplit)
+                                                  int illum, int unsplit, long long src_off)
 {
     const int it = *itp;
     for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
@@ -229,7 +232,7 @@ __global__ __launch_bounds__(TZ *TXB) void bstress(const BShot *__restrict__ S, 
         if (MODE == 2 && it % image_every == 0) {
             const int jx = ix - b.np, jz = iz - b.np;
             if (jx >= 0 && jx < b.nx && jz >= 0 && jz < b.nz)
-                b.img[jz + (long long)jx * b.nz] += b.f[6][id] * Pn;
+                b.img[jz + (long long)jx * b.nz] += b.f[6][id + src_off] * Pn;
         }
         if (XS > 1) {
 #pragma unroll
@@ -347,6 +350,7 @@ struct vti_batch {
     int shape = 0, grid_mult = 0;   // autotuned
     int ntile = 0;
     int nrows = 0;                  // grid rows of all shots (checkpoint copies)
+    long long src_off = 0;          // receiver steps: where the imaged source P lies (see bstress)
     std::vector<float *> slots;
     std::vector<float> vp1, eps1;   // padded model at (ix 0, iz 1): the direct-wave mute's velocity
     int graph_k = 0;
@@ -378,11 +382,11 @@ void launch_phase(vti_batch *B, int phase, int sv_on, int grid)
     const int us = B->unsplit ? 1 : 0;
     switch (phase) {
     case 0: hipLaunchKernelGGL((bvel<TZ, TXB, XS, 0>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, 1); break;
-    case 1: hipLaunchKernelGGL((bstress<TZ, TXB, XS, 0>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 0, us); break;
-    case 2: hipLaunchKernelGGL((bstress<TZ, TXB, XS, 1>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 0, us); break;
+    case 1: hipLaunchKernelGGL((bstress<TZ, TXB, XS, 0>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 0, us, 0LL); break;
+    case 2: hipLaunchKernelGGL((bstress<TZ, TXB, XS, 1>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 0, us, 0LL); break;
     case 3: hipLaunchKernelGGL((bvel<TZ, TXB, XS, 1>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, 0); break;
-    case 4: hipLaunchKernelGGL((bstress<TZ, TXB, XS, 2>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 0, us); break;
-    case 5: hipLaunchKernelGGL((bstress<TZ, TXB, XS, 1>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 2, us); break;
+    case 4: hipLaunchKernelGGL((bstress<TZ, TXB, XS, 2>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 0, us, B->src_off); break;
+    case 5: hipLaunchKernelGGL((bstress<TZ, TXB, XS, 1>), dim3(grid), blk, 0, B->stream, B->db, B->nshot, B->ntile, B->itp, sv_on, B->image_every, 2, us, 0LL); break;
     }
 }
 
@@ -854,6 +858,15 @@ int vti_batch_rtm(vti_batch *B, const vti_rtm_opts *o, const float *const *obs, 
         B->itp = save + 1;
         int st = run_steps(B, 2, 1);
         B->itp = save;
+        return st;
+    };
+    // a checkpointed state imaged from its slot (the slot has the states'
+    // layout and holds P whole in both checkpoint forms)
+    op.consume_slot = [&](int k, int i) {
+        B->src_off = ((long long)reinterpret_cast<intptr_t>(B->slots[i]) -
+                      (long long)reinterpret_cast<intptr_t>(B->state_base)) / (long long)sizeof(float);
+        const int st = op.consume(k);
+        B->src_off = 0;
         return st;
     };
     // graphs capture the device-index pointer in effect: keep separate graphs

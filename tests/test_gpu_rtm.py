@@ -102,6 +102,7 @@ def test_rtm_3d_matches_oracle_bitwise():
         o = prop.rtm_defaults()
         o.nsnap = 5
         o.inject_z = npml - 1               # 3D reference injects at npml-1
+        assert prop.checkpoint_bytes() == prop.state_bytes()   # split form: whole-state slots
         prop.rtm_reset()
         prop.rtm_shot(o, obs)
         img, ill = prop.rtm_image(finalize=False)
@@ -132,6 +133,10 @@ def test_rtm_3d_unsplit_interior():
     rng = np.random.default_rng(2)
     with vti.Propagator(cfg, vp, e, d) as prop:
         prop.set_unsplit(True)
+        # trimmed slots: u, v, w, P, Q whole (5 of the 11 volumes) and the
+        # split fields of the absorbing layer only (most of this small grid)
+        ck, full = prop.checkpoint_bytes(), prop.state_bytes()
+        assert 5 * full // 11 < ck < full
         sh = vti.shot_ref3d(cfg, 0, 14, 0, 13, 0, 1, 0, rec_depth=1, rec_step=2)
         prop.set_shot(sh)
         obs = (rng.standard_normal((prop.nrec, nt)) * 0.05).astype(np.float32)
