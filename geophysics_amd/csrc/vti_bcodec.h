@@ -143,6 +143,10 @@ __device__ __forceinline__ void tile_store(uint16_t *lds, const tile_f4 acc[4][4
     const long long nblk = (long long)nb * nb * nb;
     float *const dst = sk.img ? sk.img : sk.out;
     const bool vec = (reinterpret_cast<uintptr_t>(dst) & 15) == 0;
+    // the receiver field in 16-byte pieces too when every z line's start is
+    // 16-byte aligned (the dense snapshot copy; not a padded field)
+    const bool rvec = sk.img && (reinterpret_cast<uintptr_t>(sk.rp + sk.off) & 15) == 0 && (sk.sy & 3) == 0 &&
+                      (sk.sx & 3) == 0;
     for (int it = threadIdx.x; it < kTileBlocks * 128; it += 64 * kTileWaves) {
         const int half = it & 1, blk = (it >> 1) & (kTileBlocks - 1), line = it >> 7;
         const long long b = b0 + blk;
@@ -159,9 +163,10 @@ __device__ __forceinline__ void tile_store(uint16_t *lds, const tile_f4 acc[4][4
                     if (z + u >= sk.z0 && z + u < sk.z0 + sk.bz) v[u] = bl[z + u];
             }
             const float *rp = sk.rp + sk.off + y * sk.sy + x * sk.sx + z;
+            const tile_f4 g = rvec ? *reinterpret_cast<const tile_f4 *>(rp) : tile_f4{rp[0], rp[1], rp[2], rp[3]};
             tile_f4 m = vec ? *reinterpret_cast<const tile_f4 *>(q) : tile_f4{q[0], q[1], q[2], q[3]};
 #pragma unroll
-            for (int u = 0; u < 4; ++u) m[u] = m[u] + v[u] * rp[u];
+            for (int u = 0; u < 4; ++u) m[u] = m[u] + v[u] * g[u];
             v = m;
         }
         if (vec) {

@@ -224,19 +224,20 @@ def test_decode_into_unaligned_output():
     assert np.array_equal(got[1:1 + n ** 3].reshape(n, n, n), out)
 
 
-@pytest.mark.parametrize("codec", [1, 2])
-def test_decode_into_image_equals_decode_then_image(codec):
+@pytest.mark.parametrize("codec,dense", [(1, False), (2, False), (2, True)])
+def test_decode_into_image_equals_decode_then_image(codec, dense):
     """vti_codec_decode_image (the RTM's fused decode and imaging): img +
     v * rp bit for bit as numpy's float32 arithmetic on the dense decode, v
     the box's values inside the exactly kept box (partly outside the volume
     on no side, cut by block boundaries), for an aligned and a misaligned
-    image; the receiver field strided inside a larger array."""
+    image; the receiver field strided inside a larger array (single loads),
+    or a dense n^3 volume (16-byte loads)."""
     import torch
     dev = torch.device("cuda:0")
     n = 24
-    rng = np.random.default_rng(11 + codec)
+    rng = np.random.default_rng(11 + codec + 5 * dense)
     big = rng.standard_normal((n + 3, n + 2, n + 5)).astype(np.float32)
-    rp = rng.standard_normal((n + 4, n + 6, n + 7)).astype(np.float32)
+    rp = rng.standard_normal((n, n, n) if dense else (n + 4, n + 6, n + 7)).astype(np.float32)
     img0 = rng.standard_normal(n ** 3).astype(np.float32)
     geom = (5, 3, 9, 10, 13, 7)                    # y0, x0, z0, by, bx, bz
     box = rng.standard_normal(geom[3:]).astype(np.float32)
@@ -259,7 +260,8 @@ def test_decode_into_image_equals_decode_then_image(codec):
                                                      out.data_ptr())
         rpd = torch.from_numpy(rp).to(dev)
         boxd = torch.from_numpy(box).to(dev)
-        roff = 4 + 1 * rp.shape[2] + 2 * rp.shape[1] * rp.shape[2]
+        ry, rx, rz = (0, 0, 0) if dense else (2, 1, 4)
+        roff = rz + rx * rp.shape[2] + ry * rp.shape[1] * rp.shape[2]
         got = []
         for shift in (0, 1):                       # image 16-byte aligned, then one float past it
             buf = torch.from_numpy(np.concatenate([np.full(shift, 7.0, np.float32), img0,
@@ -274,7 +276,7 @@ def test_decode_into_image_equals_decode_then_image(codec):
         v = out.cpu().numpy().reshape(n, n, n).copy()
     y0, x0, z0, by, bx, bz = geom
     v[y0:y0 + by, x0:x0 + bx, z0:z0 + bz] = box
-    r = rp[2:2 + n, 1:1 + n, 4:4 + n]
+    r = rp[ry:ry + n, rx:rx + n, rz:rz + n]
     exp = (img0.reshape(n, n, n) + v * r).reshape(-1)
     assert np.abs(v).max() > 0
     for g in got:
