@@ -93,27 +93,41 @@ __global__ __launch_bounds__(64 * EW) void encode64_kernel(Basis V, const float 
             bx = (int)((b / nb) % nb);
             by = (int)(b / ((long long)nb * nb));
         }
-#pragma unroll 2
-        for (int i = 0; i < 8; ++i) {
-            const int line = w + 8 * i;
+        // two lines at a time, each line's loads issued before its conversion
+        auto stage = [&](int line, const float *v) {   // the line's hi / lo parts into LDS, its energy into e2
             uint16_t hh[8], ll[8];
 #pragma unroll
             for (int q = 0; q < 8; ++q) hh[q] = ll[q] = 0;
             if (live) {
                 const int y = 8 * by + (line >> 3), x = 8 * bx + (line & 7), z0 = 8 * bz;
-                const float *src = P + off + (long long)y * sy + (long long)x * sx + z0;
                 const bool yin = y >= m.y0 && y < m.y0 + m.by && x >= m.x0 && x < m.x0 + m.bx;
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
                     const bool masked = yin && z0 + q >= m.z0 && z0 + q < m.z0 + m.bz;
-                    const float v = masked ? 0.0f : src[q];
-                    hh[q] = to_bf(v);
-                    ll[q] = to_bf(v - from_bf(hh[q]));
-                    e2 += v * v;
+                    const float vv = masked ? 0.0f : v[q];
+                    hh[q] = to_bf(vv);
+                    ll[q] = to_bf(vv - from_bf(hh[q]));
+                    e2 += vv * vv;
                 }
             }
             *reinterpret_cast<us8 *>(&xh[l][line * 8]) = pack8(hh);
             *reinterpret_cast<us8 *>(&xl[l][line * 8]) = pack8(ll);
+        };
+        auto line_src = [&](int line) {
+            return P + off + (long long)(8 * by + (line >> 3)) * sy + (long long)(8 * bx + (line & 7)) * sx + 8 * bz;
+        };
+#pragma unroll 2
+        for (int i = 0; i < 8; ++i) {
+            float v[8];
+            if (live) {
+                const float *src = line_src(w + 8 * i);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) v[q] = src[q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) v[q] = 0.0f;
+            }
+            stage(w + 8 * i, v);
         }
         ep[w][l] = e2;
     }
