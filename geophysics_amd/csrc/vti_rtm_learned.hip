@@ -266,7 +266,7 @@ int vti_rtm_shot_learned(vti_ctx *c, vti_ae *A, const vti_rtm_opts *o, const vti
         return fail(VTI_ERR_ARG, "vti_rtm_shot_learned: 3D cubic interiors with a side divisible by 8 only");
     if (c->nrec <= 0) return fail(VTI_ERR_STATE, "vti_rtm_shot_learned: the shot has no receivers");
     if (lo->save_every < 1 || lo->check_every < 0 || lo->ck_every < 0 || lo->max_exact < 0 || lo->src_box < 0 ||
-        lo->codec < 0 || lo->codec > 2 || lo->budget < 0 ||
+        lo->codec < 0 || lo->codec > 2 || lo->budget < 0 || lo->serial_codec < 0 || lo->serial_codec > 2 ||
         (lo->codec >= 1 && (!(lo->tol >= 0.0f) || !(lo->floor >= 0.0f))))
         return fail(VTI_ERR_ARG, "vti_rtm_shot_learned: bad options");
     if (c->adcig_na > 0)

@@ -394,6 +394,8 @@ def test_codec_stream_overlap_equals_serial():
             return img, ill, st
 
         full = nsave * vti.Autoencoder.bq_max_words(N) * 4
+        with pytest.raises(vti.VtiError, match="bad options"):   # placements are 0, 1, 2
+            run(2, full, 0, 0, 3)
         # the words of codec 2 without a source box: the store less its 6-byte block tables
         words = run(2, full, 0, 0, 1)[2]["store_bytes"] - nsave * nblk * 6
         for codec, budget, ck, box in ((1, full, 0, 8), (2, full, 0, 8), (2, int(0.4 * words), 16, 0)):
